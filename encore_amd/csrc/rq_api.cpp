@@ -1,0 +1,360 @@
+// rq_api.cpp -- the C ABI of libencore_rq.so (declared in include/rq.h).
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+
+#include "rq_internal.h"
+
+namespace rq {
+
+namespace {
+thread_local std::string g_last_error;
+// Allocation fault injection for tests: the Nth dev_alloc from now fails
+// (0 = off).  Set through rq_debug_fail_alloc_after.
+std::atomic<int64_t> g_fail_after{0};
+}  // namespace
+
+void hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        throw Error(RQ_ERR_OOM, std::string(what) + ": " + hipGetErrorString(e));
+    throw Error(RQ_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void* dev_alloc(size_t bytes) {
+    int64_t f = g_fail_after.load();
+    if (f > 0) {
+        if (g_fail_after.fetch_sub(1) == 1) throw Error(RQ_ERR_OOM, "device allocation failed (injected)");
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(RQ_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    return p;
+}
+
+void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+static void set_error(const std::string& m) { g_last_error = m; }
+
+template <class F>
+static rq_status guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return RQ_OK;
+    } catch (const Error& e) {
+        set_error(e.what());
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return RQ_ERR_OOM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return RQ_ERR_DEVICE;
+    } catch (...) {
+        set_error("unknown error");
+        return RQ_ERR_DEVICE;
+    }
+}
+
+static void require(bool ok, const char* msg) {
+    if (!ok) throw Error(RQ_ERR_INVALID_ARG, msg);
+}
+
+// Per-stream scratch: [counter: 256 B][spill stacks].  Grows on demand.
+static uint32_t* stream_scratch(rq_device* dev, hipStream_t s, size_t spill_bytes) {
+    std::lock_guard<std::mutex> g(dev->mu);
+    DevBuf& b = dev->scratch[s];
+    const size_t need = 256 + spill_bytes;
+    if (b.bytes < need) {
+        // The old buffer may still be read by queued work on this stream.
+        RQ_HIP(hipStreamSynchronize(s));
+        b = DevBuf(need);
+    }
+    return b.as<uint32_t>();
+}
+
+static int grid_for(rq_device* dev, uint32_t accel, uint32_t pred, TraceKind kind, uint32_t n) {
+    const KernelConfig& kc = dev->kc;
+    int per_cu = kc.blocks_per_cu > 0 ? kc.blocks_per_cu : occupancy_blocks_per_cu(accel, pred, kind, kc);
+    int64_t grid = (int64_t)per_cu * dev->num_cus;
+    const int64_t need = ((int64_t)n + 255) / 256;
+    grid = std::max<int64_t>(1, std::min(grid, need));
+    return (int)grid;
+}
+
+static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, uint32_t* occ, TraceKind kind,
+                  hipStream_t s, unsigned long long* counts = nullptr) {
+    if (n == 0) return;
+    rq_device* dev = sc->dev;
+    RQ_HIP(hipSetDevice(dev->ordinal));
+    const int grid = grid_for(dev, sc->accel, sc->predicate, kind, n);
+    const uint32_t depth = sc->stats.max_depth + 2;
+    const uint32_t words = spill_entry_words(sc->accel);
+    const size_t spill_bytes = (size_t)grid * 4 * 64 * depth * words * sizeof(uint32_t);
+    uint32_t* scratch = stream_scratch(dev, s, spill_bytes);
+    TraceLaunch L{};
+    L.nodes = sc->nodes.ptr;
+    L.tris = sc->tris.as<TriRec>();
+    L.leaf_prims = sc->leaf_prims.as<uint32_t>();
+    L.rays = rays;
+    L.hits = hits;
+    L.occluded = occ;
+    L.n = n;
+    L.counter = scratch;
+    L.spill = scratch + 64;
+    L.spill_depth = depth;
+    for (int k = 0; k < 3; ++k) {
+        L.root_pos[k] = sc->root_pos[k];
+        L.root_size[k] = sc->root_size[k];
+    }
+    L.counts = counts;
+    launch_zero_u32(scratch, 1, s);
+    if (counts)
+        launch_count(L, sc->accel, sc->predicate, kind, grid, s);
+    else if (sc->accel == RQ_ACCEL_KD)
+        launch_kd(L, sc->predicate, kind, grid, dev->kc, s);
+    else
+        launch_bvh(L, sc->predicate, kind, grid, dev->kc, s);
+}
+
+// Model-list bounds for the kd root: given, or the vertex bounds.
+static void vertex_bounds(const float* xyz, uint32_t nverts, const rq_scene_desc& dd, float bmin[3], float bmax[3]) {
+    if (dd.has_bounds) {
+        for (int k = 0; k < 3; ++k) {
+            bmin[k] = dd.bounds_min[k];
+            bmax[k] = dd.bounds_max[k];
+        }
+        return;
+    }
+    for (int k = 0; k < 3; ++k) {
+        bmin[k] = INFINITY;
+        bmax[k] = -INFINITY;
+    }
+    for (uint32_t v = 0; v < nverts; ++v)
+        for (int k = 0; k < 3; ++k) {
+            bmin[k] = std::min(bmin[k], xyz[3 * (size_t)v + k]);
+            bmax[k] = std::max(bmax[k], xyz[3 * (size_t)v + k]);
+        }
+}
+
+}  // namespace rq
+
+struct rq_host_tree {
+    uint32_t accel = 0, ntris = 0;
+    rq::BvhHost bvh;
+    rq::KdHost kd;
+};
+
+using namespace rq;
+
+extern "C" {
+
+const char* rq_version(void) { return "encore-rq 0.1.0 gfx950"; }
+
+const char* rq_last_error(void) { return g_last_error.c_str(); }
+
+void rq_debug_fail_alloc_after(int64_t n) { g_fail_after.store(n); }
+
+rq_status rq_device_open(int ordinal, rq_device** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is null");
+        int count = 0;
+        RQ_HIP(hipGetDeviceCount(&count));
+        require(ordinal >= 0 && ordinal < count, "device ordinal out of range");
+        RQ_HIP(hipSetDevice(ordinal));
+        hipDeviceProp_t p;
+        RQ_HIP(hipGetDeviceProperties(&p, ordinal));
+        auto d = std::make_unique<rq_device>();
+        d->ordinal = ordinal;
+        d->num_cus = p.multiProcessorCount;
+        d->total_mem = p.totalGlobalMem;
+        d->arch = p.gcnArchName;
+        d->max_waves_per_cu = p.maxThreadsPerMultiProcessor / 64;
+        if (const char* e = std::getenv("RQ_STACK")) d->kc.stack = std::atoi(e);
+        if (const char* e = std::getenv("RQ_BLOCKS_PER_CU")) d->kc.blocks_per_cu = std::atoi(e);
+        if (d->kc.stack != 8 && d->kc.stack != 16 && d->kc.stack != 32) d->kc.stack = 16;
+        *out = d.release();
+    });
+}
+
+void rq_device_close(rq_device* dev) {
+    if (!dev) return;
+    (void)hipSetDevice(dev->ordinal);
+    (void)hipDeviceSynchronize();
+    delete dev;
+}
+
+rq_status rq_scene_create(rq_device* dev, const float* xyz, uint32_t nverts, const uint32_t* tri_idx,
+                          uint32_t ntris, const rq_scene_desc* desc, rq_scene** out) {
+    return guarded([&] {
+        require(dev && xyz && tri_idx && out, "null argument");
+        require(ntris > 0 && nverts > 0, "empty triangle list");
+        rq_scene_desc dd{};
+        if (desc) dd = *desc;
+        require(dd.accel == RQ_ACCEL_BVH || dd.accel == RQ_ACCEL_KD, "unknown accel kind");
+        require(dd.predicate == RQ_PRED_ENCORE || dd.predicate == RQ_PRED_WATERTIGHT, "unknown predicate");
+        RQ_HIP(hipSetDevice(dev->ordinal));
+        auto sc = std::make_unique<rq_scene>();
+        sc->dev = dev;
+        sc->accel = dd.accel;
+        sc->predicate = dd.predicate;
+        sc->ntris = ntris;
+        std::vector<TriRec> tris;
+        if (dd.accel == RQ_ACCEL_BVH) {
+            BvhHost h;
+            build_bvh(xyz, nverts, tri_idx, ntris, dd.leaf_size, h);
+            pack_triangles(xyz, tri_idx, h.tri_order.data(), ntris, dd.predicate, tris);
+            sc->stats = h.stats;
+            sc->nodes = DevBuf(h.nodes.size() * sizeof(BvhNode));
+            sc->nodes.upload(h.nodes.data(), sc->nodes.bytes);
+        } else {
+            float bmin[3], bmax[3];
+            vertex_bounds(xyz, nverts, dd, bmin, bmax);
+            KdHost h;
+            build_kd(xyz, nverts, tri_idx, ntris, bmin, bmax, h);
+            pack_triangles(xyz, tri_idx, nullptr, ntris, dd.predicate, tris);
+            sc->stats = h.stats;
+            for (int k = 0; k < 3; ++k) {
+                sc->root_pos[k] = h.root_pos[k];
+                sc->root_size[k] = h.root_size[k];
+            }
+            sc->nodes = DevBuf(h.nodes.size() * sizeof(KdNodeG));
+            sc->nodes.upload(h.nodes.data(), sc->nodes.bytes);
+            sc->leaf_prims = DevBuf(std::max<size_t>(1, h.leaf_prims.size()) * sizeof(uint32_t));
+            if (!h.leaf_prims.empty()) sc->leaf_prims.upload(h.leaf_prims.data(), h.leaf_prims.size() * 4);
+        }
+        sc->tris = DevBuf(tris.size() * sizeof(TriRec));
+        sc->tris.upload(tris.data(), sc->tris.bytes);
+        sc->device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->leaf_prims.bytes;
+        *out = sc.release();
+    });
+}
+
+void rq_scene_destroy(rq_scene* scene) {
+    if (!scene) return;
+    (void)hipSetDevice(scene->dev->ordinal);
+    delete scene;
+}
+
+rq_status rq_scene_stats(const rq_scene* sc, rq_scene_info* info) {
+    return guarded([&] {
+        require(sc && info, "null argument");
+        info->accel = sc->accel;
+        info->predicate = sc->predicate;
+        info->ntris = sc->ntris;
+        info->nnodes = sc->stats.nnodes;
+        info->nleaves = sc->stats.nleaves;
+        info->nrefs = sc->stats.nrefs;
+        info->max_depth = sc->stats.max_depth;
+        info->kd_max_depth_limit = sc->stats.kd_depth_limit;
+        info->device_bytes = sc->device_bytes;
+        info->build_seconds = sc->stats.seconds;
+    });
+}
+
+rq_status rq_intersect_closest(rq_scene* scene, const rq_ray* d_rays, uint32_t n, rq_hit* d_hits, void* stream) {
+    return guarded([&] {
+        require(scene != nullptr, "scene is null");
+        require(n == 0 || (d_rays && d_hits), "null ray or hit buffer");
+        trace(scene, d_rays, n, d_hits, nullptr, TraceKind::Closest, (hipStream_t)stream);
+    });
+}
+
+rq_status rq_occluded(rq_scene* scene, const rq_ray* d_rays, uint32_t n, uint32_t* d_occluded, void* stream) {
+    return guarded([&] {
+        require(scene != nullptr, "scene is null");
+        require(n == 0 || (d_rays && d_occluded), "null ray or output buffer");
+        trace(scene, d_rays, n, nullptr, d_occluded, TraceKind::Any, (hipStream_t)stream);
+    });
+}
+
+rq_status rq_trace_counts(rq_scene* scene, const rq_ray* d_rays, uint32_t n, uint32_t any_hit, uint64_t* counts,
+                          void* stream) {
+    return guarded([&] {
+        require(scene && counts, "null argument");
+        require(n == 0 || d_rays, "null ray buffer");
+        hipStream_t s = (hipStream_t)stream;
+        RQ_HIP(hipSetDevice(scene->dev->ordinal));
+        DevBuf out(n * (any_hit ? sizeof(uint32_t) : sizeof(rq_hit)) + 16);
+        DevBuf cnt(2 * sizeof(unsigned long long));
+        RQ_HIP(hipMemsetAsync(cnt.ptr, 0, cnt.bytes, s));
+        trace(scene, d_rays, n, any_hit ? nullptr : out.as<rq_hit>(), any_hit ? out.as<uint32_t>() : nullptr,
+              any_hit ? TraceKind::Any : TraceKind::Closest, s, cnt.as<unsigned long long>());
+        unsigned long long h[2] = {0, 0};
+        RQ_HIP(hipMemcpyAsync(h, cnt.ptr, sizeof(h), hipMemcpyDeviceToHost, s));
+        RQ_HIP(hipStreamSynchronize(s));
+        counts[0] = n;
+        counts[1] = h[0];
+        counts[2] = h[1];
+    });
+}
+
+rq_status rq_host_tree_build(const float* xyz, uint32_t nverts, const uint32_t* tri_idx, uint32_t ntris,
+                             const rq_scene_desc* desc, rq_host_tree** out) {
+    return guarded([&] {
+        require(xyz && tri_idx && out, "null argument");
+        require(ntris > 0 && nverts > 0, "empty triangle list");
+        rq_scene_desc dd{};
+        if (desc) dd = *desc;
+        require(dd.accel == RQ_ACCEL_BVH || dd.accel == RQ_ACCEL_KD, "unknown accel kind");
+        auto t = std::make_unique<rq_host_tree>();
+        t->accel = dd.accel;
+        t->ntris = ntris;
+        if (dd.accel == RQ_ACCEL_BVH) {
+            build_bvh(xyz, nverts, tri_idx, ntris, dd.leaf_size, t->bvh);
+        } else {
+            float bmin[3], bmax[3];
+            vertex_bounds(xyz, nverts, dd, bmin, bmax);
+            build_kd(xyz, nverts, tri_idx, ntris, bmin, bmax, t->kd);
+        }
+        *out = t.release();
+    });
+}
+
+rq_status rq_host_tree_info(const rq_host_tree* t, rq_scene_info* info) {
+    return guarded([&] {
+        require(t && info, "null argument");
+        const BuildStats& s = t->accel == RQ_ACCEL_BVH ? t->bvh.stats : t->kd.stats;
+        *info = rq_scene_info{};
+        info->accel = t->accel;
+        info->ntris = t->ntris;
+        info->nnodes = s.nnodes;
+        info->nleaves = s.nleaves;
+        info->nrefs = s.nrefs;
+        info->max_depth = s.max_depth;
+        info->kd_max_depth_limit = s.kd_depth_limit;
+        info->build_seconds = s.seconds;
+    });
+}
+
+rq_status rq_host_tree_copy(const rq_host_tree* t, void* nodes, uint32_t* leaf_prims, float* root_box) {
+    return guarded([&] {
+        require(t != nullptr, "null tree");
+        if (t->accel == RQ_ACCEL_BVH) {
+            if (nodes) std::memcpy(nodes, t->bvh.nodes.data(), t->bvh.nodes.size() * sizeof(BvhNode));
+            if (leaf_prims) std::memcpy(leaf_prims, t->bvh.tri_order.data(), t->bvh.tri_order.size() * 4);
+            if (root_box)
+                for (int k = 0; k < 6; ++k) root_box[k] = 0.0f;
+        } else {
+            if (nodes) std::memcpy(nodes, t->kd.nodes.data(), t->kd.nodes.size() * sizeof(KdNodeG));
+            if (leaf_prims) std::memcpy(leaf_prims, t->kd.leaf_prims.data(), t->kd.leaf_prims.size() * 4);
+            if (root_box)
+                for (int k = 0; k < 3; ++k) {
+                    root_box[k] = t->kd.root_pos[k];
+                    root_box[3 + k] = t->kd.root_size[k];
+                }
+        }
+    });
+}
+
+void rq_host_tree_destroy(rq_host_tree* t) { delete t; }
+
+}  // extern "C"

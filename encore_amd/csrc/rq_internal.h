@@ -1,0 +1,201 @@
+// rq_internal.h -- shared declarations of the ray-query engine (not public).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/rq.h"
+
+namespace rq {
+
+// ---------------------------------------------------------------------------
+// Errors: internal code throws rq::Error; the C ABI converts it to a status
+// and a thread-local message.
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+    rq_status status;
+    Error(rq_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+#define RQ_HIP(call) ::rq::hip_check((call), #call)
+
+// ---------------------------------------------------------------------------
+// Device memory: every engine allocation goes through dev_alloc so that an
+// allocation failure surfaces as RQ_ERR_OOM and can be injected in tests.
+// ---------------------------------------------------------------------------
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) : ptr(n ? dev_alloc(n) : nullptr), bytes(n) {}
+    ~DevBuf() { dev_free(ptr); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+        o.ptr = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            dev_free(ptr);
+            ptr = o.ptr;
+            bytes = o.bytes;
+            o.ptr = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+    void upload(const void* src, size_t n) { RQ_HIP(hipMemcpy(ptr, src, n, hipMemcpyHostToDevice)); }
+};
+
+// ---------------------------------------------------------------------------
+// Layouts in HBM
+// ---------------------------------------------------------------------------
+// BVH2 node, 64 bytes: the boxes of both children, interleaved so that one
+// float4 holds one axis of both (lo, hi) pairs, then the two child refs.
+//   x = (c0.lo.x, c0.hi.x, c1.lo.x, c1.hi.x), y = ..., z = ...
+//   ref: bit 31 clear -> inner node index; bit 31 set -> leaf with
+//        count-1 in bits 27..30 and the first triangle in bits 0..26.
+struct alignas(64) BvhNode {
+    float x[4], y[4], z[4];
+    uint32_t ref[2];
+    uint32_t pad[2];
+};
+static_assert(sizeof(BvhNode) == 64, "BVH node must be one 64-byte line");
+
+constexpr uint32_t kRefLeaf = 0x80000000u;
+constexpr uint32_t kRefSentinel = 0x7FFFFFFFu;
+constexpr uint32_t kLeafMaxCount = 16;
+constexpr uint32_t kLeafMaxFirst = (1u << 27) - 1;
+
+inline uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
+    return kRefLeaf | ((count - 1) << 27) | first;
+}
+
+// Triangle record, 48 bytes: three float4.  ENCORE predicate: (v0, prim),
+// (v1 - v0, 0), (v2 - v0, 0) -- the edges are rounded exactly as
+// TrianglePrim::intersect rounds them (TrianglePrim.cpp:210-211).
+// Watertight predicate: (v0, prim), (v1, 0), (v2, 0).
+struct alignas(16) TriRec {
+    float a[4], b[4], c[4];
+};
+static_assert(sizeof(TriRec) == 48, "triangle record is 48 bytes");
+
+// kd-tree node, 8 bytes, in ENCORE's numbering (root 0, children of an inner
+// node at child and child + 1): word = leaf ? (count << 2) | 3 :
+// (split bits & ~3) | axis (Kdtree.h:27-52); child = inner ? left child :
+// offset of the leaf's primitive list.
+struct KdNodeG {
+    uint32_t word;
+    uint32_t child;
+};
+
+struct BuildStats {
+    uint32_t nnodes = 0, nleaves = 0, nrefs = 0, max_depth = 0, kd_depth_limit = 0;
+    double seconds = 0;
+};
+
+// Host-side BVH build result.
+struct BvhHost {
+    std::vector<BvhNode> nodes;
+    std::vector<uint32_t> tri_order;  // leaf order -> original triangle index
+    BuildStats stats;
+};
+
+void build_bvh(const float* xyz, uint32_t nverts, const uint32_t* idx, uint32_t ntris, uint32_t leaf_size,
+               BvhHost& out);
+
+// Host-side kd build result (ENCORE's tree).
+struct KdHost {
+    std::vector<KdNodeG> nodes;
+    std::vector<uint32_t> leaf_prims;
+    float root_pos[3], root_size[3];
+    BuildStats stats;
+};
+
+void build_kd(const float* xyz, uint32_t nverts, const uint32_t* idx, uint32_t ntris, const float bmin[3],
+              const float bmax[3], KdHost& out);
+
+void pack_triangles(const float* xyz, const uint32_t* idx, const uint32_t* order, uint32_t ntris,
+                    uint32_t predicate, std::vector<TriRec>& out);
+
+// ---------------------------------------------------------------------------
+// Kernel launch parameters (by value).
+// ---------------------------------------------------------------------------
+struct TraceLaunch {
+    const void* nodes;       // BvhNode* or KdNodeG*
+    const TriRec* tris;
+    const uint32_t* leaf_prims;  // kd only
+    const rq_ray* rays;
+    rq_hit* hits;            // closest
+    uint32_t* occluded;      // any hit
+    uint32_t n;
+    uint32_t* counter;       // work counter, zeroed before launch
+    uint32_t* spill;         // per-lane spill stacks
+    uint32_t spill_depth;    // entries per lane
+    unsigned long long* counts;  // counting variant: [nodes visited, triangles tested]
+    float root_pos[3], root_size[3];  // kd only
+};
+
+enum class TraceKind { Closest = 0, Any = 1 };
+
+struct KernelConfig {
+    int block = 256;
+    int blocks_per_cu = 0;   // 0 = occupancy maximum
+    int stack = 16;          // LDS ring entries per lane
+    int fetch_threshold = 0; // dynamic fetch when fewer active lanes (0 = default)
+};
+
+// Launchers (traverse.hip).
+void launch_bvh(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, const KernelConfig& kc,
+                hipStream_t s);
+void launch_kd(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, const KernelConfig& kc,
+               hipStream_t s);
+void launch_count(const TraceLaunch& L, uint32_t accel, uint32_t predicate, TraceKind kind, int grid,
+                  hipStream_t s);
+void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
+int occupancy_blocks_per_cu(uint32_t accel, uint32_t predicate, TraceKind kind, const KernelConfig& kc);
+uint32_t spill_entry_words(uint32_t accel);
+
+}  // namespace rq
+
+// ---------------------------------------------------------------------------
+// Opaque public objects
+// ---------------------------------------------------------------------------
+struct rq_device {
+    int ordinal = 0;
+    int num_cus = 0;
+    int max_waves_per_cu = 0;
+    size_t total_mem = 0;
+    std::string arch;
+    std::mutex mu;
+    // Per-stream scratch: the work counter in the first 256 bytes, spill
+    // stacks after it.  Launches on one stream are ordered, so they can share.
+    std::map<hipStream_t, rq::DevBuf> scratch;
+    rq::KernelConfig kc;
+};
+
+struct rq_scene {
+    rq_device* dev = nullptr;
+    uint32_t accel = 0, predicate = 0, ntris = 0;
+    rq::DevBuf nodes, tris, leaf_prims;
+    float root_pos[3] = {0, 0, 0}, root_size[3] = {0, 0, 0};
+    rq::BuildStats stats;
+    uint64_t device_bytes = 0;
+};

@@ -1,0 +1,226 @@
+"""ctypes front of libencore_rq.so for tests and the benchmark.
+
+The engine itself is C++/HIP behind the C ABI in include/rq.h; this module
+only loads the library, converts status codes to exceptions and passes
+device pointers of torch tensors.  It raises if the library is not built:
+there is no fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libencore_rq.so")
+
+RQ_OK = 0
+STATUS = {0: "RQ_OK", 1: "RQ_ERR_INVALID_ARG", 2: "RQ_ERR_OOM", 3: "RQ_ERR_DEVICE", 4: "RQ_ERR_UNSUPPORTED",
+          5: "RQ_ERR_CAPACITY", 6: "RQ_ERR_TIMEOUT", 7: "RQ_ERR_CANCELLED"}
+ACCEL = {"bvh": 0, "kd": 1}
+PRED = {"encore": 0, "watertight": 1}
+NO_HIT = 0xFFFFFFFF
+
+
+class RQError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (STATUS.get(status, status), msg))
+        self.status = status
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [("accel", ctypes.c_uint32), ("predicate", ctypes.c_uint32), ("leaf_size", ctypes.c_uint32),
+                ("has_bounds", ctypes.c_uint32), ("bounds_min", ctypes.c_float * 3),
+                ("bounds_max", ctypes.c_float * 3)]
+
+
+class SceneInfo(ctypes.Structure):
+    _fields_ = [("accel", ctypes.c_uint32), ("predicate", ctypes.c_uint32), ("ntris", ctypes.c_uint32),
+                ("nnodes", ctypes.c_uint32), ("nleaves", ctypes.c_uint32), ("nrefs", ctypes.c_uint32),
+                ("max_depth", ctypes.c_uint32), ("kd_max_depth_limit", ctypes.c_uint32),
+                ("device_bytes", ctypes.c_uint64), ("build_seconds", ctypes.c_double)]
+
+
+_LIB = None
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+
+
+def load():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libencore_rq.so is not built (run __graft_entry__.build()); no fallback exists")
+    L = ctypes.CDLL(LIB_PATH)
+    L.rq_version.restype = ctypes.c_char_p
+    L.rq_last_error.restype = ctypes.c_char_p
+    L.rq_device_open.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+    L.rq_device_close.argtypes = [_vp]
+    L.rq_device_close.restype = None
+    L.rq_scene_create.argtypes = [_vp, _vp, _u32, _vp, _u32, ctypes.POINTER(SceneDesc), ctypes.POINTER(_vp)]
+    L.rq_scene_destroy.argtypes = [_vp]
+    L.rq_scene_destroy.restype = None
+    L.rq_scene_stats.argtypes = [_vp, ctypes.POINTER(SceneInfo)]
+    L.rq_intersect_closest.argtypes = [_vp, _vp, _u32, _vp, _vp]
+    L.rq_occluded.argtypes = [_vp, _vp, _u32, _vp, _vp]
+    L.rq_trace_counts.argtypes = [_vp, _vp, _u32, _u32, _vp, _vp]
+    L.rq_debug_fail_alloc_after.argtypes = [ctypes.c_int64]
+    L.rq_debug_fail_alloc_after.restype = None
+    L.rq_host_tree_build.argtypes = [_vp, _u32, _vp, _u32, ctypes.POINTER(SceneDesc), ctypes.POINTER(_vp)]
+    L.rq_host_tree_info.argtypes = [_vp, ctypes.POINTER(SceneInfo)]
+    L.rq_host_tree_copy.argtypes = [_vp, _vp, _vp, _vp]
+    L.rq_host_tree_destroy.argtypes = [_vp]
+    L.rq_host_tree_destroy.restype = None
+    for name in ("rq_device_open", "rq_scene_create", "rq_scene_stats", "rq_intersect_closest", "rq_occluded",
+                 "rq_host_tree_build", "rq_host_tree_info", "rq_host_tree_copy", "rq_trace_counts"):
+        getattr(L, name).restype = ctypes.c_int
+    _LIB = L
+    return L
+
+
+def check(status):
+    if status != RQ_OK:
+        raise RQError(status, load().rq_last_error().decode())
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or an int address)."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+class Device:
+    def __init__(self, ordinal=0):
+        L = load()
+        h = _vp()
+        check(L.rq_device_open(ordinal, ctypes.byref(h)))
+        self.h = h
+        self.ordinal = ordinal
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().rq_device_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Scene:
+    """An acceleration structure resident in HBM."""
+
+    def __init__(self, device, xyz, idx, accel="bvh", predicate="encore", leaf_size=0, bounds=None):
+        L = load()
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
+        idx = np.ascontiguousarray(idx, np.uint32).reshape(-1)
+        d = SceneDesc()
+        d.accel = ACCEL[accel]
+        d.predicate = PRED[predicate]
+        d.leaf_size = leaf_size
+        if bounds is not None:
+            d.has_bounds = 1
+            for k in range(3):
+                d.bounds_min[k] = float(bounds[0][k])
+                d.bounds_max[k] = float(bounds[1][k])
+        h = _vp()
+        check(L.rq_scene_create(device.h, xyz.ctypes.data, len(xyz) // 3, idx.ctypes.data, len(idx) // 3,
+                                ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        self.device = device
+        self.accel = accel
+        self.predicate = predicate
+
+    def destroy(self):
+        if getattr(self, "h", None):
+            load().rq_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.destroy()
+
+    def stats(self):
+        info = SceneInfo()
+        check(load().rq_scene_stats(self.h, ctypes.byref(info)))
+        return {f: getattr(info, f) for f, _ in SceneInfo._fields_}
+
+    def intersect(self, rays, hits, n=None, stream=None):
+        """rays: (n,8) float32 device tensor; hits: (n,4) float32 device tensor."""
+        n = rays.shape[0] if n is None else n
+        check(load().rq_intersect_closest(self.h, _ptr(rays), n, _ptr(hits), _stream(stream)))
+
+    def counts(self, rays, any_hit=False, n=None, stream=None):
+        """(rays, nodes visited, triangles tested) of one query (synchronous)."""
+        n = rays.shape[0] if n is None else n
+        c = np.zeros(3, np.uint64)
+        check(load().rq_trace_counts(self.h, _ptr(rays), n, int(any_hit), c.ctypes.data, _stream(stream)))
+        return int(c[0]), int(c[1]), int(c[2])
+
+    def occluded(self, rays, out, n=None, stream=None):
+        n = rays.shape[0] if n is None else n
+        check(load().rq_occluded(self.h, _ptr(rays), n, _ptr(out), _stream(stream)))
+
+
+BVH_NODE_DTYPE = np.dtype([("x", "<f4", 4), ("y", "<f4", 4), ("z", "<f4", 4), ("ref", "<u4", 2),
+                           ("pad", "<u4", 2)])
+
+
+def _desc(accel, predicate, leaf_size, bounds):
+    d = SceneDesc()
+    d.accel = ACCEL[accel]
+    d.predicate = PRED[predicate]
+    d.leaf_size = leaf_size
+    if bounds is not None:
+        d.has_bounds = 1
+        for k in range(3):
+            d.bounds_min[k] = float(bounds[0][k])
+            d.bounds_max[k] = float(bounds[1][k])
+    return d
+
+
+def host_tree(xyz, idx, accel="kd", leaf_size=0, bounds=None):
+    """Builds the engine's tree on the host (no GPU) and returns its arrays.
+    kd: dict(words, childs, leaf_prims, root_box, info); bvh: dict(nodes,
+    tri_order, info)."""
+    L = load()
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
+    idx = np.ascontiguousarray(idx, np.uint32).reshape(-1)
+    d = _desc(accel, "encore", leaf_size, bounds)
+    h = _vp()
+    check(L.rq_host_tree_build(xyz.ctypes.data, len(xyz) // 3, idx.ctypes.data, len(idx) // 3, ctypes.byref(d),
+                               ctypes.byref(h)))
+    try:
+        info = SceneInfo()
+        check(L.rq_host_tree_info(h, ctypes.byref(info)))
+        inf = {f: getattr(info, f) for f, _ in SceneInfo._fields_}
+        box = np.zeros(6, np.float32)
+        if accel == "kd":
+            nodes = np.zeros((info.nnodes, 2), np.uint32)
+            lp = np.zeros(max(1, info.nrefs), np.uint32)
+            check(L.rq_host_tree_copy(h, nodes.ctypes.data, lp.ctypes.data, box.ctypes.data))
+            return dict(words=nodes[:, 0].copy(), childs=nodes[:, 1].copy(), leaf_prims=lp[: info.nrefs],
+                        root_box=box, info=inf)
+        nodes = np.zeros(info.nnodes, BVH_NODE_DTYPE)
+        order = np.zeros(info.ntris, np.uint32)
+        check(L.rq_host_tree_copy(h, nodes.ctypes.data, order.ctypes.data, box.ctypes.data))
+        return dict(nodes=nodes, tri_order=order, info=inf)
+    finally:
+        L.rq_host_tree_destroy(h)
+
+
+def version():
+    return load().rq_version().decode()
+
+
+def fail_alloc_after(n):
+    load().rq_debug_fail_alloc_after(int(n))

@@ -1,0 +1,424 @@
+"""Deterministic synthetic workloads for the ray-query engine.
+
+Scenes are indexed triangle lists (xyz float32 (V,3), idx uint32 (T,3)); rays
+are float32 (n,8) rows in the rq_ray layout (ox, oy, oz, tmin, dx, dy, dz,
+tmax).  Every generator is seeded through a counter-based SplitMix64 stream so
+results do not depend on the numpy version.
+
+Ray formulas follow ENCORE where it defines them:
+  eye rays        RayTracer::CalcEyeRaySamples (RayTracer.cpp:489-525) with
+                  Camera::CalcCamVectors (Camera.cpp:157-166); up is always
+                  (0,1,0) because Camera::set ignores its argument (:40-47)
+  shadow rays     PhotonMapperCPU::IsInShadow (PhotonMapperCPU.cpp:1192-1207)
+  bounce rays     cosine-weighted about the normal, origin pushed along the
+                  new direction (PhotonMapperCPU.cpp:743-744)
+"""
+import math
+
+import numpy as np
+
+SEED_BASE = 0x454E434F52450000
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(seed, counter):
+    """SplitMix64 output for (seed, counter) pairs; counter is a uint64 array."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (np.asarray(counter, np.uint64) + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+class Stream:
+    """Counter-based uniform stream: draw(n) gives the next n doubles in [0,1)."""
+
+    def __init__(self, seed):
+        self.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        self.pos = 0
+
+    def draw(self, n):
+        c = np.arange(self.pos, self.pos + n, dtype=np.uint64)
+        self.pos += n
+        return (splitmix64(self.seed, c) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+    def normal(self, n):
+        u1 = np.maximum(self.draw(n), 1e-300)
+        u2 = self.draw(n)
+        return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+# ---------------------------------------------------------------------------
+# ENCORE's float32 vector helpers (Vector3.h:137-162, Vector3.cpp:105-140)
+# ---------------------------------------------------------------------------
+def _flush(x):
+    x = np.asarray(x, np.float32)
+    return np.where((x > -0.001) & (x < 0.001), np.float32(0), x).astype(np.float32)
+
+
+def cross_flush(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    x = a[..., 1] * b[..., 2] - a[..., 2] * b[..., 1]
+    y = np.float32(-1) * (a[..., 0] * b[..., 2] - a[..., 2] * b[..., 0])
+    z = a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]
+    return np.stack([_flush(x), _flush(y), _flush(z)], axis=-1)
+
+
+def normalize_flush(v):
+    v = np.asarray(v, np.float32)
+    m2 = (v[..., 0] * v[..., 0] + v[..., 1] * v[..., 1]) + v[..., 2] * v[..., 2]
+    m = np.sqrt(m2).astype(np.float32)
+    return np.stack([_flush(v[..., k] / m) for k in range(3)], axis=-1)
+
+
+# ---------------------------------------------------------------------------
+# Scenes
+# ---------------------------------------------------------------------------
+_QUAD = np.array([[0, 1, 2], [2, 3, 0]], np.uint32)
+# Unit box faces in the winding of models/bwl/box.ply.
+_BOX_FACES = np.array([[0, 5, 1], [0, 4, 5], [3, 2, 6], [6, 7, 3], [0, 7, 4], [0, 3, 7],
+                       [2, 1, 5], [5, 6, 2], [4, 6, 5], [4, 7, 6], [0, 1, 2], [2, 3, 0]], np.uint32)
+_BOX_VERTS = np.array([[0, 0, 0], [1, 0, 0], [1, 0, 1], [0, 0, 1],
+                       [0, 1, 0], [1, 1, 0], [1, 1, 1], [0, 1, 1]], np.float64)
+
+
+def _merge(parts):
+    xyz, idx, base = [], [], 0
+    for v, f in parts:
+        xyz.append(np.asarray(v, np.float64))
+        idx.append(np.asarray(f, np.uint32) + base)
+        base += len(v)
+    return np.concatenate(xyz).astype(np.float32), np.concatenate(idx).astype(np.uint32)
+
+
+def cornell(seed=SEED_BASE + 1, scale=1.0):
+    """34-triangle Cornell box: five walls at +-50 (2 triangles each) and two
+    upright boxes (12 triangles each) standing on the floor.  The cube's edge
+    is U(20,35); the 1x2x1 tall box is U(50,70) high; rotations about +y are
+    U(-45,45) degrees; centres are U(-30,30) in x and z.  Geometry follows
+    environment/scenes/diffuseCornell.txt:5-30.  `scale` multiplies every
+    coordinate (1/100 gives the unit-size box used for photon mapping)."""
+    st = Stream(seed)
+    h = 50.0
+    walls = [
+        [[-h, -h, -h], [-h, -h, h], [h, -h, h], [h, -h, -h]],  # floor
+        [[-h, h, -h], [h, h, -h], [h, h, h], [-h, h, h]],      # ceiling
+        [[-h, -h, -h], [h, -h, -h], [h, h, -h], [-h, h, -h]],  # back
+        [[-h, -h, -h], [-h, h, -h], [-h, h, h], [-h, -h, h]],  # left
+        [[h, -h, -h], [h, -h, h], [h, h, h], [h, h, -h]],      # right
+    ]
+    parts = [(np.array(w), _QUAD) for w in walls]
+    r = st.draw(8)
+    for bi, (lo, hi, tall) in enumerate([(20.0, 35.0, False), (50.0, 70.0, True)]):
+        size = lo + (hi - lo) * r[4 * bi]
+        ang = math.radians(-45.0 + 90.0 * r[4 * bi + 1])
+        cx = -30.0 + 60.0 * r[4 * bi + 2]
+        cz = -30.0 + 60.0 * r[4 * bi + 3]
+        v = _BOX_VERTS.copy()
+        if tall:
+            v[:, 1] *= 2.0
+            v *= size / 2.0
+        else:
+            v *= size
+        v[:, 0] -= v[:, 0].mean()
+        v[:, 2] -= v[:, 2].mean()
+        c, s = math.cos(ang), math.sin(ang)
+        x = c * v[:, 0] + s * v[:, 2]
+        z = -s * v[:, 0] + c * v[:, 2]
+        v = np.stack([x + cx, v[:, 1] - h, z + cz], axis=1)
+        parts.append((v, _BOX_FACES))
+    xyz, idx = _merge(parts)
+    if scale != 1.0:
+        xyz = (xyz.astype(np.float64) * scale).astype(np.float32)
+    return xyz, idx
+
+
+def _icosahedron():
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t],
+                  [0, -1, -t], [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4],
+                  [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8],
+                  [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    return v / np.linalg.norm(v, axis=1, keepdims=True), f
+
+
+def icosphere(level):
+    """Unit icosphere: 20*4^level triangles, 10*4^level + 2 vertices."""
+    v, f = _icosahedron()
+    for _ in range(level):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e.sort(axis=1)
+        key = e[:, 0] * (len(v) + 1) + e[:, 1]
+        uk, inv = np.unique(key, return_inverse=True)
+        a = uk // (len(v) + 1)
+        b = uk % (len(v) + 1)
+        mid = v[a] + v[b]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        m = (inv + len(v)).reshape(3, -1).T  # midpoints of edges 01, 12, 20
+        v = np.concatenate([v, mid])
+        f0, f1, f2 = f[:, 0], f[:, 1], f[:, 2]
+        m01, m12, m20 = m[:, 0], m[:, 1], m[:, 2]
+        f = np.concatenate([np.stack([f0, m01, m20], 1), np.stack([f1, m12, m01], 1),
+                            np.stack([f2, m20, m12], 1), np.stack([m01, m12, m20], 1)])
+    return v, f
+
+
+def geodesic_sphere(freq):
+    """Icosahedron with each face split into freq^2 triangles, on the unit
+    sphere: 20*freq^2 triangles."""
+    v0, f0 = _icosahedron()
+    verts, faces = [], []
+    # barycentric lattice per face; shared edges are welded by rounding keys.
+    ii, jj = np.meshgrid(np.arange(freq + 1), np.arange(freq + 1), indexing="ij")
+    mask = ii + jj <= freq
+    ii, jj = ii[mask], jj[mask]
+    lid = -np.ones((freq + 1, freq + 1), np.int64)
+    lid[ii, jj] = np.arange(len(ii))
+    tri = []
+    for i in range(freq):
+        for j in range(freq - i):
+            tri.append([lid[i, j], lid[i + 1, j], lid[i, j + 1]])
+            if j + i + 1 < freq:
+                tri.append([lid[i + 1, j], lid[i + 1, j + 1], lid[i, j + 1]])
+    tri = np.array(tri, np.int64)
+    base = 0
+    for a, b, c in f0:
+        p = (v0[a][None] * (freq - ii - jj)[:, None] + v0[b][None] * ii[:, None] + v0[c][None] * jj[:, None]) / freq
+        verts.append(p)
+        faces.append(tri + base)
+        base += len(p)
+    v = np.concatenate(verts)
+    f = np.concatenate(faces)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    key = np.round(v * 1e9).astype(np.int64)
+    _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    inv = inv.reshape(-1)
+    return v[first], inv[f]
+
+
+def value_noise3(p, seed, freqs=(4.0, 8.0, 16.0), amps=(1.0, 0.5, 0.25)):
+    """Sum of value-noise octaves at points p (n,3), normalised to [-1,1].
+    Lattice values are hashed from (seed, octave, cell) with SplitMix64 and
+    interpolated trilinearly with a smoothstep fade."""
+    out = np.zeros(len(p))
+    for o, (fr, am) in enumerate(zip(freqs, amps)):
+        q = p * fr
+        c = np.floor(q)
+        w = q - c
+        w = w * w * (3.0 - 2.0 * w)
+        ci = c.astype(np.int64)
+        acc = np.zeros(len(p))
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    key = (((ci[:, 0] + dx) & 0xFFFFF) << 40) | (((ci[:, 1] + dy) & 0xFFFFF) << 20) | (
+                        (ci[:, 2] + dz) & 0xFFFFF)
+                    hv = splitmix64(seed + 0x1000 * (o + 1), key.astype(np.uint64))
+                    val = (hv >> np.uint64(11)).astype(np.float64) * (2.0 / 9007199254740992.0) - 1.0
+                    wx = w[:, 0] if dx else 1.0 - w[:, 0]
+                    wy = w[:, 1] if dy else 1.0 - w[:, 1]
+                    wz = w[:, 2] if dz else 1.0 - w[:, 2]
+                    acc += val * wx * wy * wz
+        out += am * acc
+    return out / sum(amps)
+
+
+def displaced_sphere(level=None, freq=None, seed=SEED_BASE + 2, amplitude=0.1, world_scale=100.0,
+                     ground=True):
+    """Icosphere (level) or geodesic sphere (freq) displaced radially by
+    r = 1 + amplitude * noise(p), scaled by world_scale, optionally above a
+    two-triangle ground plane at y = -1.2 S spanning +-10 S."""
+    if level is not None:
+        v, f = icosphere(level)
+    else:
+        v, f = geodesic_sphere(freq)
+    r = 1.0 + amplitude * value_noise3(v, seed)
+    v = v * r[:, None] * world_scale
+    parts = [(v, f)]
+    if ground:
+        S = world_scale
+        g = np.array([[-10 * S, -1.2 * S, -10 * S], [-10 * S, -1.2 * S, 10 * S],
+                      [10 * S, -1.2 * S, 10 * S], [10 * S, -1.2 * S, -10 * S]])
+        parts.append((g, _QUAD))
+    return _merge(parts)
+
+
+# ---------------------------------------------------------------------------
+# Rays
+# ---------------------------------------------------------------------------
+def camera_basis(eye, look):
+    """(n, u, v) of Camera::CalcCamVectors with up = (0,1,0), float32."""
+    eye = np.asarray(eye, np.float32)
+    look = np.asarray(look, np.float32)
+    n = (eye - look).astype(np.float32)
+    up = np.array([0, 1, 0], np.float32)
+    u = cross_flush(up, n)
+    n = normalize_flush(n)
+    u = normalize_flush(u)
+    v = cross_flush(n, u)
+    return n, u, v
+
+
+def eye_rays(cols, rows, eye, look, fov_deg=45.0, aspect=None, near=0.1):
+    """One pixel-centre ray per pixel, row-major from row 0, with the float32
+    operation order of RayTracer::CalcEyeRaySamples (PI = 3.14159f)."""
+    if aspect is None:
+        aspect = cols / rows
+    near = np.float32(near)
+    H = np.float32(float(near) * math.tan(float(np.float32(fov_deg)) * (float(np.float32(3.14159)) / 360.0)))
+    W = np.float32(H * np.float32(aspect))
+    n, u, v = camera_basis(eye, look)
+    col = np.arange(cols, dtype=np.float32)
+    row = np.arange(rows, dtype=np.float32)
+    x = col + np.float32(0.5)
+    y = row + np.float32(0.5)
+    Uc = -W + (W * (np.float32(2) * x) / np.float32(cols))
+    Vr = -H + (H * (np.float32(2) * y) / np.float32(rows))
+    Uc = np.broadcast_to(Uc[None, :], (rows, cols)).reshape(-1).astype(np.float32)
+    Vr = np.broadcast_to(Vr[:, None], (rows, cols)).reshape(-1).astype(np.float32)
+    nn = n * (-near)
+    d = (nn[None, :] + u[None, :] * Uc[:, None]).astype(np.float32)
+    d = (d + v[None, :] * Vr[:, None]).astype(np.float32)
+    d = normalize_flush(d)
+    rays = np.zeros((len(d), 8), np.float32)
+    rays[:, 0:3] = np.asarray(eye, np.float32)
+    rays[:, 3] = 0.0
+    rays[:, 4:7] = d
+    rays[:, 7] = np.inf
+    return rays
+
+
+def hit_points(xyz, idx, hits):
+    """ENCORE's hitPoint = v0 + (v1-v0)*alpha + (v2-v0)*beta (TrianglePrim.cpp:239)
+    and the normal Normalize(Cross(e1, e2)) (:236-238) for hit rows; hits is
+    (n,4) float32 with the prim id in column 3 as uint32 bits."""
+    prim = np.ascontiguousarray(hits[:, 3]).view(np.uint32)
+    ok = prim != 0xFFFFFFFF
+    tri = idx[prim[ok]]
+    v0, v1, v2 = xyz[tri[:, 0]], xyz[tri[:, 1]], xyz[tri[:, 2]]
+    a = hits[ok, 1:2].astype(np.float32)
+    b = hits[ok, 2:3].astype(np.float32)
+    e1 = (v1 - v0).astype(np.float32)
+    e2 = (v2 - v0).astype(np.float32)
+    p = (v0 + e1 * a).astype(np.float32)
+    p = (p + e2 * b).astype(np.float32)
+    nrm = normalize_flush(cross_flush(e1, e2))
+    return ok, p, nrm
+
+
+def shadow_rays(points, light_center, light_size, stream):
+    """One ray per point toward a uniform point on a square light facing -y;
+    origin pushed by 0.01 along the ray; tmax = |L - p| measured from the
+    un-pushed point (PhotonMapperCPU.cpp:1192-1207)."""
+    n = len(points)
+    r = stream.draw(2 * n).reshape(n, 2)
+    L = np.empty((n, 3), np.float32)
+    L[:, 0] = light_center[0] + (r[:, 0] - 0.5) * light_size
+    L[:, 1] = light_center[1]
+    L[:, 2] = light_center[2] + (r[:, 1] - 0.5) * light_size
+    dvec = (L - points).astype(np.float32)
+    dist = np.sqrt((dvec[:, 0] * dvec[:, 0] + dvec[:, 1] * dvec[:, 1]) + dvec[:, 2] * dvec[:, 2]).astype(np.float32)
+    d = normalize_flush(dvec)
+    o = (points + d * np.float32(0.01)).astype(np.float32)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 4:7] = d
+    rays[:, 7] = dist
+    return rays
+
+
+def cosine_dirs(normals, stream):
+    """Cosine-weighted directions about unit normals (float64 math)."""
+    n = len(normals)
+    r = stream.draw(2 * n).reshape(n, 2)
+    cos_t = np.sqrt(r[:, 0])
+    sin_t = np.sqrt(1.0 - r[:, 0])
+    phi = 2.0 * np.pi * r[:, 1]
+    N = np.asarray(normals, np.float64)
+    N = N / np.linalg.norm(N, axis=1, keepdims=True)
+    a = np.where(np.abs(N[:, 0:1]) > 0.9, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    T = np.cross(a, N)
+    T /= np.linalg.norm(T, axis=1, keepdims=True)
+    B = np.cross(N, T)
+    d = T * (sin_t * np.cos(phi))[:, None] + B * (sin_t * np.sin(phi))[:, None] + N * cos_t[:, None]
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def bounce_rays(points, normals, incoming, count, stream, offset):
+    """`count` diffuse-bounce rays from points sampled with replacement: the
+    normal is flipped to face the incoming ray, the direction is
+    cosine-weighted about it and the origin is pushed by `offset` along it."""
+    pick = np.minimum((stream.draw(count) * len(points)).astype(np.int64), len(points) - 1)
+    P = points[pick].astype(np.float64)
+    N = normals[pick].astype(np.float64)
+    D_in = incoming[pick].astype(np.float64)
+    flip = np.sum(N * D_in, axis=1) > 0
+    N[flip] = -N[flip]
+    d = cosine_dirs(N, stream)
+    rays = np.zeros((count, 8), np.float32)
+    rays[:, 0:3] = (P + d * offset).astype(np.float32)
+    rays[:, 4:7] = d.astype(np.float32)
+    rays[:, 7] = np.inf
+    return rays
+
+
+def lognormal_batches(total, count, stream, median=65536, sigma=1.18, lo=4096, hi=1048576):
+    """`count` batch sizes: exp(ln median + sigma z) clipped to [lo, hi], then
+    rescaled (and re-clipped) until they sum to exactly `total`."""
+    z = stream.normal(count)
+    s = np.clip(np.exp(np.log(median) + sigma * z), lo, hi)
+    for _ in range(64):
+        s = np.clip(s * (total / s.sum()), lo, hi)
+    s = np.floor(s).astype(np.int64)
+    diff = int(total - s.sum())
+    i = 0
+    while diff != 0:
+        j = i % count
+        step = 1 if diff > 0 else -1
+        if lo <= s[j] + step <= hi:
+            s[j] += step
+            diff -= step
+        i += 1
+    return s
+
+
+# ---------------------------------------------------------------------------
+# Named workloads
+# ---------------------------------------------------------------------------
+def config1(seed=SEED_BASE + 1, res=512):
+    """Cornell box (34 triangles) and res x res pixel-centre eye rays from
+    (0,0,170) looking at the origin, fov 45, aspect 1, near 0.1."""
+    xyz, idx = cornell(seed)
+    rays = eye_rays(res, res, (0.0, 0.0, 170.0), (0.0, 0.0, 0.0), 45.0, 1.0, 0.1)
+    return xyz, idx, rays
+
+
+def config2_scene(level=8, seed=SEED_BASE + 2, world_scale=100.0):
+    """Displaced icosphere (level 8: 1,310,720 triangles) plus a 2-triangle
+    ground, world scale S = 100."""
+    return displaced_sphere(level=level, seed=seed, world_scale=world_scale)
+
+
+def config2_primary(cols=1920, rows=1080, world_scale=100.0):
+    S = world_scale
+    return eye_rays(cols, rows, (0.0, 0.3 * S, 3.5 * S), (0.0, 0.0, 0.0), 45.0, cols / rows, 0.1)
+
+
+def config2_shadow(xyz, idx, primary_rays, primary_hits, seed=SEED_BASE + 2, world_scale=100.0):
+    S = world_scale
+    ok, p, nrm = hit_points(xyz, idx, primary_hits)
+    st = Stream(seed + 0x5000)
+    return ok, shadow_rays(p, (2 * S, 4 * S, 2 * S), 1.0 * S, st)
+
+
+def config3_rays(xyz, idx, primary_rays, primary_hits, count=8388608, nbatches=64,
+                 seed=SEED_BASE + 3, world_scale=100.0):
+    """Incoherent secondary rays from primary hit points and the 64 lognormal
+    batch sizes that partition them."""
+    ok, p, nrm = hit_points(xyz, idx, primary_hits)
+    st = Stream(seed)
+    rays = bounce_rays(p, nrm, primary_rays[ok, 4:7], count, st, 1e-3 * world_scale)
+    sizes = lognormal_batches(count, nbatches, Stream(seed + 0x77))
+    return rays, sizes

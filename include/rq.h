@@ -1,0 +1,183 @@
+/*
+ * rq.h -- C ABI of the MI355X ray-query engine (libencore_rq.so).
+ *
+ * The engine answers the queries that ENCORE's renderers send through
+ * `AccelerationStructure` (src/accelerators/AccelerationStructure.h:33-67) and
+ * `PhotonMap` (src/renderers/PhotonMap.h:28-104): build from a triangle list,
+ * closest hit, any hit (shadow), and photon k-nearest-neighbour gather.
+ *
+ * Conventions
+ *  - Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ *    returns an `rq_status`.  No exception crosses the ABI.  On failure the
+ *    message is available from `rq_last_error()` (thread-local).
+ *  - Pointers named `d_*` are device memory owned by the caller (hipMalloc'd
+ *    or any device allocation on the scene's GPU).  Pointers without the
+ *    prefix are host memory and are copied before the call returns.
+ *  - `stream` is a `hipStream_t` passed as `void*`; NULL is the default
+ *    stream.  Query calls are asynchronous with respect to the host.
+ *  - Scenes and photon maps are immutable after creation, so any number of
+ *    threads may issue queries on one object concurrently on distinct streams.
+ */
+#ifndef ENCORE_RQ_H
+#define ENCORE_RQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rq_status {
+    RQ_OK = 0,
+    RQ_ERR_INVALID_ARG = 1,  /* null pointer, bad size, bad enum value */
+    RQ_ERR_OOM = 2,          /* device or host allocation failed; state unchanged */
+    RQ_ERR_DEVICE = 3,       /* a HIP runtime call failed */
+    RQ_ERR_UNSUPPORTED = 4,  /* option not available for this object */
+    RQ_ERR_CAPACITY = 5,     /* request larger than a configured limit */
+    RQ_ERR_TIMEOUT = 6,      /* a wait or a kernel's iteration cap expired */
+    RQ_ERR_CANCELLED = 7     /* request cancelled before completion */
+} rq_status;
+
+typedef struct rq_device rq_device;
+typedef struct rq_scene rq_scene;
+
+/* A ray: origin, direction and the open parameter interval (tmin, tmax).
+ * A triangle hit at t is accepted iff tmin < t < tmax.  ENCORE's eye, shadow
+ * and bounce rays use tmin = 0 and tmax = +inf (Kdtree.cpp:682-784 clips the
+ * ray to the scene box starting at t = 0 and accepts any t > 0,
+ * TrianglePrim.cpp:223-224).  32 bytes. */
+typedef struct rq_ray {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} rq_ray;
+
+/* Closest-hit record, 16 bytes.  `prim` is the index of the triangle in the
+ * list given to rq_scene_create (ENCORE's model-list x primitive-list
+ * concatenation order, Kdtree.cpp:99-114), RQ_NO_HIT on a miss.  `u` is the
+ * barycentric weight of v1 and `v` the weight of v2 (ENCORE's alpha and beta,
+ * TrianglePrim.cpp:219-220,239).  On a miss t = tmax and u = v = 0. */
+typedef struct rq_hit {
+    float t, u, v;
+    uint32_t prim;
+} rq_hit;
+
+#define RQ_NO_HIT 0xFFFFFFFFu
+
+typedef enum rq_accel_kind {
+    RQ_ACCEL_BVH = 0,  /* binned-SAH BVH2, 64-byte nodes */
+    RQ_ACCEL_KD = 1    /* ENCORE's spatial-median kd-tree, same build rules */
+} rq_accel_kind;
+
+typedef enum rq_predicate {
+    /* ENCORE's Moller-Trumbore variant, bit-exact: Cross() flushes components
+     * below 1e-3 to zero, no determinant test, inclusive edges, t > tmin
+     * (TrianglePrim.cpp:204-249, Vector3.h:148-162, Vector3.cpp:126-140). */
+    RQ_PRED_ENCORE = 0,
+    /* Watertight test of Woop, Benthin and Wald (JCGT 2013): no gaps or
+     * double hits along shared edges; FP64 fallback for exact-zero edge
+     * functions. */
+    RQ_PRED_WATERTIGHT = 1
+} rq_predicate;
+
+typedef struct rq_scene_desc {
+    uint32_t accel;        /* rq_accel_kind */
+    uint32_t predicate;    /* rq_predicate */
+    uint32_t leaf_size;    /* BVH only: max triangles per leaf, 1..16 (0 = 4) */
+    uint32_t has_bounds;   /* KD only: 1 = bounds_min/max are the model-list
+                              bounds Kdtree::build reads from IModel::MinPoint
+                              and MaxPoint (Kdtree.cpp:99-107); 0 = vertex
+                              bounds of the triangle list */
+    float bounds_min[3];
+    float bounds_max[3];
+} rq_scene_desc;
+
+/* Scene statistics, filled by rq_scene_stats. */
+typedef struct rq_scene_info {
+    uint32_t accel, predicate;
+    uint32_t ntris;
+    uint32_t nnodes;       /* BVH: inner nodes; KD: nodes incl. leaves */
+    uint32_t nleaves;
+    uint32_t nrefs;        /* triangle references in leaves */
+    uint32_t max_depth;
+    uint32_t kd_max_depth_limit;  /* KD: 11 + floor(1.3 ln N) (Kdtree.cpp:123) */
+    uint64_t device_bytes; /* bytes resident in HBM for this scene */
+    double build_seconds;  /* host build time */
+} rq_scene_info;
+
+/* Library version string, e.g. "encore-rq 0.1.0 gfx950". */
+const char* rq_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* rq_last_error(void);
+
+/* Open GPU `ordinal` (hipSetDevice numbering).  One rq_device per GPU and
+ * process is enough; it is thread-safe. */
+rq_status rq_device_open(int ordinal, rq_device** out);
+void rq_device_close(rq_device* dev);
+
+/* Build an acceleration structure over an indexed triangle list (host
+ * arrays: xyz = 3*nverts floats, tri_idx = 3*ntris vertex indices) and
+ * upload it.  Replaces AccelerationStructure::build
+ * (AccelerationStructure.h:47; Kdtree.cpp:84-200; BVH.cpp:123-187).
+ * desc may be NULL (BVH, ENCORE predicate). */
+rq_status rq_scene_create(rq_device* dev, const float* xyz, uint32_t nverts,
+                          const uint32_t* tri_idx, uint32_t ntris,
+                          const rq_scene_desc* desc, rq_scene** out);
+void rq_scene_destroy(rq_scene* scene);
+rq_status rq_scene_stats(const rq_scene* scene, rq_scene_info* info);
+
+/* Closest hit for n rays: d_rays[n] -> d_hits[n].  Replaces
+ * AccelerationStructure::intersect (AccelerationStructure.h:63,
+ * Kdtree.cpp:682-784) for a batch. */
+rq_status rq_intersect_closest(rq_scene* scene, const rq_ray* d_rays, uint32_t n,
+                               rq_hit* d_hits, void* stream);
+
+/* Any hit: d_occluded[i] = 1 iff some triangle is hit with tmin < t < tmax,
+ * else 0.  With tmax = |L - p| this is PhotonMapperCPU::IsInShadow
+ * (PhotonMapperCPU.cpp:1192-1207), which ENCORE answers with a full
+ * closest-hit query. */
+rq_status rq_occluded(rq_scene* scene, const rq_ray* d_rays, uint32_t n,
+                      uint32_t* d_occluded, void* stream);
+
+/* Traversal work of one query over n rays, counted by an instrumented copy
+ * of the same kernel (synchronous): counts[0] = rays, counts[1] = nodes
+ * visited (64-byte BVH nodes or 8-byte kd nodes), counts[2] = triangles
+ * tested (48-byte records; on the kd path each also reads a 4-byte leaf
+ * reference).  These are the N_node and N_tri of the algorithmic-bytes
+ * figure used for the roofline.  No counterpart in ENCORE (measurement). */
+rq_status rq_trace_counts(rq_scene* scene, const rq_ray* d_rays, uint32_t n, uint32_t any_hit,
+                          uint64_t* counts, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Host-side trees.  Builds the same structure rq_scene_create uploads, on
+ * the host only (no GPU needed), and copies it out: for inspection, for
+ * comparison with ENCORE's own Kdtree numbering, and for storing a packed
+ * scene.  KD nodes are 8 bytes {word, child} in Kdtree's numbering
+ * (Kdtree.h:25-58: word = (count << 2) | 3 for a leaf, else the split's float
+ * bits with the axis in the low two bits; child = left child, or the leaf's
+ * offset into leaf_prims).  BVH nodes are 64 bytes: float x[4], y[4], z[4]
+ * holding (lo, hi) of child 0 then child 1 per axis, then uint32 ref[2]
+ * (bit 31 clear: inner node index; set: leaf, count-1 in bits 27..30, first
+ * triangle in bits 0..26) and 8 bytes of padding; `leaf_prims` then receives
+ * the triangle order of the leaves (leaf slot -> original triangle index).
+ * --------------------------------------------------------------------- */
+typedef struct rq_host_tree rq_host_tree;
+
+rq_status rq_host_tree_build(const float* xyz, uint32_t nverts, const uint32_t* tri_idx, uint32_t ntris,
+                             const rq_scene_desc* desc, rq_host_tree** out);
+rq_status rq_host_tree_info(const rq_host_tree* tree, rq_scene_info* info);
+/* nodes: info.nnodes * (8 or 64) bytes; leaf_prims: info.nrefs (KD) or
+ * info.ntris (BVH) entries; root_box: pos[3], size[3] (KD) or zeros. */
+rq_status rq_host_tree_copy(const rq_host_tree* tree, void* nodes, uint32_t* leaf_prims, float* root_box);
+void rq_host_tree_destroy(rq_host_tree* tree);
+
+/* Test hook: the n-th device allocation from now fails with RQ_ERR_OOM
+ * (n = 0 disables).  For exercising the out-of-memory paths. */
+void rq_debug_fail_alloc_after(int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ENCORE_RQ_H */

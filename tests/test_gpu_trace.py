@@ -1,0 +1,183 @@
+"""GPU parity: closest hit and any hit through the C ABI against the oracle.
+
+ENCORE's CPU kd-tree (restated in oracle/) is the reference.  The engine's kd
+path must match it bit for bit on every ray; the BVH path must match it
+exactly except where two candidate t agree within 1e-6 t (ties) or where the
+reference kd-tree itself misses the brute-force closest hit."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from encore_amd import rq, scenes
+from helpers import brute_occluded, classify_closest, log, split_hits
+
+pytestmark = pytest.mark.gpu
+
+
+def _trace(scene, rays, kind="closest"):
+    d_rays = torch.from_numpy(np.ascontiguousarray(rays, np.float32)).cuda()
+    if kind == "closest":
+        out = torch.full((len(rays), 4), -7.0, dtype=torch.float32, device="cuda")
+        scene.intersect(d_rays, out)
+    else:
+        out = torch.full((len(rays),), 7, dtype=torch.int32, device="cuda")
+        scene.occluded(d_rays, out)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def _check_vs_oracle(xyz, idx, rays, got, kd_ref, bit_exact=True):
+    c = classify_closest(xyz, idx, rays, got, kd_ref, bit_exact=bit_exact)
+    if c["bad"]:
+        # adjudicate the remaining rays by brute force
+        bad = np.array(c["bad_idx"])
+        brute = oracle.brute_closest(xyz, idx, rays[bad])
+        c2 = classify_closest(xyz, idx, rays[bad], got[bad], kd_ref[bad], brute=brute, bit_exact=bit_exact)
+        c["ref_defect"] = c2["ref_defect"]
+        c["tie"] += c2["tie"]
+        c["bad"] = c2["bad"]
+        c["bad_idx"] = bad[c2["bad_idx"]].tolist()
+    return c
+
+
+@pytest.fixture(scope="module")
+def cornell():
+    xyz, idx, rays = scenes.config1(res=512)
+    kd = oracle.KdTree(xyz, idx)
+    return xyz, idx, rays, kd.intersect(rays)
+
+
+@pytest.fixture(scope="module")
+def sphere():
+    """Displaced icosphere (level 5, 20,482 triangles incl. ground) with
+    primary, shadow and bounce rays."""
+    xyz, idx = scenes.displaced_sphere(level=5)
+    prim = scenes.config2_primary(cols=320, rows=180)
+    kd = oracle.KdTree(xyz, idx)
+    ph = kd.intersect(prim)
+    ok, shadow = scenes.config2_shadow(xyz, idx, prim, ph)
+    bounce, _ = scenes.config3_rays(xyz, idx, prim, ph, count=40000, nbatches=4)
+    return dict(xyz=xyz, idx=idx, kd=kd, primary=prim, primary_ref=ph, shadow=shadow, bounce=bounce,
+                bounce_ref=kd.intersect(bounce), shadow_ref=kd.occluded(shadow))
+
+
+def test_config1_kd_path_is_bit_exact(device, cornell):
+    xyz, idx, rays, ref = cornell
+    log("case: config1 512x512 kd/encore closest")
+    sc = rq.Scene(device, xyz, idx, accel="kd")
+    got = _trace(sc, rays)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_config1_bvh_closest(device, cornell):
+    xyz, idx, rays, ref = cornell
+    log("case: config1 512x512 bvh/encore closest")
+    sc = rq.Scene(device, xyz, idx, accel="bvh")
+    got = _trace(sc, rays)
+    c = _check_vs_oracle(xyz, idx, rays, got, ref)
+    log("config1 bvh:", {k: v for k, v in c.items() if k != "bad_idx"})
+    assert c["bad"] == 0, c["bad_idx"][:10]
+    assert c["exact"] + c["tie"] + c["ref_defect"] == c["n"]
+
+
+@pytest.mark.parametrize("leaf", [1, 4, 8])
+def test_sphere_bvh_closest(device, sphere, leaf):
+    s = sphere
+    log("case: sphere L5 bvh/encore leaf", leaf)
+    sc = rq.Scene(device, s["xyz"], s["idx"], accel="bvh", leaf_size=leaf)
+    for name in ("primary", "bounce"):
+        rays, ref = s[name], s[name + "_ref"]
+        got = _trace(sc, rays)
+        c = _check_vs_oracle(s["xyz"], s["idx"], rays, got, ref)
+        log(name, {k: v for k, v in c.items() if k != "bad_idx"})
+        assert c["bad"] == 0, (name, c["bad_idx"][:10])
+
+
+def test_sphere_kd_path_is_bit_exact(device, sphere):
+    s = sphere
+    log("case: sphere L5 kd/encore closest + occluded")
+    sc = rq.Scene(device, s["xyz"], s["idx"], accel="kd")
+    for name in ("primary", "bounce"):
+        got = _trace(sc, s[name])
+        assert np.array_equal(got.view(np.uint32), s[name + "_ref"].view(np.uint32)), name
+    occ = _trace(sc, s["shadow"], "any")
+    assert np.array_equal(occ.astype(np.uint32), s["shadow_ref"])
+
+
+def test_sphere_bvh_occluded(device, sphere):
+    s = sphere
+    log("case: sphere L5 bvh/encore occluded")
+    sc = rq.Scene(device, s["xyz"], s["idx"], accel="bvh")
+    occ = _trace(sc, s["shadow"], "any").astype(np.uint32)
+    diff = np.nonzero(occ != s["shadow_ref"])[0]
+    if len(diff):
+        # every disagreement must be a reference kd miss: brute force decides
+        b = brute_occluded(s["xyz"], s["idx"], s["shadow"][diff])
+        assert np.array_equal(b, occ[diff])
+    assert occ.mean() > 0.01
+
+
+def test_watertight_bvh(device, cornell, sphere):
+    """Watertight predicate: same closest hits within 1e-5 relative in t and
+    the barycentrics, except ties and rays that graze triangle edges."""
+    for name, (xyz, idx, rays, ref) in (("cornell", cornell),
+                                       ("sphere", (sphere["xyz"], sphere["idx"], sphere["bounce"],
+                                                   sphere["bounce_ref"]))):
+        log("case: watertight bvh", name)
+        sc = rq.Scene(device, xyz, idx, accel="bvh", predicate="watertight")
+        got = _trace(sc, rays)
+        c = classify_closest(xyz, idx, rays, got, ref, bit_exact=False)
+        log(name, {k: v for k, v in c.items() if k != "bad_idx"})
+        assert c["bad"] == 0, c["bad_idx"][:10]
+        assert c["edge_band"] <= 1e-3 * c["n"]
+
+
+def test_fp64_error_report(device, sphere):
+    """FP64 brute-force oracle on a 1% sample: the engine's closest prim is the
+    FP64 closest prim (up to ties) and t, u, v errors stay below 1e-5 rel."""
+    s = sphere
+    rays = s["bounce"][::100]
+    log("case: fp64 sample", len(rays))
+    sc = rq.Scene(device, s["xyz"], s["idx"], accel="bvh")
+    got = _trace(sc, rays)
+    gt, gu, gv, gp = split_hits(got)
+    f, fp = oracle.fp64_closest(s["xyz"], s["idx"], rays)
+    hit = (fp != oracle.NO_HIT) & (gp != oracle.NO_HIT)
+    same = hit & (gp == fp)
+    rel_t = np.abs(gt[same] - f[same, 0]) / np.abs(f[same, 0])
+    err_u = np.abs(gu[same] - f[same, 1])
+    err_v = np.abs(gv[same] - f[same, 2])
+    log("fp64: n=%d same=%d maxrel_t=%.3g max|du|=%.3g max|dv|=%.3g" % (len(rays), same.sum(), rel_t.max(),
+                                                                      err_u.max(), err_v.max()))
+    assert same.sum() >= 0.99 * hit.sum()
+    assert rel_t.max() < 1e-5
+    assert max(err_u.max(), err_v.max()) < 1e-4
+
+
+def test_edge_cases(device):
+    xyz, idx = scenes.cornell()
+    sc = rq.Scene(device, xyz, idx, accel="bvh")
+    kd = oracle.KdTree(xyz, idx)
+    log("case: edge cases")
+    # n = 0 is a no-op
+    out = torch.full((1, 4), 3.0, device="cuda")
+    sc.intersect(torch.zeros((1, 8), device="cuda"), out, n=0)
+    torch.cuda.synchronize()
+    assert float(out[0, 0]) == 3.0
+    # axis-aligned rays (zero direction components), rays from inside boxes,
+    # rays starting exactly on a wall plane, tmax cut-off
+    rays = []
+    for o in ([0, 0, 0], [10, -20, 5], [-49.5, 0, 0], [0, -50, 0], [50, 10, 10], [0, 0, 170]):
+        for d in ([1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1], [0.6, 0.8, 0]):
+            rays.append(list(o) + [0.0] + list(d) + [np.inf])
+    rays = np.array(rays, np.float32)
+    got = _trace(sc, rays)
+    c = _check_vs_oracle(xyz, idx, rays, got, kd.intersect(rays))
+    assert c["bad"] == 0, c
+    # tmax: nothing is hit before t = 1e-3
+    short = rays.copy()
+    short[:, 7] = 1e-3
+    g2 = _trace(sc, short)
+    assert np.all(split_hits(g2)[3] == oracle.NO_HIT)
+    assert np.all(g2[:, 0] == np.float32(1e-3))
