@@ -510,6 +510,8 @@ __global__ void zero_u32(uint32_t* p, uint32_t n) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+// kd ring: 8 entries x 3 dwords x 64 lanes x 4 waves = 24 KB of LDS per block.
+constexpr int kKdStack = 8;
 template <class Pred, bool ANY>
 static void launch_bvh_t(const TraceLaunch& L, int grid, const KernelConfig& kc, hipStream_t s) {
     switch (kc.stack) {
@@ -527,7 +529,7 @@ static void launch_bvh_t(const TraceLaunch& L, int grid, const KernelConfig& kc,
 
 template <class Pred, bool ANY>
 static void launch_kd_t(const TraceLaunch& L, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((dev::kd_trace<Pred, ANY, 16>), dim3(grid), dim3(256), 0, s, L);
+    hipLaunchKernelGGL((dev::kd_trace<Pred, ANY, kKdStack>), dim3(grid), dim3(256), 0, s, L);
 }
 
 void launch_bvh(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, const KernelConfig& kc,
@@ -569,11 +571,12 @@ void launch_count(const TraceLaunch& L, uint32_t accel, uint32_t predicate, Trac
                   hipStream_t s) {
     const bool any = kind == TraceKind::Any;
 #define RQ_CNT(K, P, A) hipLaunchKernelGGL((dev::K<P, A, 16, true>), dim3(grid), dim3(256), 0, s, L)
+#define RQ_CNTK(K, P, A) hipLaunchKernelGGL((dev::K<P, A, kKdStack, true>), dim3(grid), dim3(256), 0, s, L)
     if (accel == RQ_ACCEL_KD) {
         if (predicate == RQ_PRED_WATERTIGHT) {
-            if (any) RQ_CNT(kd_trace, dev::PredWatertight, true); else RQ_CNT(kd_trace, dev::PredWatertight, false);
+            if (any) RQ_CNTK(kd_trace, dev::PredWatertight, true); else RQ_CNTK(kd_trace, dev::PredWatertight, false);
         } else {
-            if (any) RQ_CNT(kd_trace, dev::PredEncore, true); else RQ_CNT(kd_trace, dev::PredEncore, false);
+            if (any) RQ_CNTK(kd_trace, dev::PredEncore, true); else RQ_CNTK(kd_trace, dev::PredEncore, false);
         }
     } else {
         if (predicate == RQ_PRED_WATERTIGHT) {
@@ -583,6 +586,7 @@ void launch_count(const TraceLaunch& L, uint32_t accel, uint32_t predicate, Trac
         }
     }
 #undef RQ_CNT
+#undef RQ_CNTK
     RQ_HIP(hipGetLastError());
 }
 
@@ -597,11 +601,11 @@ int occupancy_blocks_per_cu(uint32_t accel, uint32_t predicate, TraceKind kind, 
     const void* fn = nullptr;
     if (accel == RQ_ACCEL_KD) {
         if (predicate == RQ_PRED_WATERTIGHT)
-            fn = any ? (const void*)dev::kd_trace<dev::PredWatertight, true, 16>
-                     : (const void*)dev::kd_trace<dev::PredWatertight, false, 16>;
+            fn = any ? (const void*)dev::kd_trace<dev::PredWatertight, true, kKdStack>
+                     : (const void*)dev::kd_trace<dev::PredWatertight, false, kKdStack>;
         else
-            fn = any ? (const void*)dev::kd_trace<dev::PredEncore, true, 16>
-                     : (const void*)dev::kd_trace<dev::PredEncore, false, 16>;
+            fn = any ? (const void*)dev::kd_trace<dev::PredEncore, true, kKdStack>
+                     : (const void*)dev::kd_trace<dev::PredEncore, false, kKdStack>;
     } else {
 #define RQ_PICK(P, A)                                                               \
     (kc.stack == 8    ? (const void*)dev::bvh_trace<P, A, 8>                       \

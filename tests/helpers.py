@@ -66,8 +66,9 @@ def classify_closest(xyz, idx, rays, got, ref, brute=None, bit_exact=True):
     else:
         # t within 1e-5 relative; barycentrics within 1e-5 relative or 1e-3
         # absolute near zero (where ENCORE's flush shifts them).
-        exact = miss_both | (same & close_rel(gt, rt) & close_rel(gu, ru, abs_tol=1e-3) &
-                             close_rel(gv, rv, abs_tol=1e-3))
+        with np.errstate(invalid="ignore"):
+            exact = miss_both | (same & close_rel(gt, rt) & close_rel(gu, ru, abs_tol=1e-3) &
+                                 close_rel(gv, rv, abs_tol=1e-3))
     out = dict(n=len(rays), exact=int(exact.sum()), tie=0, ref_defect=0, edge_band=0, bad=0, bad_idx=[])
     for i in np.nonzero(~exact)[0]:
         if not bit_exact and (_near_edge(gu[i], gv[i], gp[i]) or _near_edge(ru[i], rv[i], rp[i])):
@@ -95,6 +96,19 @@ def classify_closest(xyz, idx, rays, got, ref, brute=None, bit_exact=True):
         out["bad"] += 1
         out["bad_idx"].append(int(i))
     return out
+
+
+def fp64_adjudicate(xyz, idx, rays, got, bad_idx):
+    """Of the rays in bad_idx, those where `got` is the FP64 brute-force
+    closest hit (same prim, or t within 1e-6 t): there ENCORE's float32 flush
+    arithmetic, not the engine, departs from the exact answer."""
+    if not len(bad_idx):
+        return 0, []
+    bad_idx = np.asarray(bad_idx)
+    f, fp = oracle.fp64_closest(xyz, idx, rays[bad_idx])
+    gt, _, _, gp = split_hits(got[bad_idx])
+    agree = (gp == fp) | ((gp != oracle.NO_HIT) & (fp != oracle.NO_HIT) & (np.abs(gt - f[:, 0]) <= 1e-6 * np.abs(f[:, 0])))
+    return int(agree.sum()), bad_idx[~agree].tolist()
 
 
 def brute_occluded(xyz, idx, rays):

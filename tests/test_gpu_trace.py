@@ -10,7 +10,7 @@ import torch
 
 import oracle
 from encore_amd import rq, scenes
-from helpers import brute_occluded, classify_closest, log, split_hits
+from helpers import brute_occluded, classify_closest, fp64_adjudicate, log, split_hits
 
 pytestmark = pytest.mark.gpu
 
@@ -120,7 +120,10 @@ def test_sphere_bvh_occluded(device, sphere):
 
 def test_watertight_bvh(device, cornell, sphere):
     """Watertight predicate: same closest hits within 1e-5 relative in t and
-    the barycentrics, except ties and rays that graze triangle edges."""
+    the barycentrics, except ties, rays that graze triangle edges, and rays
+    where the FP64 brute-force closest hit sides with the engine (ENCORE's
+    Cross() flush moves or drops the hit: bounce rays start 0.1 units from
+    the surface, so s x e1 has components below 1e-3)."""
     for name, (xyz, idx, rays, ref) in (("cornell", cornell),
                                        ("sphere", (sphere["xyz"], sphere["idx"], sphere["bounce"],
                                                    sphere["bounce_ref"]))):
@@ -128,8 +131,11 @@ def test_watertight_bvh(device, cornell, sphere):
         sc = rq.Scene(device, xyz, idx, accel="bvh", predicate="watertight")
         got = _trace(sc, rays)
         c = classify_closest(xyz, idx, rays, got, ref, bit_exact=False)
+        c["fp64_agrees"], c["bad_idx"] = fp64_adjudicate(xyz, idx, rays, got, c["bad_idx"])
+        c["bad"] = len(c["bad_idx"])
         log(name, {k: v for k, v in c.items() if k != "bad_idx"})
         assert c["bad"] == 0, c["bad_idx"][:10]
+        assert c["fp64_agrees"] <= 1e-3 * c["n"]
         assert c["edge_band"] <= 1e-3 * c["n"]
 
 
