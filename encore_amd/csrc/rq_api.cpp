@@ -297,6 +297,117 @@ rq_status rq_trace_counts(rq_scene* scene, const rq_ray* d_rays, uint32_t n, uin
     });
 }
 
+rq_status rq_photons_create(rq_device* dev, const float* pos, const float* normal, const float* power, uint32_t n,
+                            uint32_t layout, rq_photons** out) {
+    return guarded([&] {
+        require(dev && out, "null argument");
+        require(n == 0 || (pos && normal), "null photon arrays");
+        require(layout == RQ_PHOTONS_ENCORE_ORDER || layout == RQ_PHOTONS_SPATIAL, "unknown photon layout");
+        RQ_HIP(hipSetDevice(dev->ordinal));
+        PhotonHost h;
+        build_photons(pos, normal, power, n, layout, h);
+        auto p = std::make_unique<rq_photons>();
+        p->dev = dev;
+        p->n = n;
+        p->nclasses = (uint32_t)h.normals.size();
+        p->table_mask = (uint32_t)(h.table.size() / 4 - 1);
+        const size_t m = std::max<size_t>(1, n);
+        p->hot = DevBuf(m * sizeof(PhotonHot));
+        p->power = DevBuf(m * sizeof(PhotonPower));
+        p->perm = DevBuf(m * sizeof(uint32_t));
+        p->table = DevBuf(h.table.size() * sizeof(uint32_t));
+        if (n) {
+            p->hot.upload(h.hot.data(), n * sizeof(PhotonHot));
+            p->power.upload(h.power.data(), n * sizeof(PhotonPower));
+            p->perm.upload(h.perm.data(), n * sizeof(uint32_t));
+        }
+        p->table.upload(h.table.data(), p->table.bytes);
+        p->perm_host = std::move(h.perm);
+        p->axis_host = std::move(h.axis);
+        *out = p.release();
+    });
+}
+
+void rq_photons_destroy(rq_photons* p) {
+    if (!p) return;
+    (void)hipSetDevice(p->dev->ordinal);
+    delete p;
+}
+
+rq_status rq_photons_order(const rq_photons* p, uint32_t* perm, uint8_t* axis, uint32_t* nclasses) {
+    return guarded([&] {
+        require(p != nullptr, "null photon map");
+        if (perm && p->n) std::memcpy(perm, p->perm_host.data(), p->n * sizeof(uint32_t));
+        if (axis && p->n) std::memcpy(axis, p->axis_host.data(), p->n);
+        if (nclasses) *nclasses = p->nclasses;
+    });
+}
+
+rq_status rq_photons_balance_host(const float* pos, uint32_t n, uint32_t layout, uint32_t* perm, uint8_t* axis) {
+    return guarded([&] {
+        require(n == 0 || (pos && perm && axis), "null argument");
+        require(layout == RQ_PHOTONS_ENCORE_ORDER || layout == RQ_PHOTONS_SPATIAL, "unknown photon layout");
+        std::vector<float> nrm(3 * (size_t)n, 0.0f);
+        PhotonHost h;
+        build_photons(pos, nrm.data(), nullptr, n, layout, h);
+        if (n) {
+            std::memcpy(perm, h.perm.data(), n * sizeof(uint32_t));
+            std::memcpy(axis, h.axis.data(), n);
+        }
+    });
+}
+
+static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint32_t nq, uint32_t k, float max_dist,
+                       uint32_t* idx, float* d2, uint32_t* cnt, float* rgb, uint32_t caustic, hipStream_t s) {
+    require(p != nullptr, "null photon map");
+    require(k >= 1 && k <= kKnnMaxK, "k must be 1..256");
+    require(nq == 0 || (qpos && qnrm), "null query arrays");
+    if (nq == 0) return;
+    rq_device* dev = p->dev;
+    RQ_HIP(hipSetDevice(dev->ordinal));
+    const int grid = knn_grid(nq, dev->num_cus, k);
+    const uint32_t depth = knn_spill_words_per_lane(p->n);
+    uint32_t* scratch = stream_scratch(dev, s, (size_t)grid * 64 * depth * 3 * sizeof(uint32_t));
+    KnnLaunch L{};
+    L.hot = p->hot.as<PhotonHot>();
+    L.power = p->power.as<PhotonPower>();
+    L.perm = p->perm.as<uint32_t>();
+    L.table = p->table.as<uint32_t>();
+    L.table_mask = p->table_mask;
+    L.n = p->n;
+    L.qpos = qpos;
+    L.qnrm = qnrm;
+    L.nq = nq;
+    L.k = k;
+    L.r2 = max_dist * max_dist;  // fl32(maxDistance * maxDistance) (PhotonMap.cpp:89)
+    L.out_idx = idx;
+    L.out_d2 = d2;
+    L.out_cnt = cnt;
+    L.out_rgb = rgb;
+    L.caustic = caustic;
+    L.spill = scratch + 64;
+    L.spill_depth = depth;
+    launch_knn(L, grid, s);
+}
+
+rq_status rq_knn(rq_photons* p, const float* d_qpos, const float* d_qnormal, uint32_t nq, uint32_t k, float max_dist,
+                 uint32_t* d_idx, float* d_d2, uint32_t* d_count, void* stream) {
+    return guarded([&] {
+        require(nq == 0 || (d_idx && d_d2), "null output arrays");
+        knn_common(p, d_qpos, d_qnormal, nq, k, max_dist, d_idx, d_d2, d_count, nullptr, 0, (hipStream_t)stream);
+    });
+}
+
+rq_status rq_radiance_estimate(rq_photons* p, const float* d_qpos, const float* d_qnormal, uint32_t nq, uint32_t k,
+                               float max_dist, uint32_t caustic, float* d_rgb, uint32_t* d_idx, float* d_d2,
+                               uint32_t* d_count, void* stream) {
+    return guarded([&] {
+        require(nq == 0 || (d_rgb && d_idx && d_d2), "null output arrays");
+        knn_common(p, d_qpos, d_qnormal, nq, k, max_dist, d_idx, d_d2, d_count, d_rgb, caustic ? 1u : 0u,
+                   (hipStream_t)stream);
+    });
+}
+
 rq_status rq_host_tree_build(const float* xyz, uint32_t nverts, const uint32_t* tri_idx, uint32_t ntris,
                              const rq_scene_desc* desc, rq_host_tree** out) {
     return guarded([&] {

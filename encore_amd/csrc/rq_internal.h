@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -135,6 +136,67 @@ void build_kd(const float* xyz, uint32_t nverts, const uint32_t* idx, uint32_t n
 void pack_triangles(const float* xyz, const uint32_t* idx, const uint32_t* order, uint32_t ntris,
                     uint32_t predicate, std::vector<TriRec>& out);
 
+// Photon record in balanced (slot) order, 16 bytes: position and
+// meta = splitting axis (bits 0..1) | normal class << 2.
+struct alignas(16) PhotonHot {
+    float x, y, z;
+    uint32_t meta;
+};
+struct alignas(16) PhotonPower {
+    float c[4];
+};
+constexpr uint32_t kClassNever = (1u << 30) - 1;  // matches no query (NaN normal)
+
+struct PhotonHost {
+    std::vector<PhotonHot> hot;
+    std::vector<PhotonPower> power;
+    std::vector<uint32_t> perm;   // slot -> insertion index
+    std::vector<uint8_t> axis;    // slot -> splitting axis
+    std::vector<std::array<uint32_t, 3>> normals;  // class -> canonical normal bits
+    std::vector<uint32_t> table;  // open addressing: (x, y, z, class) per slot, class ~0 = empty
+};
+
+void build_photons(const float* pos, const float* nrm, const float* power, uint32_t n, uint32_t layout,
+                   PhotonHost& out);
+// Hash of a canonical normal (bits of x, y, z with -0 folded to +0); shared
+// by the host table builder and the device lookup.
+__host__ __device__ inline uint32_t normal_hash(uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t h = x * 0x9E3779B1u;
+    h ^= (y + 0x7F4A7C15u) * 0x85EBCA77u;
+    h ^= (z + 0x165667B1u) * 0xC2B2AE3Du;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 13;
+    return h;
+}
+
+struct KnnLaunch {
+    const PhotonHot* hot;
+    const PhotonPower* power;
+    const uint32_t* perm;
+    const uint32_t* table;
+    uint32_t table_mask;
+    uint32_t n;
+    const float* qpos;
+    const float* qnrm;
+    uint32_t nq;
+    uint32_t k;
+    float r2;
+    uint32_t* out_idx;   // nq*k insertion indices (ascending distance), ~0 padding
+    float* out_d2;       // nq*k squared distances, +inf padding
+    uint32_t* out_cnt;   // nq
+    float* out_rgb;      // radiance estimate (nq*3) or null
+    uint32_t caustic;
+    uint32_t* spill;     // per-lane stack spill
+    uint32_t spill_depth;
+};
+
+void launch_knn(const KnnLaunch& L, int grid, hipStream_t s);
+int knn_grid(uint32_t nq, int num_cus, uint32_t k);
+size_t knn_lds_bytes(uint32_t k);
+constexpr uint32_t kKnnMaxK = 256;
+uint32_t knn_spill_words_per_lane(uint32_t n);
+
 // ---------------------------------------------------------------------------
 // Kernel launch parameters (by value).
 // ---------------------------------------------------------------------------
@@ -189,6 +251,14 @@ struct rq_device {
     // stacks after it.  Launches on one stream are ordered, so they can share.
     std::map<hipStream_t, rq::DevBuf> scratch;
     rq::KernelConfig kc;
+};
+
+struct rq_photons {
+    rq_device* dev = nullptr;
+    uint32_t n = 0, nclasses = 0, table_mask = 0;
+    rq::DevBuf hot, power, perm, table;
+    std::vector<uint32_t> perm_host;
+    std::vector<uint8_t> axis_host;
 };
 
 struct rq_scene {

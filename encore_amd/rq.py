@@ -19,6 +19,7 @@ STATUS = {0: "RQ_OK", 1: "RQ_ERR_INVALID_ARG", 2: "RQ_ERR_OOM", 3: "RQ_ERR_DEVIC
 ACCEL = {"bvh": 0, "kd": 1}
 PRED = {"encore": 0, "watertight": 1}
 NO_HIT = 0xFFFFFFFF
+PHOTON_LAYOUT = {"encore": 0, "spatial": 1}
 
 
 class RQError(RuntimeError):
@@ -64,6 +65,13 @@ def load():
     L.rq_intersect_closest.argtypes = [_vp, _vp, _u32, _vp, _vp]
     L.rq_occluded.argtypes = [_vp, _vp, _u32, _vp, _vp]
     L.rq_trace_counts.argtypes = [_vp, _vp, _u32, _u32, _vp, _vp]
+    L.rq_photons_create.argtypes = [_vp, _vp, _vp, _vp, _u32, _u32, ctypes.POINTER(_vp)]
+    L.rq_photons_destroy.argtypes = [_vp]
+    L.rq_photons_destroy.restype = None
+    L.rq_photons_order.argtypes = [_vp, _vp, _vp, _vp]
+    L.rq_photons_balance_host.argtypes = [_vp, _u32, _u32, _vp, _vp]
+    L.rq_knn.argtypes = [_vp, _vp, _vp, _u32, _u32, ctypes.c_float, _vp, _vp, _vp, _vp]
+    L.rq_radiance_estimate.argtypes = [_vp, _vp, _vp, _u32, _u32, ctypes.c_float, _u32, _vp, _vp, _vp, _vp, _vp]
     L.rq_debug_fail_alloc_after.argtypes = [ctypes.c_int64]
     L.rq_debug_fail_alloc_after.restype = None
     L.rq_host_tree_build.argtypes = [_vp, _u32, _vp, _u32, ctypes.POINTER(SceneDesc), ctypes.POINTER(_vp)]
@@ -72,7 +80,9 @@ def load():
     L.rq_host_tree_destroy.argtypes = [_vp]
     L.rq_host_tree_destroy.restype = None
     for name in ("rq_device_open", "rq_scene_create", "rq_scene_stats", "rq_intersect_closest", "rq_occluded",
-                 "rq_host_tree_build", "rq_host_tree_info", "rq_host_tree_copy", "rq_trace_counts"):
+                 "rq_host_tree_build", "rq_host_tree_info", "rq_host_tree_copy", "rq_trace_counts",
+                 "rq_photons_create", "rq_photons_order", "rq_photons_balance_host", "rq_knn",
+                 "rq_radiance_estimate"):
         getattr(L, name).restype = ctypes.c_int
     _LIB = L
     return L
@@ -169,6 +179,57 @@ class Scene:
     def occluded(self, rays, out, n=None, stream=None):
         n = rays.shape[0] if n is None else n
         check(load().rq_occluded(self.h, _ptr(rays), n, _ptr(out), _stream(stream)))
+
+
+class PhotonMap:
+    """ENCORE's balanced photon map resident in HBM."""
+
+    def __init__(self, device, pos, normal, power=None, layout="spatial"):
+        L = load()
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        normal = np.ascontiguousarray(normal, np.float32).reshape(-1, 3)
+        pw = None if power is None else np.ascontiguousarray(power, np.float32).reshape(-1, 3)
+        h = _vp()
+        check(L.rq_photons_create(device.h, pos.ctypes.data, normal.ctypes.data,
+                                  None if pw is None else pw.ctypes.data, len(pos), PHOTON_LAYOUT[layout],
+                                  ctypes.byref(h)))
+        self.h = h
+        self.n = len(pos)
+
+    def destroy(self):
+        if getattr(self, "h", None):
+            load().rq_photons_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.destroy()
+
+    def order(self):
+        perm = np.zeros(max(1, self.n), np.uint32)
+        axis = np.zeros(max(1, self.n), np.uint8)
+        nc = ctypes.c_uint32()
+        check(load().rq_photons_order(self.h, perm.ctypes.data, axis.ctypes.data, ctypes.byref(nc)))
+        return perm[: self.n], axis[: self.n], nc.value
+
+    def knn(self, qpos, qnrm, k, max_dist, idx, d2, count=None, stream=None):
+        """qpos, qnrm: (nq,3) float32 device tensors; idx (nq,k) int32, d2 (nq,k) float32, count (nq,) int32."""
+        check(load().rq_knn(self.h, _ptr(qpos), _ptr(qnrm), qpos.shape[0], k, float(max_dist), _ptr(idx), _ptr(d2),
+                            _ptr(count), _stream(stream)))
+
+    def radiance(self, qpos, qnrm, k, max_dist, rgb, idx, d2, count=None, caustic=False, stream=None):
+        check(load().rq_radiance_estimate(self.h, _ptr(qpos), _ptr(qnrm), qpos.shape[0], k, float(max_dist),
+                                          int(caustic), _ptr(rgb), _ptr(idx), _ptr(d2), _ptr(count),
+                                          _stream(stream)))
+
+
+def photon_balance_host(pos, layout="encore"):
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    n = len(pos)
+    perm = np.zeros(max(1, n), np.uint32)
+    axis = np.zeros(max(1, n), np.uint8)
+    check(load().rq_photons_balance_host(pos.ctypes.data, n, PHOTON_LAYOUT[layout], perm.ctypes.data,
+                                         axis.ctypes.data))
+    return perm[:n], axis[:n]
 
 
 BVH_NODE_DTYPE = np.dtype([("x", "<f4", 4), ("y", "<f4", 4), ("z", "<f4", 4), ("ref", "<u4", 2),

@@ -422,3 +422,115 @@ def config3_rays(xyz, idx, primary_rays, primary_hits, count=8388608, nbatches=6
     rays = bounce_rays(p, nrm, primary_rays[ok, 4:7], count, st, 1e-3 * world_scale)
     sizes = lognormal_batches(count, nbatches, Stream(seed + 0x77))
     return rays, sizes
+
+
+# ---------------------------------------------------------------------------
+# Photon mapping workload
+# ---------------------------------------------------------------------------
+def tessellate(xyz, idx, f):
+    """Split every triangle into f*f triangles on a barycentric grid (vertices
+    are not shared between input triangles)."""
+    ii, jj = np.meshgrid(np.arange(f + 1), np.arange(f + 1), indexing="ij")
+    mask = ii + jj <= f
+    ii, jj = ii[mask], jj[mask]
+    lid = -np.ones((f + 1, f + 1), np.int64)
+    lid[ii, jj] = np.arange(len(ii))
+    tri = []
+    for i in range(f):
+        for j in range(f - i):
+            tri.append([lid[i, j], lid[i + 1, j], lid[i, j + 1]])
+            if i + j + 1 < f:
+                tri.append([lid[i + 1, j], lid[i + 1, j + 1], lid[i, j + 1]])
+    tri = np.array(tri, np.int64)
+    v = xyz.astype(np.float64)[idx]  # (T,3,3)
+    w0 = (f - ii - jj)[None, :, None] / f
+    w1 = ii[None, :, None] / f
+    w2 = jj[None, :, None] / f
+    pts = v[:, 0:1, :] * w0 + v[:, 1:2, :] * w1 + v[:, 2:3, :] * w2  # (T, P, 3)
+    P = len(ii)
+    out_idx = (tri[None, :, :] + (np.arange(len(idx)) * P)[:, None, None]).reshape(-1, 3)
+    return pts.reshape(-1, 3).astype(np.float32), out_idx.astype(np.uint32)
+
+
+def config4_scene(scale=1.0 / 20.0, tess=32, seed=SEED_BASE + 1):
+    """The Cornell box of config 1 at world scale `scale` with each triangle
+    split tess x tess (34 * 1024 = 34,816 triangles).  At 1/20 the box is
+    5 units wide and a 0.05 gather radius holds about a hundred photons,
+    while tessellated edges (0.04-0.3) mostly stay clear of Cross()'s 1e-3
+    flush, so faces keep well-defined normals."""
+    xyz, idx = cornell(seed, scale=scale)
+    return tessellate(xyz, idx, tess)
+
+
+def normals_of(xyz, idx, prims):
+    """ENCORE's hit normal Normalize(Cross(e1, e2)) (TrianglePrim.cpp:236-238)."""
+    tri = idx[prims]
+    v0, v1, v2 = xyz[tri[:, 0]], xyz[tri[:, 1]], xyz[tri[:, 2]]
+    e1 = (v1 - v0).astype(np.float32)
+    e2 = (v2 - v0).astype(np.float32)
+    return normalize_flush(cross_flush(e1, e2))
+
+
+def trace_photons(xyz, idx, trace, count, seed=SEED_BASE + 4, scale=1.0 / 20.0, light_frac=0.3, bounces=3,
+                  albedo=0.7, batch=262144):
+    """Photons from a square light centred under the ceiling (side light_frac of
+    the box, emitting downward with cosine weighting), stored at every
+    diffuse hit; a photon survives each hit with probability `albedo` and
+    bounces at most `bounces` times.  `trace(rays) -> hits (n,4)` is the
+    closest-hit query.  Returns positions, normals, powers (count x 3)."""
+    st = Stream(seed)
+    half = 50.0 * scale
+    side = 2 * half * light_frac
+    pos_out, nrm_out, pow_out = [], [], []
+    stored = 0
+    emitted = 0
+    while stored < count:
+        n = batch
+        r = st.draw(2 * n).reshape(n, 2)
+        o = np.empty((n, 3))
+        o[:, 0] = (r[:, 0] - 0.5) * side
+        o[:, 1] = half * 0.999
+        o[:, 2] = (r[:, 1] - 0.5) * side
+        d = cosine_dirs(np.tile([[0.0, -1.0, 0.0]], (n, 1)), st)
+        rays = np.zeros((n, 8), np.float32)
+        rays[:, 0:3] = o
+        rays[:, 4:7] = d
+        rays[:, 7] = np.inf
+        power = np.full((n, 3), 1.0, np.float32)
+        emitted += n
+        for _ in range(bounces + 1):
+            hits = trace(rays)
+            ok, p, nrm = hit_points(xyz, idx, hits)
+            prims = np.ascontiguousarray(hits[:, 3]).view(np.uint32)[ok]
+            good = ~np.isnan(nrm).any(axis=1)
+            pos_out.append(p[good])
+            nrm_out.append(nrm[good])
+            pow_out.append(power[ok][good])
+            stored += int(good.sum())
+            surv = st.draw(len(p)) < albedo
+            keep = good & surv
+            if not keep.any() or stored >= count:
+                break
+            N = nrm[keep].astype(np.float64)
+            D_in = rays[ok, 4:7][keep].astype(np.float64)
+            flip = np.sum(N * D_in, axis=1) > 0
+            N[flip] = -N[flip]
+            nd = cosine_dirs(N, st)
+            rays = np.zeros((len(nd), 8), np.float32)
+            rays[:, 0:3] = (p[keep].astype(np.float64) + nd * 0.001).astype(np.float32)
+            rays[:, 4:7] = nd.astype(np.float32)
+            rays[:, 7] = np.inf
+            power = (power[ok][keep] * np.float32(albedo)).astype(np.float32)
+    P = np.concatenate(pos_out)[:count]
+    Nn = np.concatenate(nrm_out)[:count]
+    W = np.concatenate(pow_out)[:count] / np.float32(emitted / 1e5)
+    return P.astype(np.float32), Nn.astype(np.float32), W.astype(np.float32)
+
+
+def photon_queries(xyz, idx, trace, res=1000, scale=1.0 / 20.0):
+    """Gather points: the hits of a res x res view from the config-1 camera
+    (scaled), with the hit triangle's normal."""
+    rays = eye_rays(res, res, (0.0, 0.0, 170.0 * scale), (0.0, 0.0, 0.0), 45.0, 1.0, 0.1 * scale)
+    hits = trace(rays)
+    ok, p, nrm = hit_points(xyz, idx, hits)
+    return p.astype(np.float32), nrm.astype(np.float32)

@@ -41,6 +41,7 @@ typedef enum rq_status {
 
 typedef struct rq_device rq_device;
 typedef struct rq_scene rq_scene;
+typedef struct rq_photons rq_photons;
 
 /* A ray: origin, direction and the open parameter interval (tmin, tmax).
  * A triangle hit at t is accepted iff tmin < t < tmax.  ENCORE's eye, shadow
@@ -139,6 +140,56 @@ rq_status rq_intersect_closest(rq_scene* scene, const rq_ray* d_rays, uint32_t n
  * closest-hit query. */
 rq_status rq_occluded(rq_scene* scene, const rq_ray* d_rays, uint32_t n,
                       uint32_t* d_occluded, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Photon maps (PhotonMap.h:28-104).
+ * --------------------------------------------------------------------- */
+
+typedef enum rq_photon_layout {
+    /* ENCORE's balanced tree, photon for photon: node = middle of its index
+     * range, axis from BalanceAndConstruct's bookkeeping box
+     * (PhotonMap.cpp:167-221; that box picks z at nearly every node). */
+    RQ_PHOTONS_ENCORE_ORDER = 0,
+    /* Same implicit layout, axis = longest extent of each range's box.
+     * Gathers the same sets with far fewer visits. */
+    RQ_PHOTONS_SPATIAL = 1
+} rq_photon_layout;
+
+/* Build a balanced photon kd-tree over n photons given in insertion order
+ * (PhotonMap::AddPhoton then BalanceAndConstruct, PhotonMap.cpp:46-53,
+ * 167-221) and upload it.  Host arrays of 3*n floats: position, surface
+ * normal, power (power may be NULL when only k-NN is needed). */
+rq_status rq_photons_create(rq_device* dev, const float* pos, const float* normal, const float* power,
+                            uint32_t n, uint32_t layout, rq_photons** out);
+void rq_photons_destroy(rq_photons* photons);
+/* The balanced order: perm[slot] = insertion index and axis[slot] = the
+ * splitting axis ENCORE stores in the photon (Photon::SetPlane); nclasses =
+ * number of distinct normals.  Any output may be NULL. */
+rq_status rq_photons_order(const rq_photons* photons, uint32_t* perm, uint8_t* axis, uint32_t* nclasses);
+/* Host only (no GPU): the balanced order rq_photons_create would use for n
+ * positions (3*n floats) and a layout: perm[n], axis[n]. */
+rq_status rq_photons_balance_host(const float* pos, uint32_t n, uint32_t layout, uint32_t* perm, uint8_t* axis);
+
+/* k nearest photons of each query (GetNearestNPhotons, PhotonMap.cpp:76-95,
+ * 224-297): photons whose normal equals the query normal (== per component,
+ * so -0 equals +0) with squared distance (x*x)+(y*y)+(z*z) strictly below
+ * fl(max_dist*max_dist); the k nearest of them.  Device arrays: d_qpos and
+ * d_qnormal are 3*nq floats; d_idx and d_d2 receive nq*k entries per query,
+ * nearest first (insertion indices; ~0 and +inf pad short lists); d_count[nq]
+ * (may be NULL) receives the count.  1 <= k <= 256.  Photons tied with the
+ * k-th distance may be chosen differently from ENCORE's heap. */
+rq_status rq_knn(rq_photons* photons, const float* d_qpos, const float* d_qnormal, uint32_t nq, uint32_t k,
+                 float max_dist, uint32_t* d_idx, float* d_d2, uint32_t* d_count, void* stream);
+
+/* Radiance estimate (GetRadianceEstimate, PhotonMap.cpp:121-164) fused with
+ * the k-NN gather: d_rgb[3*nq] = sum of photon power over the gathered set
+ * (caustic = 0), or the cone-weighted sum (r - d)/d times k / (PI r^2) with r
+ * and d the squared distances of the farthest and of each photon and
+ * PI = 3.14159f (caustic = 1).  The gathered set is also written to d_idx,
+ * d_d2, d_count as in rq_knn. */
+rq_status rq_radiance_estimate(rq_photons* photons, const float* d_qpos, const float* d_qnormal, uint32_t nq,
+                               uint32_t k, float max_dist, uint32_t caustic, float* d_rgb, uint32_t* d_idx,
+                               float* d_d2, uint32_t* d_count, void* stream);
 
 /* Traversal work of one query over n rays, counted by an instrumented copy
  * of the same kernel (synchronous): counts[0] = rays, counts[1] = nodes
