@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", choices=["plain", "batched"], default="plain")
+    ap.add_argument("--mode", choices=["plain", "batched"], default="batched")
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--rays", type=int, default=8388608)
     ap.add_argument("--batches", type=int, default=64)
@@ -99,7 +99,7 @@ def main():
     nb = len(sizes)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * nb)]
 
-    def step(timed=False):
+    def plain_step(timed=False):
         for b in range(nb):
             o, n = int(offs[b]), int(sizes[b])
             if timed:
@@ -107,25 +107,54 @@ def main():
             scene.intersect(d_rays[o:o + n], d_hits[o:o + n], stream=stream)
             if timed:
                 ev[2 * b + 1].record(stream)
+        return nb
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    launch_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(timed=True)
+    batch = None
+    tuned = None
+    if args.mode == "batched":
+        batch = rq.Batch(dev)
+        batch.set([(scene, d_rays[int(offs[b]):int(offs[b + 1])], d_hits[int(offs[b]):int(offs[b + 1])])
+                   for b in range(nb)])
+        t = time.time()
+        tuned = batch.autotune(stream=stream)
+        log("autotuned in %.1fs: %s" % (time.time() - t, tuned))
+
+    def batched_step(timed=False):
+        if timed:
+            ev[0].record(stream)
+        batch.launch(graph=True, stream=stream)
+        if timed:
+            ev[1].record(stream)
+        return 1
+
+    def run(step, steps, warmup):
+        for _ in range(warmup):
+            step()
         torch.cuda.synchronize()
-        launch_ms += [ev[2 * b].elapsed_time(ev[2 * b + 1]) for b in range(nb)]
-    t1 = time.perf_counter()
+        launch_ms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            nl = step(timed=True)
+            torch.cuda.synchronize()
+            launch_ms += [ev[2 * b].elapsed_time(ev[2 * b + 1]) for b in range(nl)]
+        secs = time.perf_counter() - t0
+        return secs, launch_ms
+
+    step = batched_step if args.mode == "batched" else plain_step
+    secs, launch_ms = run(step, args.steps, args.warmup)
     # per-step wall time includes the host synchronisation used to read the
     # per-launch events; it is the conservative figure.
-    secs = t1 - t0
     ms_per_step = secs * 1000.0 / args.steps
     value = N * args.steps / secs / 1e6
     kernel_ms_total = float(np.sum(launch_ms)) / args.steps
-    log("timed: %.2f ms/step, %.1f Mrays/s; sum of launch times %.2f ms/step" % (ms_per_step, value,
-                                                                               kernel_ms_total))
+    launches_per_step = len(launch_ms) // args.steps
+    log("timed (%s): %.2f ms/step, %.1f Mrays/s; launch time %.2f ms/step over %d launches" % (
+        args.mode, ms_per_step, value, kernel_ms_total, launches_per_step))
+    plain_value = None
+    if args.mode == "batched":
+        psecs, _ = run(plain_step, max(1, min(args.steps, 3)), 1)
+        plain_value = N * max(1, min(args.steps, 3)) / psecs / 1e6
+        log("plain per-batch launches: %.1f Mrays/s" % plain_value)
     hits = d_hits.cpu().numpy()
 
     exact = None
@@ -171,8 +200,11 @@ def main():
                 traffic = json.load(f).get("hbm_bytes_per_launch")
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "alg_bytes_per_launch": int(alg_bytes / nb), "nodes_per_ray": round(n_nodes / n_rays, 2),
-                "tris_per_ray": round(n_tris / n_rays, 2), "avg_launch_ms": round(kernel_ms_total / nb, 4)}
+                "alg_bytes_per_launch": int(alg_bytes / launches_per_step), "nodes_per_ray": round(n_nodes / n_rays, 2),
+                "tris_per_ray": round(n_tris / n_rays, 2),
+                "avg_launch_ms": round(kernel_ms_total / launches_per_step, 4),
+                "kernel": "bvh_trace<PredEncore, closest, stack %s, batched>" % (tuned["stack"] if tuned else 16)
+                if args.mode == "batched" else "bvh_trace<PredEncore, closest, stack 16>"}
         log("roofline: %.1f nodes/ray, %.1f tris/ray, %.1f GB/s algorithmic" % (n_nodes / n_rays, n_tris / n_rays,
                                                                              achieved))
 
@@ -262,14 +294,17 @@ def main():
         "dtype": "f32",
         "data": "synthetic: seeded displaced icosphere and seeded bounce rays",
         "config": {"workload": "8,388,608 incoherent bounce rays in %d lognormal batches on a displaced icosphere "
-                               "(level %d) + ground, %d triangles; BVH2 SAH, ENCORE predicate; %s launch" % (
-                                   nb, args.level, len(idx), "per-batch" if args.mode == "plain" else "batched"),
+                               "(level %d) + ground, %d triangles; BVH2 SAH, ENCORE predicate; %s" % (
+                                   nb, args.level, len(idx), "one launch per batch" if args.mode == "plain" else
+                                   "one persistent hipGraph launch per step, autotuned"),
                    "rays": N, "batches": nb, "triangles": int(len(idx)), "accel": "bvh", "predicate": "encore",
-                   "mode": args.mode},
+                   "mode": args.mode, "autotuned": tuned},
         "roofline": roof,
         "cpu_baseline": cpu,
         "parity": parity,
     }
+    if plain_value is not None:
+        out["plain_per_batch_value"] = round(plain_value, 2)
     if exact:
         out["exact_path"] = exact
     if cpu:
@@ -277,6 +312,8 @@ def main():
         if exact:
             exact["speedup_vs_cpu_all_cores"] = round(exact["value"] / cpu["value"], 1)
     print(json.dumps(out), flush=True)
+    if batch is not None:
+        batch.destroy()
     scene.destroy()
     dev.close()
     return 0

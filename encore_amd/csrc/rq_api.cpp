@@ -64,6 +64,10 @@ static rq_status guarded(F&& f) {
     }
 }
 
+rq_status guard_call(void (*fn)(void*), void* ctx) {
+    return guarded([&] { fn(ctx); });
+}
+
 static void require(bool ok, const char* msg) {
     if (!ok) throw Error(RQ_ERR_INVALID_ARG, msg);
 }
@@ -116,6 +120,7 @@ static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, ui
         L.root_size[k] = sc->root_size[k];
     }
     L.counts = counts;
+    L.fetch_threshold = (uint32_t)dev->kc.fetch_threshold;
     launch_zero_u32(scratch, 1, s);
     if (counts)
         launch_count(L, sc->accel, sc->predicate, kind, grid, s);

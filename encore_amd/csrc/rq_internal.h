@@ -27,6 +27,8 @@ struct Error : std::runtime_error {
 };
 
 void hip_check(hipError_t e, const char* what);
+// Runs fn(ctx) under the C ABI's error convention (status + rq_last_error).
+rq_status guard_call(void (*fn)(void*), void* ctx);
 #define RQ_HIP(call) ::rq::hip_check((call), #call)
 
 // ---------------------------------------------------------------------------
@@ -200,6 +202,19 @@ uint32_t knn_spill_words_per_lane(uint32_t n);
 // ---------------------------------------------------------------------------
 // Kernel launch parameters (by value).
 // ---------------------------------------------------------------------------
+// One item of a batched launch (device copy), 64 bytes.
+struct alignas(64) BatchItemDev {
+    const BvhNode* nodes;
+    const TriRec* tris;
+    const rq_ray* rays;
+    rq_hit* hits;
+    uint32_t* occ;
+    uint32_t begin;  // global index of the item's first ray
+    uint32_t n;
+    uint32_t pad[4];
+};
+static_assert(sizeof(BatchItemDev) == 64, "batch descriptor is 64 bytes");
+
 struct TraceLaunch {
     const void* nodes;       // BvhNode* or KdNodeG*
     const TriRec* tris;
@@ -212,6 +227,9 @@ struct TraceLaunch {
     uint32_t* spill;         // per-lane spill stacks
     uint32_t spill_depth;    // entries per lane
     unsigned long long* counts;  // counting variant: [nodes visited, triangles tested]
+    const BatchItemDev* items;   // batched launch: descriptor table sorted by begin
+    uint32_t nitems;
+    uint32_t fetch_threshold;    // refill idle lanes when fewer than this many are busy
     float root_pos[3], root_size[3];  // kd only
 };
 
@@ -221,12 +239,15 @@ struct KernelConfig {
     int block = 256;
     int blocks_per_cu = 0;   // 0 = occupancy maximum
     int stack = 16;          // LDS ring entries per lane
-    int fetch_threshold = 0; // dynamic fetch when fewer active lanes (0 = default)
+    int fetch_threshold = 20; // refill idle lanes when fewer than this many of 64 are busy
 };
 
 // Launchers (traverse.hip).
 void launch_bvh(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, const KernelConfig& kc,
                 hipStream_t s);
+void launch_bvh_batched(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, const KernelConfig& kc,
+                        hipStream_t s);
+int occupancy_bvh_batched(uint32_t predicate, TraceKind kind, int stack);
 void launch_kd(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, const KernelConfig& kc,
                hipStream_t s);
 void launch_count(const TraceLaunch& L, uint32_t accel, uint32_t predicate, TraceKind kind, int grid,

@@ -238,7 +238,21 @@ __device__ __forceinline__ bool box2(const RayState& r, const float4 nx, const f
     return tn0 <= tf0;
 }
 
-template <class Pred, bool ANY, int S, bool COUNT = false>
+// Item of a batched launch whose descriptor holds global ray index `id`:
+// the last item with begin <= id (items sorted by begin).
+__device__ __forceinline__ uint32_t find_item(const BatchItemDev* items, uint32_t nitems, uint32_t id) {
+    uint32_t lo = 0, hi = nitems;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (items[mid].begin <= id)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <class Pred, bool ANY, int S, bool COUNT = false, bool BATCHED = false>
 __global__ __launch_bounds__(256) void bvh_trace(TraceLaunch L) {
     __shared__ uint32_t lds_stack[256 / kWave][S][kWave];
     const int lane = lane_id();
@@ -247,21 +261,37 @@ __global__ __launch_bounds__(256) void bvh_trace(TraceLaunch L) {
     RingStack<S, 1> stk;
     stk.lds = &lds_stack[wave][0][lane];
     stk.spill = L.spill + (size_t)gwave * L.spill_depth * kWave + lane;
+    // Per-lane scene and buffers: uniform in a plain launch, per item in a
+    // batched one.
     const BvhNode* nodes = static_cast<const BvhNode*>(L.nodes);
     const TriRec* tris = L.tris;
+    rq_hit* hout = L.hits;
+    uint32_t* oout = L.occluded;
+    uint32_t local = 0;
 
     RayState r;
     float tmin = 0, tbest = 0, bu = 0, bv = 0;
     uint32_t bprim = kNone;
     uint32_t ray = kNone;
     uint32_t cur = kRefSentinel, pend = kNone;
-    const uint32_t thresh = 20;
+    const uint32_t thresh = L.fetch_threshold;
     uint32_t n_nodes = 0, n_tris = 0;  // COUNT only
 
     for (;;) {
         ray = fetch_work(ray, L.counter, L.n);
         if (ray != kNone && ray != kDone && cur == kRefSentinel && pend == kNone) {
-            init_ray(r, L.rays, ray, tmin, tbest);
+            const rq_ray* rin = L.rays;
+            local = ray;
+            if (BATCHED) {
+                const BatchItemDev* it = L.items + find_item(L.items, L.nitems, ray);
+                nodes = it->nodes;
+                tris = it->tris;
+                rin = it->rays;
+                hout = it->hits;
+                oout = it->occ;
+                local = ray - it->begin;
+            }
+            init_ray(r, rin, local, tmin, tbest);
             Pred::prepare(r);
             bu = bv = 0.0f;
             bprim = kNone;
@@ -341,14 +371,11 @@ __global__ __launch_bounds__(256) void bvh_trace(TraceLaunch L) {
         }
         if (ray != kNone && ray != kDone && cur == kRefSentinel && pend == kNone) {
             if (ANY) {
-                L.occluded[ray] = (bprim != kNone) ? 1u : 0u;
+                oout[local] = (bprim != kNone) ? 1u : 0u;
             } else {
-                rq_hit h;
-                h.t = tbest;
-                h.u = bprim != kNone ? bu : 0.0f;
-                h.v = bprim != kNone ? bv : 0.0f;
-                h.prim = bprim;
-                *reinterpret_cast<float4*>(&L.hits[ray]) = make_float4(h.t, h.u, h.v, __uint_as_float(h.prim));
+                const bool has = bprim != kNone;
+                *reinterpret_cast<float4*>(&hout[local]) =
+                    make_float4(tbest, has ? bu : 0.0f, has ? bv : 0.0f, __uint_as_float(bprim));
             }
             if (COUNT) {
                 atomicAdd(&L.counts[0], (unsigned long long)n_nodes);
@@ -525,6 +552,55 @@ static void launch_bvh_t(const TraceLaunch& L, int grid, const KernelConfig& kc,
             hipLaunchKernelGGL((dev::bvh_trace<Pred, ANY, 16>), dim3(grid), dim3(256), 0, s, L);
             break;
     }
+}
+
+template <class Pred, bool ANY>
+static void launch_bvh_batched_t(const TraceLaunch& L, int grid, const KernelConfig& kc, hipStream_t s) {
+    switch (kc.stack) {
+        case 8:
+            hipLaunchKernelGGL((dev::bvh_trace<Pred, ANY, 8, false, true>), dim3(grid), dim3(256), 0, s, L);
+            break;
+        case 32:
+            hipLaunchKernelGGL((dev::bvh_trace<Pred, ANY, 32, false, true>), dim3(grid), dim3(256), 0, s, L);
+            break;
+        default:
+            hipLaunchKernelGGL((dev::bvh_trace<Pred, ANY, 16, false, true>), dim3(grid), dim3(256), 0, s, L);
+            break;
+    }
+}
+
+void launch_bvh_batched(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, const KernelConfig& kc,
+                        hipStream_t s) {
+    const bool any = kind == TraceKind::Any;
+    if (predicate == RQ_PRED_WATERTIGHT) {
+        if (any)
+            launch_bvh_batched_t<dev::PredWatertight, true>(L, grid, kc, s);
+        else
+            launch_bvh_batched_t<dev::PredWatertight, false>(L, grid, kc, s);
+    } else {
+        if (any)
+            launch_bvh_batched_t<dev::PredEncore, true>(L, grid, kc, s);
+        else
+            launch_bvh_batched_t<dev::PredEncore, false>(L, grid, kc, s);
+    }
+    RQ_HIP(hipGetLastError());
+}
+
+int occupancy_bvh_batched(uint32_t predicate, TraceKind kind, int stack) {
+    const bool any = kind == TraceKind::Any;
+    const void* fn;
+#define RQ_PICKB(P, A)                                                                        \
+    (stack == 8    ? (const void*)dev::bvh_trace<P, A, 8, false, true>                       \
+     : stack == 32 ? (const void*)dev::bvh_trace<P, A, 32, false, true>                      \
+                   : (const void*)dev::bvh_trace<P, A, 16, false, true>)
+    if (predicate == RQ_PRED_WATERTIGHT)
+        fn = any ? RQ_PICKB(dev::PredWatertight, true) : RQ_PICKB(dev::PredWatertight, false);
+    else
+        fn = any ? RQ_PICKB(dev::PredEncore, true) : RQ_PICKB(dev::PredEncore, false);
+#undef RQ_PICKB
+    int nb = 0;
+    RQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0));
+    return nb > 0 ? nb : 1;
 }
 
 template <class Pred, bool ANY>

@@ -41,6 +41,16 @@ class SceneInfo(ctypes.Structure):
                 ("device_bytes", ctypes.c_uint64), ("build_seconds", ctypes.c_double)]
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_void_p), ("d_rays", ctypes.c_void_p), ("d_out", ctypes.c_void_p),
+                ("n", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class KernelConfig(ctypes.Structure):
+    _fields_ = [("stack", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32), ("fetch_threshold", ctypes.c_uint32),
+                ("grid", ctypes.c_uint32)]
+
+
 _LIB = None
 _vp = ctypes.c_void_p
 _u32 = ctypes.c_uint32
@@ -65,6 +75,14 @@ def load():
     L.rq_intersect_closest.argtypes = [_vp, _vp, _u32, _vp, _vp]
     L.rq_occluded.argtypes = [_vp, _vp, _u32, _vp, _vp]
     L.rq_trace_counts.argtypes = [_vp, _vp, _u32, _u32, _vp, _vp]
+    L.rq_batch_create.argtypes = [_vp, _u32, ctypes.POINTER(_vp)]
+    L.rq_batch_set.argtypes = [_vp, _vp, _u32]
+    L.rq_batch_launch.argtypes = [_vp, _u32, _vp]
+    L.rq_batch_get_config.argtypes = [_vp, ctypes.POINTER(KernelConfig)]
+    L.rq_batch_set_config.argtypes = [_vp, ctypes.POINTER(KernelConfig)]
+    L.rq_batch_autotune.argtypes = [_vp, _vp, ctypes.POINTER(KernelConfig), ctypes.POINTER(ctypes.c_double)]
+    L.rq_batch_destroy.argtypes = [_vp]
+    L.rq_batch_destroy.restype = None
     L.rq_photons_create.argtypes = [_vp, _vp, _vp, _vp, _u32, _u32, ctypes.POINTER(_vp)]
     L.rq_photons_destroy.argtypes = [_vp]
     L.rq_photons_destroy.restype = None
@@ -82,7 +100,8 @@ def load():
     for name in ("rq_device_open", "rq_scene_create", "rq_scene_stats", "rq_intersect_closest", "rq_occluded",
                  "rq_host_tree_build", "rq_host_tree_info", "rq_host_tree_copy", "rq_trace_counts",
                  "rq_photons_create", "rq_photons_order", "rq_photons_balance_host", "rq_knn",
-                 "rq_radiance_estimate"):
+                 "rq_radiance_estimate", "rq_batch_create", "rq_batch_set", "rq_batch_launch", "rq_batch_get_config",
+                 "rq_batch_set_config", "rq_batch_autotune"):
         getattr(L, name).restype = ctypes.c_int
     _LIB = L
     return L
@@ -179,6 +198,55 @@ class Scene:
     def occluded(self, rays, out, n=None, stream=None):
         n = rays.shape[0] if n is None else n
         check(load().rq_occluded(self.h, _ptr(rays), n, _ptr(out), _stream(stream)))
+
+
+class Batch:
+    """Many ray batches in one persistent launch (optionally a replayed hipGraph)."""
+
+    def __init__(self, device, any_hit=False):
+        L = load()
+        h = _vp()
+        check(L.rq_batch_create(device.h, int(any_hit), ctypes.byref(h)))
+        self.h = h
+        self._keep = None
+
+    def set(self, items):
+        """items: list of (scene, rays tensor, out tensor)."""
+        arr = (BatchItem * max(1, len(items)))()
+        for i, (sc, rays, out) in enumerate(items):
+            arr[i].scene = sc.h
+            arr[i].d_rays = _ptr(rays)
+            arr[i].d_out = _ptr(out)
+            arr[i].n = rays.shape[0]
+        self._keep = items
+        check(load().rq_batch_set(self.h, arr, len(items)))
+
+    def launch(self, graph=False, stream=None):
+        check(load().rq_batch_launch(self.h, int(graph), _stream(stream)))
+
+    def config(self):
+        c = KernelConfig()
+        check(load().rq_batch_get_config(self.h, ctypes.byref(c)))
+        return dict(stack=c.stack, blocks_per_cu=c.blocks_per_cu, fetch_threshold=c.fetch_threshold, grid=c.grid)
+
+    def set_config(self, stack=16, blocks_per_cu=0, fetch_threshold=20):
+        c = KernelConfig(stack, blocks_per_cu, fetch_threshold, 0)
+        check(load().rq_batch_set_config(self.h, ctypes.byref(c)))
+
+    def autotune(self, stream=None):
+        c = KernelConfig()
+        ms = ctypes.c_double()
+        check(load().rq_batch_autotune(self.h, _stream(stream), ctypes.byref(c), ctypes.byref(ms)))
+        return dict(stack=c.stack, blocks_per_cu=c.blocks_per_cu, fetch_threshold=c.fetch_threshold, grid=c.grid,
+                    ms=ms.value)
+
+    def destroy(self):
+        if getattr(self, "h", None):
+            load().rq_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.destroy()
 
 
 class PhotonMap:

@@ -42,6 +42,7 @@ typedef enum rq_status {
 typedef struct rq_device rq_device;
 typedef struct rq_scene rq_scene;
 typedef struct rq_photons rq_photons;
+typedef struct rq_batch rq_batch;
 
 /* A ray: origin, direction and the open parameter interval (tmin, tmax).
  * A triangle hit at t is accepted iff tmin < t < tmax.  ENCORE's eye, shadow
@@ -140,6 +141,43 @@ rq_status rq_intersect_closest(rq_scene* scene, const rq_ray* d_rays, uint32_t n
  * closest-hit query. */
 rq_status rq_occluded(rq_scene* scene, const rq_ray* d_rays, uint32_t n,
                       uint32_t* d_occluded, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Batched launches.  Many ray batches of any sizes, on any resident BVH
+ * scenes sharing one predicate, traced by ONE persistent launch: waves pull
+ * global ray indices from one counter and find the owning batch in a device
+ * descriptor table.  The launch can be captured once into a hipGraph and
+ * replayed.  No counterpart in ENCORE, whose renderers trace one ray per
+ * AccelerationStructure::intersect call.
+ * --------------------------------------------------------------------- */
+typedef struct rq_batch_item {
+    rq_scene* scene;       /* RQ_ACCEL_BVH scene */
+    const rq_ray* d_rays;  /* n rays */
+    void* d_out;           /* n rq_hit (closest) or n uint32_t (any hit) */
+    uint32_t n;
+    uint32_t reserved;
+} rq_batch_item;
+
+typedef struct rq_kernel_config {
+    uint32_t stack;            /* LDS ring entries per lane: 8, 16 or 32 */
+    uint32_t blocks_per_cu;    /* 256-thread workgroups per CU (0 = occupancy limit) */
+    uint32_t fetch_threshold;  /* refill idle lanes when fewer than this many of 64 are busy (0..64) */
+    uint32_t grid;             /* out: workgroups of the last launch */
+} rq_kernel_config;
+
+rq_status rq_batch_create(rq_device* dev, uint32_t any_hit, rq_batch** out);
+/* Replace the batch's items (host array; copied).  Waits for queued work of
+ * the device, uploads the descriptor table and drops a captured graph. */
+rq_status rq_batch_set(rq_batch* batch, const rq_batch_item* items, uint32_t count);
+/* One persistent launch over every item.  use_graph = 1 captures the launch
+ * (counter reset + kernel) into a hipGraph on first use and replays it. */
+rq_status rq_batch_launch(rq_batch* batch, uint32_t use_graph, void* stream);
+rq_status rq_batch_get_config(const rq_batch* batch, rq_kernel_config* cfg);
+rq_status rq_batch_set_config(rq_batch* batch, const rq_kernel_config* cfg);
+/* Time every stack x blocks-per-CU x fetch-threshold candidate on the
+ * batch's current items (outputs are overwritten) and keep the fastest. */
+rq_status rq_batch_autotune(rq_batch* batch, void* stream, rq_kernel_config* chosen, double* best_ms);
+void rq_batch_destroy(rq_batch* batch);
 
 /* ---------------------------------------------------------------------
  * Photon maps (PhotonMap.h:28-104).

@@ -1,0 +1,103 @@
+"""Batched persistent launches (descriptor table, hipGraph replay, autotuner)
+give exactly the per-batch launch results."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from encore_amd import rq, scenes
+from helpers import log
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def two_scenes(device):
+    a = scenes.displaced_sphere(level=4)
+    b = scenes.cornell()
+    sa = rq.Scene(device, *a, accel="bvh")
+    sb = rq.Scene(device, *b, accel="bvh")
+    kda = oracle.KdTree(*a)
+    prim = scenes.config2_primary(cols=160, rows=90)
+    ray_a, _ = scenes.config3_rays(a[0], a[1], prim, kda.intersect(prim), count=30000, nbatches=2)
+    _, _, ray_b = scenes.config1(res=128)
+    yield sa, sb, ray_a, ray_b
+    sa.destroy()
+    sb.destroy()
+
+
+def _plain(scene, rays):
+    d = torch.from_numpy(rays).cuda()
+    out = torch.empty((len(rays), 4), dtype=torch.float32, device="cuda")
+    scene.intersect(d, out)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def test_batched_launch_matches_plain(device, two_scenes):
+    sa, sb, ray_a, ray_b = two_scenes
+    log("case: batched mixed scenes")
+    cuts_a = [0, 1, 4097, 4097, 17000, len(ray_a)]  # includes empty and 1-ray items
+    cuts_b = [0, 5000, len(ray_b)]
+    items, refs = [], []
+    for s, rays, cuts in ((sa, ray_a, cuts_a), (sb, ray_b, cuts_b)):
+        for lo, hi in zip(cuts[:-1], cuts[1:]):
+            r = np.ascontiguousarray(rays[lo:hi])
+            d = torch.from_numpy(r).cuda() if len(r) else torch.empty((0, 8), device="cuda")
+            o = torch.full((max(1, len(r)), 4), -5.0, device="cuda")
+            items.append((s, d, o))
+            refs.append(_plain(s, r) if len(r) else None)
+    b = rq.Batch(device)
+    b.set(items)
+    for graph in (False, True, True, True):
+        for _, _, o in items:
+            o.fill_(-5.0)
+        b.launch(graph=graph)
+        torch.cuda.synchronize()
+        _ = torch.randn(1000, device="cuda").sum().item()  # eager work between replays
+        for (s, d, o), ref in zip(items, refs):
+            if ref is None:
+                continue
+            got = o[: d.shape[0]].cpu().numpy()
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), graph
+    cfg = b.autotune()
+    log("autotuned:", cfg)
+    assert cfg["stack"] in (8, 16, 32) and cfg["ms"] > 0
+    for _, _, o in items:
+        o.fill_(-5.0)
+    b.launch(graph=True)
+    torch.cuda.synchronize()
+    for (s, d, o), ref in zip(items, refs):
+        if ref is not None:
+            assert np.array_equal(o[: d.shape[0]].cpu().numpy().view(np.uint32), ref.view(np.uint32))
+    b.destroy()
+
+
+def test_batched_any_hit(device, two_scenes):
+    sa, _, ray_a, _ = two_scenes
+    log("case: batched any-hit")
+    r = ray_a.copy()
+    r[:, 7] = 50.0
+    d = torch.from_numpy(r).cuda()
+    plain = torch.empty((len(r),), dtype=torch.int32, device="cuda")
+    sa.occluded(d, plain)
+    out = torch.full((len(r),), 9, dtype=torch.int32, device="cuda")
+    b = rq.Batch(device, any_hit=True)
+    b.set([(sa, d[:1000], out[:1000]), (sa, d[1000:], out[1000:])])
+    b.launch(graph=True)
+    torch.cuda.synchronize()
+    assert torch.equal(out, plain)
+    b.destroy()
+
+
+def test_batch_rejects_kd_scene(device):
+    xyz, idx = scenes.cornell()
+    kd = rq.Scene(device, xyz, idx, accel="kd")
+    b = rq.Batch(device)
+    d = torch.zeros((4, 8), device="cuda")
+    o = torch.zeros((4, 4), device="cuda")
+    with pytest.raises(rq.RQError) as e:
+        b.set([(kd, d, o)])
+    assert e.value.status == 4
+    b.destroy()
+    kd.destroy()
