@@ -96,13 +96,15 @@ REF_SRC = "/root/reference/ENCORE/src"
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="ENCORE sources are only present in the dev container")
-def test_drop_in_adapter_compiles_against_encore_headers():
-    """HipAccel (encore_amd/adapter) builds inside ENCORE's tree against its
-    own AccelerationStructure.h, IModel.h and TrianglePrim.h."""
+@pytest.mark.parametrize("src", ["HipAccel.cpp", "HipPhotonMap.cpp"])
+def test_drop_in_adapter_compiles_against_encore_headers(src):
+    """The drop-in adapters (encore_amd/adapter) build inside ENCORE's tree
+    against its own AccelerationStructure.h, IModel.h, TrianglePrim.h,
+    Photon.h and PhotonMap.h."""
     import subprocess
 
     inc = ["-I" + os.path.join(REF_SRC, d) for d in ("math", "scene", "misc", "accelerators", "renderers")]
     cmd = ["g++", "-std=c++11", "-fsyntax-only", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
-           "-I" + os.path.join(ROOT, "include")] + inc + [os.path.join(ROOT, "encore_amd", "adapter", "HipAccel.cpp")]
+           "-I" + os.path.join(ROOT, "include")] + inc + [os.path.join(ROOT, "encore_amd", "adapter", src)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]
