@@ -185,7 +185,8 @@ rq_status rq_device_open(int ordinal, rq_device** out) {
         d->max_waves_per_cu = p.maxThreadsPerMultiProcessor / 64;
         if (const char* e = std::getenv("RQ_STACK")) d->kc.stack = std::atoi(e);
         if (const char* e = std::getenv("RQ_BLOCKS_PER_CU")) d->kc.blocks_per_cu = std::atoi(e);
-        if (d->kc.stack != 8 && d->kc.stack != 16 && d->kc.stack != 32) d->kc.stack = 16;
+        if (const char* e = std::getenv("RQ_FETCH_THRESHOLD")) d->kc.fetch_threshold = std::atoi(e);
+        if (d->kc.stack != 8 && d->kc.stack != 16 && d->kc.stack != 32) d->kc.stack = 8;
         *out = d.release();
     });
 }

@@ -238,8 +238,8 @@ enum class TraceKind { Closest = 0, Any = 1 };
 struct KernelConfig {
     int block = 256;
     int blocks_per_cu = 0;   // 0 = occupancy maximum
-    int stack = 16;          // LDS ring entries per lane
-    int fetch_threshold = 20; // refill idle lanes when fewer than this many of 64 are busy
+    int stack = 8;           // LDS ring entries per lane
+    int fetch_threshold = 16; // refill idle lanes when fewer than this many of 64 are busy
 };
 
 // Launchers (traverse.hip).

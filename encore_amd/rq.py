@@ -51,6 +51,23 @@ class KernelConfig(ctypes.Structure):
                 ("grid", ctypes.c_uint32)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_uint32), ("plain", ctypes.c_uint32), ("max_rays_per_launch", ctypes.c_uint32),
+                ("max_requests_per_launch", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64),
+                ("hbm_reserve_bytes", ctypes.c_uint64), ("stack", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
+                ("fetch_threshold", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double), ("launch_id", ctypes.c_uint64),
+                ("rays_in_launch", ctypes.c_uint32), ("requests_in_launch", ctypes.c_uint32)]
+
+
+class ServerCounters(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_uint64) for f in ("launches", "requests_done", "rays_done", "queued", "admitted",
+                                                "arena_bytes", "arena_used")]
+
+
 _LIB = None
 _vp = ctypes.c_void_p
 _u32 = ctypes.c_uint32
@@ -83,6 +100,14 @@ def load():
     L.rq_batch_autotune.argtypes = [_vp, _vp, ctypes.POINTER(KernelConfig), ctypes.POINTER(ctypes.c_double)]
     L.rq_batch_destroy.argtypes = [_vp]
     L.rq_batch_destroy.restype = None
+    L.rq_server_create.argtypes = [_vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]
+    L.rq_server_submit.argtypes = [_vp, _vp, _vp, _u32, _u32, _vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.rq_server_wait.argtypes = [_vp, ctypes.c_uint64, _u32, ctypes.POINTER(RequestInfo)]
+    L.rq_server_poll.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]
+    L.rq_server_cancel.argtypes = [_vp, ctypes.c_uint64]
+    L.rq_server_stats.argtypes = [_vp, ctypes.POINTER(ServerCounters)]
+    L.rq_server_destroy.argtypes = [_vp]
+    L.rq_server_destroy.restype = None
     L.rq_photons_create.argtypes = [_vp, _vp, _vp, _vp, _u32, _u32, ctypes.POINTER(_vp)]
     L.rq_photons_destroy.argtypes = [_vp]
     L.rq_photons_destroy.restype = None
@@ -101,7 +126,8 @@ def load():
                  "rq_host_tree_build", "rq_host_tree_info", "rq_host_tree_copy", "rq_trace_counts",
                  "rq_photons_create", "rq_photons_order", "rq_photons_balance_host", "rq_knn",
                  "rq_radiance_estimate", "rq_batch_create", "rq_batch_set", "rq_batch_launch", "rq_batch_get_config",
-                 "rq_batch_set_config", "rq_batch_autotune"):
+                 "rq_batch_set_config", "rq_batch_autotune", "rq_server_create", "rq_server_submit", "rq_server_wait",
+                 "rq_server_poll", "rq_server_cancel", "rq_server_stats"):
         getattr(L, name).restype = ctypes.c_int
     _LIB = L
     return L
@@ -244,6 +270,61 @@ class Batch:
         if getattr(self, "h", None):
             load().rq_batch_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        self.destroy()
+
+
+class Server:
+    """Continuous-batching request server over resident scenes."""
+
+    def __init__(self, device, slots=0, plain=False, hbm_budget=0, hbm_reserve=0, max_rays_per_launch=0,
+                 max_requests_per_launch=0, stack=0, blocks_per_cu=0, fetch_threshold=0):
+        c = ServerConfig(slots, int(plain), max_rays_per_launch, max_requests_per_launch, hbm_budget, hbm_reserve,
+                         stack, blocks_per_cu, fetch_threshold, 0)
+        h = _vp()
+        check(load().rq_server_create(device.h, ctypes.byref(c), ctypes.byref(h)))
+        self.h = h
+        self._live = {}
+
+    def submit(self, scene, rays, out, any_hit=False):
+        """rays: host tensor/array (n,8) float32; out: host (n,4) float32 or (n,) int32.
+        Both must stay alive until wait(); pinned tensors overlap copies."""
+        t = ctypes.c_uint64()
+        n = rays.shape[0]
+        rp = rays.data_ptr() if hasattr(rays, "data_ptr") else rays.ctypes.data
+        op = out.data_ptr() if hasattr(out, "data_ptr") else out.ctypes.data
+        check(load().rq_server_submit(self.h, scene.h, rp, n, int(any_hit), op, ctypes.byref(t)))
+        self._live[t.value] = (rays, out)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=0xFFFFFFFF):
+        info = RequestInfo()
+        st = load().rq_server_wait(self.h, ticket, timeout_ms, ctypes.byref(info))
+        if st != 6:  # a timeout keeps the ticket alive
+            self._live.pop(ticket, None)
+        check(st)
+        return dict(latency_ms=info.latency_ms, queue_ms=info.queue_ms, launch_id=info.launch_id,
+                    rays_in_launch=info.rays_in_launch, requests_in_launch=info.requests_in_launch)
+
+    def poll(self, ticket):
+        d = ctypes.c_int()
+        check(load().rq_server_poll(self.h, ticket, ctypes.byref(d)))
+        return bool(d.value)
+
+    def cancel(self, ticket):
+        check(load().rq_server_cancel(self.h, ticket))
+
+    def stats(self):
+        c = ServerCounters()
+        check(load().rq_server_stats(self.h, ctypes.byref(c)))
+        return {f: getattr(c, f) for f, _ in ServerCounters._fields_}
+
+    def destroy(self):
+        if getattr(self, "h", None):
+            load().rq_server_destroy(self.h)
+            self.h = None
+            self._live.clear()
 
     def __del__(self):
         self.destroy()

@@ -43,6 +43,7 @@ typedef struct rq_device rq_device;
 typedef struct rq_scene rq_scene;
 typedef struct rq_photons rq_photons;
 typedef struct rq_batch rq_batch;
+typedef struct rq_server rq_server;
 
 /* A ray: origin, direction and the open parameter interval (tmin, tmax).
  * A triangle hit at t is accepted iff tmin < t < tmax.  ENCORE's eye, shadow
@@ -178,6 +179,58 @@ rq_status rq_batch_set_config(rq_batch* batch, const rq_kernel_config* cfg);
  * batch's current items (outputs are overwritten) and keep the fastest. */
 rq_status rq_batch_autotune(rq_batch* batch, void* stream, rq_kernel_config* chosen, double* best_ms);
 void rq_batch_destroy(rq_batch* batch);
+
+/* ---------------------------------------------------------------------
+ * Serving.  Scenes stay resident; clients submit ray batches from host
+ * memory at any time.  A scheduler thread admits requests in arrival order
+ * while a device arena (HBM budget) can hold their rays and outputs, and
+ * keeps `slots` launches in flight: each free slot takes every admitted
+ * request (of one kind) and runs copy-in, ONE persistent batched launch and
+ * copy-out on its own stream, so requests arriving meanwhile overlap the
+ * running launch (continuous batching).  A result never depends on the mix.
+ * plain = 1 gives the per-request baseline (one request per launch, one
+ * slot, the rq_intersect_closest kernel).  No counterpart in ENCORE.
+ * --------------------------------------------------------------------- */
+typedef struct rq_server_config {
+    uint32_t slots;                    /* launches in flight (0 = 3) */
+    uint32_t plain;                    /* 1 = one request per launch */
+    uint32_t max_rays_per_launch;      /* 0 = 2^24 */
+    uint32_t max_requests_per_launch;  /* 0 = 1024 */
+    uint64_t hbm_budget_bytes;         /* request arena (0 = a quarter of free HBM) */
+    uint64_t hbm_reserve_bytes;        /* HBM the arena must leave free */
+    uint32_t stack, blocks_per_cu, fetch_threshold;  /* kernel config (0 = device default) */
+    uint32_t reserved;
+} rq_server_config;
+
+typedef struct rq_request_info {
+    double latency_ms;            /* submit -> results in host memory */
+    double queue_ms;              /* submit -> launch issued */
+    uint64_t launch_id;
+    uint32_t rays_in_launch;
+    uint32_t requests_in_launch;
+} rq_request_info;
+
+typedef struct rq_server_stats {
+    uint64_t launches, requests_done, rays_done, queued, admitted, arena_bytes, arena_used;
+} rq_server_counters;
+
+rq_status rq_server_create(rq_device* dev, const rq_server_config* cfg, rq_server** out);
+/* Queue n rays on a resident BVH scene.  h_rays (n rq_ray) and h_out (n
+ * rq_hit, or n uint32_t for any_hit) must stay valid until the request
+ * completes; pinned host memory (hipHostMalloc) lets the copies overlap
+ * kernels.  RQ_ERR_CAPACITY if the request can never fit the arena. */
+rq_status rq_server_submit(rq_server* server, rq_scene* scene, const rq_ray* h_rays, uint32_t n,
+                           uint32_t any_hit, void* h_out, uint64_t* ticket);
+/* Wait for a request (timeout_ms = 0xFFFFFFFF: forever) and release its
+ * ticket.  RQ_ERR_TIMEOUT leaves the ticket valid; RQ_ERR_CANCELLED after
+ * rq_server_cancel. */
+rq_status rq_server_wait(rq_server* server, uint64_t ticket, uint32_t timeout_ms, rq_request_info* info);
+rq_status rq_server_poll(rq_server* server, uint64_t ticket, int* done);
+/* Cancel a request that has not been launched yet. */
+rq_status rq_server_cancel(rq_server* server, uint64_t ticket);
+rq_status rq_server_stats(rq_server* server, rq_server_counters* stats);
+/* Cancels queued requests, waits for launched ones, frees the arena. */
+void rq_server_destroy(rq_server* server);
 
 /* ---------------------------------------------------------------------
  * Photon maps (PhotonMap.h:28-104).
