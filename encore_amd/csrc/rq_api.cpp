@@ -226,7 +226,8 @@ rq_status rq_scene_create(rq_device* dev, const float* xyz, uint32_t nverts, con
             vertex_bounds(xyz, nverts, dd, bmin, bmax);
             KdHost h;
             build_kd(xyz, nverts, tri_idx, ntris, bmin, bmax, h);
-            pack_triangles(xyz, tri_idx, nullptr, ntris, dd.predicate, tris);
+            // One record per leaf reference, in leaf order.
+            pack_triangles(xyz, tri_idx, h.leaf_prims.data(), (uint32_t)h.leaf_prims.size(), dd.predicate, tris);
             sc->stats = h.stats;
             for (int k = 0; k < 3; ++k) {
                 sc->root_pos[k] = h.root_pos[k];
@@ -234,8 +235,7 @@ rq_status rq_scene_create(rq_device* dev, const float* xyz, uint32_t nverts, con
             }
             sc->nodes = DevBuf(h.nodes.size() * sizeof(KdNodeG));
             sc->nodes.upload(h.nodes.data(), sc->nodes.bytes);
-            sc->leaf_prims = DevBuf(std::max<size_t>(1, h.leaf_prims.size()) * sizeof(uint32_t));
-            if (!h.leaf_prims.empty()) sc->leaf_prims.upload(h.leaf_prims.data(), h.leaf_prims.size() * 4);
+
         }
         sc->tris = DevBuf(tris.size() * sizeof(TriRec));
         sc->tris.upload(tris.data(), sc->tris.bytes);

@@ -419,6 +419,12 @@ __device__ __forceinline__ bool root_clip(const RayState& r, const float* pos, c
     return true;
 }
 
+// Lanes run a state machine -- one node step, or up to four triangle tests of
+// the current leaf, per iteration -- and idle lanes are refilled from the
+// counter whenever fewer than fetch_threshold lanes of the wave are busy, so
+// rays of very different cost (a few to thousands of tests) keep the SIMD
+// full.  Leaf triangles are stored in leaf order (one record per reference),
+// so a leaf's records are contiguous and need no index load.
 template <class Pred, bool ANY, int S, bool COUNT = false>
 __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
     __shared__ uint32_t lds_stack[256 / kWave][S * 3][kWave];
@@ -430,28 +436,89 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
     stk.spill = L.spill + (size_t)gwave * L.spill_depth * 3 * kWave + lane;
     const KdNodeG* nodes = static_cast<const KdNodeG*>(L.nodes);
     const TriRec* tris = L.tris;
-    const uint32_t* lp = L.leaf_prims;
+    const float kInf = __int_as_float(0x7f800000);
+    const uint32_t thresh = L.fetch_threshold;
+
+    RayState r;
+    uint32_t ray = kNone;
+    bool active = false;
+    float rtmax = 0, best = 0, bu = 0, bv = 0, tmin = 0, tmax = 0;
+    uint32_t bprim = kNone, cur = 0, lpos = 0, lend = 0;
+    uint32_t n_nodes = 0, n_tris = 0;  // COUNT only
 
     for (;;) {
-        // One ray per lane per round; the kd walk has no postponement, so
-        // lanes re-fetch whenever their ray ends.
-        const uint32_t ray = fetch_work(kNone, L.counter, L.n);
-        if (__ballot(ray != kDone) == 0) break;
-        if (ray != kDone) {
-            RayState r;
-            float rtmin, rtmax;
+        ray = fetch_work(ray, L.counter, L.n);
+        if (ray != kNone && ray != kDone && !active) {
+            float rtmin;
             init_ray(r, L.rays, ray, rtmin, rtmax);
             Pred::prepare(r);
-            float best = __int_as_float(0x7f800000);  // HitInfo.hitTime = INFINITY
-            float bu = 0, bv = 0;
-            uint32_t bprim = kNone;
-            float tmin = 0, tmax = 0;
-            bool go = root_clip(r, L.root_pos, L.root_size, tmin, tmax);
+            best = kInf;  // HitInfo.hitTime = INFINITY (Kdtree.cpp:685)
+            bu = bv = 0.0f;
+            bprim = kNone;
+            cur = 0;
+            lpos = lend = 0;
             stk.reset();
-            uint32_t cur = 0;
-            uint32_t n_nodes = 0, n_tris = 0;  // COUNT only
-            while (go) {
-                if (best < tmin) break;
+            active = root_clip(r, L.root_pos, L.root_size, tmin, tmax);
+            if (!active) {  // the ray misses the scene box
+                if (ANY)
+                    L.occluded[ray] = 0u;
+                else
+                    *reinterpret_cast<float4*>(&L.hits[ray]) = make_float4(rtmax, 0.0f, 0.0f, __uint_as_float(kNone));
+                ray = kNone;
+            }
+        }
+        if (__ballot(ray != kDone) == 0) break;
+
+        while (active) {
+            bool finish = false;
+            if (lpos < lend) {
+                // Up to four triangles of the current leaf, in leaf order.
+                const uint32_t cnt = min(4u, lend - lpos);
+                float4 ta[4], tb[4], tc[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if ((uint32_t)i < cnt) {
+                        const TriRec* tp = tris + lpos + i;
+                        ta[i] = ldg4(tp->a);
+                        tb[i] = ldg4(tp->b);
+                        tc[i] = ldg4(tp->c);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if ((uint32_t)i < cnt && !finish) {
+                        if (COUNT) ++n_tris;
+                        float t, u, v;
+                        // A closer hit replaces the record (strict <); an empty
+                        // record (hitTime FLT_MAX) replaces the initial +inf
+                        // (HitInfo.cpp:8-18, Kdtree.cpp:757-765).
+                        if (Pred::test(r, ta[i], tb[i], tc[i], 0.0f, t, u, v)) {
+                            if (t < best) {
+                                best = t;
+                                bu = u;
+                                bv = v;
+                                bprim = __float_as_uint(ta[i].w);
+                                if (ANY && t < rtmax) finish = true;
+                            }
+                        } else if (best == kInf) {
+                            best = 3.40282347e+38f;
+                        }
+                    }
+                }
+                lpos += cnt;
+                if (!finish && lpos >= lend) {
+                    uint32_t v[3];
+                    if (stk.pop(v)) {
+                        cur = v[0];
+                        tmin = __uint_as_float(v[1]);
+                        tmax = __uint_as_float(v[2]);
+                    } else {
+                        finish = true;
+                    }
+                }
+            } else if (best < tmin) {
+                finish = true;
+            } else {
                 if (COUNT) ++n_nodes;
                 const uint2 nd = *reinterpret_cast<const uint2*>(&nodes[cur]);
                 if ((nd.x & 3u) != 3u) {
@@ -477,52 +544,37 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
                         tmax = tplane;
                     }
                 } else {
-                    const uint32_t cnt = nd.x >> 2;
-                    for (uint32_t i = 0; i < cnt; ++i) {
-                        const TriRec* tp = tris + lp[nd.y + i];
-                        if (COUNT) ++n_tris;
-                        const float4 a = ldg4(tp->a), b = ldg4(tp->b), c = ldg4(tp->c);
-                        float t, u, v;
-                        // A closer hit replaces the record (strict <); an
-                        // empty record (hitTime FLT_MAX) replaces the initial
-                        // +inf (HitInfo.cpp:8-18, Kdtree.cpp:757-765).
-                        if (Pred::test(r, a, b, c, 0.0f, t, u, v)) {
-                            if (t < best) {
-                                best = t;
-                                bu = u;
-                                bv = v;
-                                bprim = __float_as_uint(a.w);
-                                if (ANY && t < rtmax) {
-                                    go = false;
-                                    break;
-                                }
-                            }
-                        } else if (best == __int_as_float(0x7f800000)) {
-                            best = 3.40282347e+38f;
+                    lpos = nd.y;
+                    lend = nd.y + (nd.x >> 2);
+                    if (lpos == lend) {  // empty leaf: straight to the next entry
+                        uint32_t v[3];
+                        if (stk.pop(v)) {
+                            cur = v[0];
+                            tmin = __uint_as_float(v[1]);
+                            tmax = __uint_as_float(v[2]);
+                        } else {
+                            finish = true;
                         }
-                    }
-                    if (!go) break;
-                    uint32_t v[3];
-                    if (stk.pop(v)) {
-                        cur = v[0];
-                        tmin = __uint_as_float(v[1]);
-                        tmax = __uint_as_float(v[2]);
-                    } else {
-                        go = false;
                     }
                 }
             }
-            if (COUNT) {
-                atomicAdd(&L.counts[0], (unsigned long long)n_nodes);
-                atomicAdd(&L.counts[1], (unsigned long long)n_tris);
+            if (finish) {
+                const bool has = (bprim != kNone) && (best < rtmax);
+                if (ANY) {
+                    L.occluded[ray] = has ? 1u : 0u;
+                } else {
+                    *reinterpret_cast<float4*>(&L.hits[ray]) = make_float4(
+                        has ? best : rtmax, has ? bu : 0.0f, has ? bv : 0.0f, __uint_as_float(has ? bprim : kNone));
+                }
+                if (COUNT) {
+                    atomicAdd(&L.counts[0], (unsigned long long)n_nodes);
+                    atomicAdd(&L.counts[1], (unsigned long long)n_tris);
+                    n_nodes = n_tris = 0;
+                }
+                active = false;
+                ray = kNone;
             }
-            const bool has = (bprim != kNone) && (best < rtmax);
-            if (ANY) {
-                L.occluded[ray] = has ? 1u : 0u;
-            } else {
-                *reinterpret_cast<float4*>(&L.hits[ray]) = make_float4(
-                    has ? best : rtmax, has ? bu : 0.0f, has ? bv : 0.0f, __uint_as_float(has ? bprim : kNone));
-            }
+            if ((uint32_t)__popcll(__ballot(true)) < thresh) break;
         }
     }
 }
