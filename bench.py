@@ -50,6 +50,9 @@ def parse():
                     help="c3 (default): 8M incoherent rays at 1.31M triangles; c1: Cornell box eye rays; c2: "
                          "1080p eye + shadow rays at 1.31M triangles; c4: photon gather; c5: serving")
     ap.add_argument("--requests", type=int, default=160, help="c5: requests per arrival-rate point")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="run one device-to-device copy of this many bytes first (counter calibration)")
+    ap.add_argument("--batch-config", default="", help="stack,blocks_per_cu,fetch_threshold: skip autotuning")
     return ap.parse_args()
 
 
@@ -115,6 +118,15 @@ def main():
                 ev[2 * b + 1].record(stream)
         return nb
 
+    if args.copy_probe:
+        src = torch.empty(args.copy_probe // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: %d bytes" % args.copy_probe)
+        del src, dst
     batch = None
     tuned = None
     if args.mode == "batched":
@@ -122,8 +134,13 @@ def main():
         batch.set([(scene, d_rays[int(offs[b]):int(offs[b + 1])], d_hits[int(offs[b]):int(offs[b + 1])])
                    for b in range(nb)])
         t = time.time()
-        tuned = batch.autotune(stream=stream)
-        log("autotuned in %.1fs: %s" % (time.time() - t, tuned))
+        if args.batch_config:
+            st_, bpc_, th_ = (int(x) for x in args.batch_config.split(","))
+            batch.set_config(st_, bpc_, th_)
+            tuned = dict(batch.config(), ms=None)
+        else:
+            tuned = batch.autotune(stream=stream)
+        log("launch config in %.1fs: %s" % (time.time() - t, tuned))
 
     def batched_step(timed=False):
         if timed:
