@@ -37,6 +37,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", choices=["plain", "batched"], default="batched")
+    ap.add_argument("--width", type=int, choices=[2, 4], default=4, help="BVH children per node")
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--rays", type=int, default=8388608)
     ap.add_argument("--batches", type=int, default=64)
@@ -85,10 +86,10 @@ def main():
     xyz, idx = scenes.config2_scene(level=args.level)
     log("scene: %d triangles (%.1fs)" % (len(idx), time.time() - t))
     t = time.time()
-    scene = rq.Scene(dev, xyz, idx, accel="bvh", predicate="encore")
+    scene = rq.Scene(dev, xyz, idx, accel="bvh", predicate="encore", width=args.width)
     st = scene.stats()
-    log("bvh: %d nodes, depth %d, %.1f MB, build %.2fs" % (st["nnodes"], st["max_depth"], st["device_bytes"] / 1e6,
-                                                          st["build_seconds"]))
+    log("bvh%d: %d nodes, depth %d, %.1f MB, build %.2fs" % (args.width, st["nnodes"], st["max_depth"],
+                                                            st["device_bytes"] / 1e6, st["build_seconds"]))
     prim = scenes.config2_primary()
     d_prim = torch.from_numpy(prim).cuda()
     d_ph = torch.empty((len(prim), 4), dtype=torch.float32, device="cuda")
@@ -214,20 +215,27 @@ def main():
     roof = None
     if not args.no_counts:
         n_rays, n_nodes, n_tris = scene.counts(d_rays)
-        alg_bytes = n_rays * (32 + 16) + 64 * n_nodes + 48 * n_tris
+        node_bytes = 128 if args.width == 4 else 64
+        alg_bytes = n_rays * (32 + 16) + node_bytes * n_nodes + 48 * n_tris
         achieved = alg_bytes / (kernel_ms_total / 1000.0) / 1e9
         traffic = None
         tf = os.path.join(ROOT, "profiles", "bvh_trace_config3_traffic.json")
         if os.path.exists(tf):
             with open(tf) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
+                tj = json.load(f)
+            # The counters were read for one node width; another width's
+            # traffic is unmeasured.
+            if tj.get("bvh_width", 2) == args.width and args.mode == "batched":
+                traffic = tj.get("hbm_bytes_per_launch")
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "alg_bytes_per_launch": int(alg_bytes / launches_per_step), "nodes_per_ray": round(n_nodes / n_rays, 2),
                 "tris_per_ray": round(n_tris / n_rays, 2),
                 "avg_launch_ms": round(kernel_ms_total / launches_per_step, 4),
-                "kernel": "bvh_trace<PredEncore, closest, stack %s, batched>" % (tuned["stack"] if tuned else 16)
-                if args.mode == "batched" else "bvh_trace<PredEncore, closest, stack 16>"}
+                "node_bytes": node_bytes,
+                "kernel": "bvh_trace<PredEncore, closest, stack %s, batched, width %d>"
+                % (tuned["stack"] if tuned else 16, args.width)
+                if args.mode == "batched" else "bvh_trace<PredEncore, closest, stack 16, width %d>" % args.width}
         log("roofline: %.1f nodes/ray, %.1f tris/ray, %.1f GB/s algorithmic" % (n_nodes / n_rays, n_tris / n_rays,
                                                                              achieved))
 
@@ -317,10 +325,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic: seeded displaced icosphere and seeded bounce rays",
         "config": {"workload": "8,388,608 incoherent bounce rays in %d lognormal batches on a displaced icosphere "
-                               "(level %d) + ground, %d triangles; BVH2 SAH, ENCORE predicate; %s" % (
-                                   nb, args.level, len(idx), "one launch per batch" if args.mode == "plain" else
+                               "(level %d) + ground, %d triangles; BVH%d SAH, ENCORE predicate; %s" % (
+                                   nb, args.level, len(idx), args.width,
+                                   "one launch per batch" if args.mode == "plain" else
                                    "one persistent hipGraph launch per step, autotuned"),
-                   "rays": N, "batches": nb, "triangles": int(len(idx)), "accel": "bvh", "predicate": "encore",
+                   "rays": N, "batches": nb, "triangles": int(len(idx)), "accel": "bvh%d" % args.width,
+                   "predicate": "encore",
                    "mode": args.mode, "autotuned": tuned},
         "roofline": roof,
         "cpu_baseline": cpu,

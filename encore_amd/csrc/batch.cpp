@@ -16,7 +16,8 @@ struct rq_batch {
     rq_device* dev = nullptr;
     uint32_t any_hit = 0;
     uint32_t predicate = 0;
-    uint32_t total = 0, nitems = 0, max_depth = 0;
+    uint32_t total = 0, nitems = 0, width = 2;
+    uint32_t spill_words = 0;  // per-lane spill region covering every item's scene
     std::vector<rq::BatchItemDev> items;
     rq::DevBuf items_dev;
     rq::DevBuf scratch;  // counter (256 B) + spill stacks; owned so a graph can bake it
@@ -46,12 +47,12 @@ static rq::TraceKind kind_of(const rq_batch* b) { return b->any_hit ? TraceKind:
 // scratch buffer moved (a captured graph must then be rebuilt).
 static bool prepare(rq_batch* b) {
     int per_cu = b->kc.blocks_per_cu;
-    const int occ = occupancy_bvh_batched(b->predicate, kind_of(b), b->kc.stack);
+    const int occ = occupancy_bvh_batched(b->predicate, kind_of(b), b->width, b->kc.stack);
     if (per_cu <= 0 || per_cu > occ) per_cu = occ;
     int64_t grid = (int64_t)per_cu * b->dev->num_cus;
     grid = std::max<int64_t>(1, std::min<int64_t>(grid, ((int64_t)b->total + 255) / 256));
     b->grid = (int)grid;
-    const size_t need = 256 + (size_t)b->grid * 4 * 64 * (b->max_depth + 2) * sizeof(uint32_t);
+    const size_t need = 256 + (size_t)b->grid * 4 * 64 * b->spill_words * sizeof(uint32_t);
     if (b->scratch.bytes < need) {
         RQ_HIP(hipDeviceSynchronize());
         b->scratch = DevBuf(need);
@@ -65,7 +66,7 @@ static TraceLaunch launch_params(const rq_batch* b) {
     L.n = b->total;
     L.counter = b->scratch.as<uint32_t>();
     L.spill = b->scratch.as<uint32_t>() + 64;
-    L.spill_depth = b->max_depth + 2;
+    L.spill_depth = b->spill_words;
     L.items = b->items_dev.as<BatchItemDev>();
     L.nitems = b->nitems;
     L.fetch_threshold = (uint32_t)b->kc.fetch_threshold;
@@ -75,7 +76,7 @@ static TraceLaunch launch_params(const rq_batch* b) {
 static void enqueue(rq_batch* b, hipStream_t s) {
     const TraceLaunch L = launch_params(b);
     launch_zero_u32(L.counter, 1, s);
-    launch_bvh_batched(L, b->predicate, kind_of(b), b->grid, b->kc, s);
+    launch_bvh_batched(L, b->predicate, kind_of(b), b->width, b->grid, b->kc, s);
 }
 
 static void launch(rq_batch* b, bool use_graph, hipStream_t s) {
@@ -139,7 +140,7 @@ rq_status rq_batch_set(rq_batch* b, const rq_batch_item* items, uint32_t count) 
         require(count == 0 || items, "null items");
         std::vector<BatchItemDev> v;
         uint64_t total = 0;
-        uint32_t pred = 0, depth = 0;
+        uint32_t pred = 0, words = 0, width = 2;
         bool first = true;
         for (uint32_t i = 0; i < count; ++i) {
             const rq_batch_item& it = items[i];
@@ -150,14 +151,17 @@ rq_status rq_batch_set(rq_batch* b, const rq_batch_item* items, uint32_t count) 
                 throw Error(RQ_ERR_UNSUPPORTED, "batched launches take BVH scenes");
             if (first) {
                 pred = it.scene->predicate;
+                width = it.scene->width;
                 first = false;
             } else if (it.scene->predicate != pred) {
                 throw Error(RQ_ERR_UNSUPPORTED, "all items of a batch must use one predicate");
+            } else if (it.scene->width != width) {
+                throw Error(RQ_ERR_UNSUPPORTED, "all items of a batch must use one BVH width");
             }
             if (it.n == 0) continue;
             require(it.d_rays && it.d_out, "null ray or output buffer");
             BatchItemDev d{};
-            d.nodes = it.scene->nodes.as<BvhNode>();
+            d.nodes = it.scene->nodes.ptr;
             d.tris = it.scene->tris.as<TriRec>();
             d.rays = it.d_rays;
             d.hits = b->any_hit ? nullptr : static_cast<rq_hit*>(it.d_out);
@@ -166,7 +170,7 @@ rq_status rq_batch_set(rq_batch* b, const rq_batch_item* items, uint32_t count) 
             d.n = it.n;
             total += it.n;
             if (total > 0xFFFFFFF0ull) throw Error(RQ_ERR_CAPACITY, "a batch holds at most 2^32-16 rays");
-            depth = std::max(depth, it.scene->stats.max_depth);
+            words = std::max(words, scene_spill_words(it.scene));
             v.push_back(d);
         }
         RQ_HIP(hipSetDevice(b->dev->ordinal));
@@ -180,7 +184,8 @@ rq_status rq_batch_set(rq_batch* b, const rq_batch_item* items, uint32_t count) 
         b->nitems = (uint32_t)b->items.size();
         b->total = (uint32_t)total;
         b->predicate = pred;
-        b->max_depth = depth;
+        b->width = width;
+        b->spill_words = words;
         b->drop_graph();
     });
 }
@@ -226,10 +231,10 @@ rq_status rq_batch_autotune(rq_batch* b, void* stream, rq_kernel_config* chosen,
         KernelConfig best = saved;
         float best_t = 1e30f;
         const int stacks[] = {8, 16, 32};
-        const int thresholds[] = {0, 16, 32, 48};
+        const int thresholds[] = {32, 48, 56, 60};
         try {
             for (int st : stacks) {
-                const int occ = occupancy_bvh_batched(b->predicate, kind_of(b), st);
+                const int occ = occupancy_bvh_batched(b->predicate, kind_of(b), b->width, st);
                 for (int bpc = 1; bpc <= occ; bpc = (bpc < 4 ? bpc + 1 : bpc + 2)) {
                     for (int th : thresholds) {
                         b->kc.stack = st;

@@ -191,8 +191,88 @@ struct Builder {
 
 }  // namespace
 
+// Collapses the binary build tree into 4-wide nodes: a node's children are
+// found by repeatedly opening the inner child with the largest surface area
+// until four children are reached or only leaves remain.
+static void flatten4(const std::vector<BNode>& bn, int32_t root, BvhHost& out, uint32_t& nleaves,
+                     uint32_t& depth4) {
+    out.nodes4.clear();
+    out.nodes4.reserve(bn.size() / 3 + 2);
+    auto empty_node = []() {
+        BvhNode4 g;
+        for (int c = 0; c < 4; ++c) {
+            g.lox[c] = g.loy[c] = g.loz[c] = INFINITY;
+            g.hix[c] = g.hiy[c] = g.hiz[c] = -INFINITY;
+            g.ref[c] = kRefEmpty;
+            g.pad[c] = 0;
+        }
+        return g;
+    };
+    auto set_child = [&](BvhNode4& g, int c, const BNode& n) {
+        g.lox[c] = n.box.lo[0];
+        g.hix[c] = n.box.hi[0];
+        g.loy[c] = n.box.lo[1];
+        g.hiy[c] = n.box.hi[1];
+        g.loz[c] = n.box.lo[2];
+        g.hiz[c] = n.box.hi[2];
+    };
+    out.nodes4.push_back(empty_node());
+    nleaves = 0;
+    depth4 = 0;
+    if (bn[root].count > 0) {
+        set_child(out.nodes4[0], 0, bn[root]);
+        out.nodes4[0].ref[0] = make_leaf_ref(bn[root].first, bn[root].count);
+        nleaves = 1;
+        depth4 = 1;
+        return;
+    }
+    struct Item {
+        int32_t bi;
+        uint32_t slot, depth;
+    };
+    std::vector<Item> todo{{root, 0, 1}};
+    while (!todo.empty()) {
+        const Item it = todo.back();
+        todo.pop_back();
+        depth4 = std::max(depth4, it.depth);
+        std::vector<int32_t> kids{bn[it.bi].left, bn[it.bi].right};
+        while (kids.size() < 4) {
+            int best = -1;
+            float best_area = -1.0f;
+            for (size_t k = 0; k < kids.size(); ++k) {
+                const BNode& n = bn[kids[k]];
+                if (n.count == 0 && n.left >= 0 && n.box.area() > best_area) {
+                    best_area = n.box.area();
+                    best = (int)k;
+                }
+            }
+            if (best < 0) break;
+            const int32_t open = kids[best];
+            kids[best] = bn[open].left;
+            kids.insert(kids.begin() + best + 1, bn[open].right);
+        }
+        std::vector<Item> next;
+        for (size_t k = 0; k < kids.size(); ++k) {
+            const BNode& n = bn[kids[k]];
+            uint32_t ref;
+            if (n.count > 0) {
+                ref = make_leaf_ref(n.first, n.count);
+                ++nleaves;
+            } else {
+                ref = (uint32_t)out.nodes4.size();
+                out.nodes4.push_back(empty_node());
+                next.push_back({kids[k], ref, it.depth + 1});
+            }
+            BvhNode4& g = out.nodes4[it.slot];
+            set_child(g, (int)k, n);
+            g.ref[k] = ref;
+        }
+        for (auto r = next.rbegin(); r != next.rend(); ++r) todo.push_back(*r);
+    }
+}
+
 void build_bvh(const float* xyz, uint32_t nverts, const uint32_t* idx, uint32_t ntris, uint32_t leaf_size,
-               BvhHost& out) {
+               uint32_t width, BvhHost& out) {
     auto t0 = std::chrono::steady_clock::now();
     if (ntris == 0) throw Error(RQ_ERR_INVALID_ARG, "scene has no triangles");
     if (ntris > kLeafMaxFirst) throw Error(RQ_ERR_CAPACITY, "BVH supports at most 2^27-1 triangles");
@@ -216,6 +296,22 @@ void build_bvh(const float* xyz, uint32_t nverts, const uint32_t* idx, uint32_t 
     Builder B(refs, ids, leaf_size);
     const int root = B.build(0, ntris, 0, bn, max_depth, 3);
 
+    if (width == 4) {
+        uint32_t nleaves4 = 0, depth4 = 0;
+        flatten4(bn, root, out, nleaves4, depth4);
+        out.width = 4;
+        out.nodes.clear();
+        out.tri_order = std::move(ids);
+        out.stats.nnodes = (uint32_t)out.nodes4.size();
+        out.stats.nleaves = nleaves4;
+        out.stats.nrefs = ntris;
+        out.stats.max_depth = depth4;
+        out.stats.stack_bound = 3 * depth4 + 2;
+        out.stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return;
+    }
+    if (width != 2) throw Error(RQ_ERR_INVALID_ARG, "bvh_width must be 2 or 4");
+    out.width = 2;
     // Flatten: one BvhNode per inner build node, pre-order, left first.
     out.nodes.clear();
     out.nodes.reserve(bn.size() / 2 + 1);
@@ -272,6 +368,7 @@ void build_bvh(const float* xyz, uint32_t nverts, const uint32_t* idx, uint32_t 
     out.stats.nleaves = nleaves;
     out.stats.nrefs = ntris;
     out.stats.max_depth = max_depth;
+    out.stats.stack_bound = max_depth + 2;
     out.stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

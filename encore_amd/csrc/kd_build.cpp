@@ -166,6 +166,7 @@ void build_kd(const float* xyz, uint32_t nverts, const uint32_t* idx, uint32_t n
     out.stats.nrefs = (uint32_t)out.leaf_prims.size();
     out.stats.max_depth = B.deepest;
     out.stats.kd_depth_limit = B.max_depth;
+    out.stats.stack_bound = B.deepest + 2;
     out.stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

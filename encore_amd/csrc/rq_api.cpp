@@ -85,9 +85,9 @@ static uint32_t* stream_scratch(rq_device* dev, hipStream_t s, size_t spill_byte
     return b.as<uint32_t>();
 }
 
-static int grid_for(rq_device* dev, uint32_t accel, uint32_t pred, TraceKind kind, uint32_t n) {
+static int grid_for(rq_device* dev, uint32_t accel, uint32_t pred, TraceKind kind, uint32_t width, uint32_t n) {
     const KernelConfig& kc = dev->kc;
-    int per_cu = kc.blocks_per_cu > 0 ? kc.blocks_per_cu : occupancy_blocks_per_cu(accel, pred, kind, kc);
+    int per_cu = kc.blocks_per_cu > 0 ? kc.blocks_per_cu : occupancy_blocks_per_cu(accel, pred, kind, width, kc);
     int64_t grid = (int64_t)per_cu * dev->num_cus;
     const int64_t need = ((int64_t)n + 255) / 256;
     grid = std::max<int64_t>(1, std::min(grid, need));
@@ -99,10 +99,9 @@ static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, ui
     if (n == 0) return;
     rq_device* dev = sc->dev;
     RQ_HIP(hipSetDevice(dev->ordinal));
-    const int grid = grid_for(dev, sc->accel, sc->predicate, kind, n);
-    const uint32_t depth = sc->stats.max_depth + 2;
-    const uint32_t words = spill_entry_words(sc->accel);
-    const size_t spill_bytes = (size_t)grid * 4 * 64 * depth * words * sizeof(uint32_t);
+    const int grid = grid_for(dev, sc->accel, sc->predicate, kind, sc->width, n);
+    const uint32_t words = scene_spill_words(sc);
+    const size_t spill_bytes = (size_t)grid * 4 * 64 * words * sizeof(uint32_t);
     uint32_t* scratch = stream_scratch(dev, s, spill_bytes);
     TraceLaunch L{};
     L.nodes = sc->nodes.ptr;
@@ -114,20 +113,20 @@ static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, ui
     L.n = n;
     L.counter = scratch;
     L.spill = scratch + 64;
-    L.spill_depth = depth;
+    L.spill_depth = words;
     for (int k = 0; k < 3; ++k) {
         L.root_pos[k] = sc->root_pos[k];
         L.root_size[k] = sc->root_size[k];
     }
     L.counts = counts;
-    L.fetch_threshold = (uint32_t)dev->kc.fetch_threshold;
+    L.fetch_threshold = (uint32_t)(sc->accel == RQ_ACCEL_KD ? dev->kc.kd_fetch_threshold : dev->kc.fetch_threshold);
     launch_zero_u32(scratch, 1, s);
     if (counts)
-        launch_count(L, sc->accel, sc->predicate, kind, grid, s);
+        launch_count(L, sc->accel, sc->predicate, kind, sc->width, grid, s);
     else if (sc->accel == RQ_ACCEL_KD)
-        launch_kd(L, sc->predicate, kind, grid, dev->kc, s);
+        launch_kd(L, sc->predicate, kind, grid, s);
     else
-        launch_bvh(L, sc->predicate, kind, grid, dev->kc, s);
+        launch_bvh(L, sc->predicate, kind, sc->width, grid, dev->kc, s);
 }
 
 // Model-list bounds for the kd root: given, or the vertex bounds.
@@ -215,12 +214,19 @@ rq_status rq_scene_create(rq_device* dev, const float* xyz, uint32_t nverts, con
         sc->ntris = ntris;
         std::vector<TriRec> tris;
         if (dd.accel == RQ_ACCEL_BVH) {
+            require(dd.bvh_width == 0 || dd.bvh_width == 2 || dd.bvh_width == 4, "bvh_width must be 0, 2 or 4");
             BvhHost h;
-            build_bvh(xyz, nverts, tri_idx, ntris, dd.leaf_size, h);
+            build_bvh(xyz, nverts, tri_idx, ntris, dd.leaf_size, dd.bvh_width ? dd.bvh_width : 4, h);
             pack_triangles(xyz, tri_idx, h.tri_order.data(), ntris, dd.predicate, tris);
             sc->stats = h.stats;
-            sc->nodes = DevBuf(h.nodes.size() * sizeof(BvhNode));
-            sc->nodes.upload(h.nodes.data(), sc->nodes.bytes);
+            sc->width = h.width;
+            if (h.width == 4) {
+                sc->nodes = DevBuf(h.nodes4.size() * sizeof(BvhNode4));
+                sc->nodes.upload(h.nodes4.data(), sc->nodes.bytes);
+            } else {
+                sc->nodes = DevBuf(h.nodes.size() * sizeof(BvhNode));
+                sc->nodes.upload(h.nodes.data(), sc->nodes.bytes);
+            }
         } else {
             float bmin[3], bmax[3];
             vertex_bounds(xyz, nverts, dd, bmin, bmax);
@@ -263,6 +269,8 @@ rq_status rq_scene_stats(const rq_scene* sc, rq_scene_info* info) {
         info->kd_max_depth_limit = sc->stats.kd_depth_limit;
         info->device_bytes = sc->device_bytes;
         info->build_seconds = sc->stats.seconds;
+        info->bvh_width = sc->accel == RQ_ACCEL_BVH ? sc->width : 0;
+        info->stack_bound = sc->stats.stack_bound;
     });
 }
 
@@ -282,24 +290,43 @@ rq_status rq_occluded(rq_scene* scene, const rq_ray* d_rays, uint32_t n, uint32_
     });
 }
 
+// Runs the instrumented kernel; h[0] nodes, h[1] triangles, h[2..4] wave
+// iterations of the node loop, the triangle loop and the refill round.
+static void count_trace(rq_scene* scene, const rq_ray* d_rays, uint32_t n, uint32_t any_hit, void* stream,
+                        unsigned long long h[5]) {
+    require(n == 0 || d_rays, "null ray buffer");
+    hipStream_t s = (hipStream_t)stream;
+    RQ_HIP(hipSetDevice(scene->dev->ordinal));
+    DevBuf out(n * (any_hit ? sizeof(uint32_t) : sizeof(rq_hit)) + 16);
+    DevBuf cnt(5 * sizeof(unsigned long long));
+    RQ_HIP(hipMemsetAsync(cnt.ptr, 0, cnt.bytes, s));
+    trace(scene, d_rays, n, any_hit ? nullptr : out.as<rq_hit>(), any_hit ? out.as<uint32_t>() : nullptr,
+          any_hit ? TraceKind::Any : TraceKind::Closest, s, cnt.as<unsigned long long>());
+    RQ_HIP(hipMemcpyAsync(h, cnt.ptr, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    RQ_HIP(hipStreamSynchronize(s));
+}
+
 rq_status rq_trace_counts(rq_scene* scene, const rq_ray* d_rays, uint32_t n, uint32_t any_hit, uint64_t* counts,
                           void* stream) {
     return guarded([&] {
         require(scene && counts, "null argument");
-        require(n == 0 || d_rays, "null ray buffer");
-        hipStream_t s = (hipStream_t)stream;
-        RQ_HIP(hipSetDevice(scene->dev->ordinal));
-        DevBuf out(n * (any_hit ? sizeof(uint32_t) : sizeof(rq_hit)) + 16);
-        DevBuf cnt(2 * sizeof(unsigned long long));
-        RQ_HIP(hipMemsetAsync(cnt.ptr, 0, cnt.bytes, s));
-        trace(scene, d_rays, n, any_hit ? nullptr : out.as<rq_hit>(), any_hit ? out.as<uint32_t>() : nullptr,
-              any_hit ? TraceKind::Any : TraceKind::Closest, s, cnt.as<unsigned long long>());
-        unsigned long long h[2] = {0, 0};
-        RQ_HIP(hipMemcpyAsync(h, cnt.ptr, sizeof(h), hipMemcpyDeviceToHost, s));
-        RQ_HIP(hipStreamSynchronize(s));
+        unsigned long long h[5] = {0, 0, 0, 0, 0};
+        count_trace(scene, d_rays, n, any_hit, stream, h);
         counts[0] = n;
         counts[1] = h[0];
         counts[2] = h[1];
+    });
+}
+
+rq_status rq_trace_profile(rq_scene* scene, const rq_ray* d_rays, uint32_t n, uint32_t any_hit, uint64_t* counts,
+                           void* stream) {
+    return guarded([&] {
+        require(scene && counts, "null argument");
+        require(scene->accel == RQ_ACCEL_BVH, "traversal profile covers the BVH kernel");
+        unsigned long long h[5] = {0, 0, 0, 0, 0};
+        count_trace(scene, d_rays, n, any_hit, stream, h);
+        counts[0] = n;
+        for (int k = 0; k < 5; ++k) counts[1 + k] = h[k];
     });
 }
 
@@ -426,7 +453,8 @@ rq_status rq_host_tree_build(const float* xyz, uint32_t nverts, const uint32_t* 
         t->accel = dd.accel;
         t->ntris = ntris;
         if (dd.accel == RQ_ACCEL_BVH) {
-            build_bvh(xyz, nverts, tri_idx, ntris, dd.leaf_size, t->bvh);
+            require(dd.bvh_width == 0 || dd.bvh_width == 2 || dd.bvh_width == 4, "bvh_width must be 0, 2 or 4");
+            build_bvh(xyz, nverts, tri_idx, ntris, dd.leaf_size, dd.bvh_width ? dd.bvh_width : 4, t->bvh);
         } else {
             float bmin[3], bmax[3];
             vertex_bounds(xyz, nverts, dd, bmin, bmax);
@@ -449,6 +477,8 @@ rq_status rq_host_tree_info(const rq_host_tree* t, rq_scene_info* info) {
         info->max_depth = s.max_depth;
         info->kd_max_depth_limit = s.kd_depth_limit;
         info->build_seconds = s.seconds;
+        info->bvh_width = t->accel == RQ_ACCEL_BVH ? t->bvh.width : 0;
+        info->stack_bound = s.stack_bound;
     });
 }
 
@@ -456,7 +486,10 @@ rq_status rq_host_tree_copy(const rq_host_tree* t, void* nodes, uint32_t* leaf_p
     return guarded([&] {
         require(t != nullptr, "null tree");
         if (t->accel == RQ_ACCEL_BVH) {
-            if (nodes) std::memcpy(nodes, t->bvh.nodes.data(), t->bvh.nodes.size() * sizeof(BvhNode));
+            if (nodes && t->bvh.width == 4)
+                std::memcpy(nodes, t->bvh.nodes4.data(), t->bvh.nodes4.size() * sizeof(BvhNode4));
+            else if (nodes)
+                std::memcpy(nodes, t->bvh.nodes.data(), t->bvh.nodes.size() * sizeof(BvhNode));
             if (leaf_prims) std::memcpy(leaf_prims, t->bvh.tri_order.data(), t->bvh.tri_order.size() * 4);
             if (root_box)
                 for (int k = 0; k < 6; ++k) root_box[k] = 0.0f;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <cstddef>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -82,6 +83,17 @@ struct alignas(64) BvhNode {
 };
 static_assert(sizeof(BvhNode) == 64, "BVH node must be one 64-byte line");
 
+// BVH4 node, 128 bytes: four child boxes as six float4 (lo.x, hi.x, lo.y,
+// hi.y, lo.z, hi.z of children 0..3), four refs, padding.  An unused slot
+// has an inverted infinite box (never hit) and ref kRefEmpty.
+struct alignas(128) BvhNode4 {
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    uint32_t ref[4];
+    uint32_t pad[4];
+};
+static_assert(sizeof(BvhNode4) == 128, "BVH4 node must be two 64-byte lines");
+constexpr uint32_t kRefEmpty = 0xFFFFFFFFu;
+
 constexpr uint32_t kRefLeaf = 0x80000000u;
 constexpr uint32_t kRefSentinel = 0x7FFFFFFFu;
 constexpr uint32_t kLeafMaxCount = 16;
@@ -111,18 +123,21 @@ struct KdNodeG {
 
 struct BuildStats {
     uint32_t nnodes = 0, nleaves = 0, nrefs = 0, max_depth = 0, kd_depth_limit = 0;
+    uint32_t stack_bound = 0;  // traversal stack entries a ray can need
     double seconds = 0;
 };
 
 // Host-side BVH build result.
 struct BvhHost {
+    uint32_t width = 2;
     std::vector<BvhNode> nodes;
+    std::vector<BvhNode4> nodes4;
     std::vector<uint32_t> tri_order;  // leaf order -> original triangle index
     BuildStats stats;
 };
 
 void build_bvh(const float* xyz, uint32_t nverts, const uint32_t* idx, uint32_t ntris, uint32_t leaf_size,
-               BvhHost& out);
+               uint32_t width, BvhHost& out);
 
 // Host-side kd build result (ENCORE's tree).
 struct KdHost {
@@ -203,17 +218,20 @@ uint32_t knn_spill_words_per_lane(uint32_t n);
 // Kernel launch parameters (by value).
 // ---------------------------------------------------------------------------
 // One item of a batched launch (device copy), 64 bytes.
+// Field order keeps (nodes, tris) and (hits, occ) in aligned 16-byte pairs,
+// each read by one vector load when a lane takes a ray of the item.
 struct alignas(64) BatchItemDev {
-    const BvhNode* nodes;
+    const void* nodes;  // BvhNode* or BvhNode4* (all items of a launch share the width)
     const TriRec* tris;
-    const rq_ray* rays;
     rq_hit* hits;
     uint32_t* occ;
+    const rq_ray* rays;
     uint32_t begin;  // global index of the item's first ray
     uint32_t n;
     uint32_t pad[4];
 };
 static_assert(sizeof(BatchItemDev) == 64, "batch descriptor is 64 bytes");
+static_assert(offsetof(BatchItemDev, hits) == 16 && offsetof(BatchItemDev, occ) == 24, "pointer pairs aligned");
 
 struct TraceLaunch {
     const void* nodes;       // BvhNode* or KdNodeG*
@@ -239,22 +257,25 @@ struct KernelConfig {
     int block = 256;
     int blocks_per_cu = 0;   // 0 = occupancy maximum
     int stack = 8;           // LDS ring entries per lane
-    int fetch_threshold = 16; // refill idle lanes when fewer than this many of 64 are busy
+    int fetch_threshold = 48; // BVH: refill idle lanes when fewer than this many of 64 are busy
+    int kd_fetch_threshold = 16;  // kd path (its refill claims rays with an atomic per round)
 };
 
 // Launchers (traverse.hip).
-void launch_bvh(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, const KernelConfig& kc,
-                hipStream_t s);
-void launch_bvh_batched(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, const KernelConfig& kc,
-                        hipStream_t s);
-int occupancy_bvh_batched(uint32_t predicate, TraceKind kind, int stack);
-void launch_kd(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, const KernelConfig& kc,
-               hipStream_t s);
-void launch_count(const TraceLaunch& L, uint32_t accel, uint32_t predicate, TraceKind kind, int grid,
+void launch_bvh(const TraceLaunch& L, uint32_t predicate, TraceKind kind, uint32_t width, int grid,
+                const KernelConfig& kc, hipStream_t s);
+void launch_bvh_batched(const TraceLaunch& L, uint32_t predicate, TraceKind kind, uint32_t width, int grid,
+                        const KernelConfig& kc, hipStream_t s);
+int occupancy_bvh_batched(uint32_t predicate, TraceKind kind, uint32_t width, int stack);
+void launch_kd(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, hipStream_t s);
+void launch_count(const TraceLaunch& L, uint32_t accel, uint32_t predicate, TraceKind kind, uint32_t width, int grid,
                   hipStream_t s);
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
-int occupancy_blocks_per_cu(uint32_t accel, uint32_t predicate, TraceKind kind, const KernelConfig& kc);
-uint32_t spill_entry_words(uint32_t accel);
+int occupancy_blocks_per_cu(uint32_t accel, uint32_t predicate, TraceKind kind, uint32_t width,
+                            const KernelConfig& kc);
+// Dwords per traversal stack entry: BVH2 1 (ref), BVH4 2 (ref, entry
+// distance), kd 3 (node, tmin, tmax).
+uint32_t stack_entry_words(uint32_t accel, uint32_t width);
 
 }  // namespace rq
 
@@ -284,9 +305,15 @@ struct rq_photons {
 
 struct rq_scene {
     rq_device* dev = nullptr;
-    uint32_t accel = 0, predicate = 0, ntris = 0;
+    uint32_t accel = 0, predicate = 0, ntris = 0, width = 2;
     rq::DevBuf nodes, tris, leaf_prims;
     float root_pos[3] = {0, 0, 0}, root_size[3] = {0, 0, 0};
     rq::BuildStats stats;
     uint64_t device_bytes = 0;
 };
+
+// Per-lane spill region (dwords) that covers the deepest traversal stack of
+// a scene.
+inline uint32_t scene_spill_words(const rq_scene* sc) {
+    return sc->stats.stack_bound * rq::stack_entry_words(sc->accel, sc->width);
+}

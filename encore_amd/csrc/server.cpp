@@ -128,7 +128,6 @@ struct rq_server {
     rq_status fatal = RQ_OK;
     std::string fatal_msg;
     rq::KernelConfig kc;
-    uint32_t max_depth = 64;
 };
 
 namespace rq {
@@ -144,10 +143,7 @@ void slot_init(rq_server* s, Slot& sl) {
     sl.cap = std::max<uint32_t>(16, s->cfg.max_requests_per_launch);
     sl.table = DevBuf(sl.cap * sizeof(BatchItemDev));
     RQ_HIP(hipHostMalloc((void**)&sl.h_table, sl.cap * sizeof(BatchItemDev), hipHostMallocDefault));
-    const int occ = occupancy_bvh_batched(RQ_PRED_ENCORE, TraceKind::Closest, s->kc.stack);
-    const int per_cu = s->kc.blocks_per_cu > 0 ? std::min(s->kc.blocks_per_cu, occ) : occ;
-    const size_t grid = (size_t)per_cu * s->dev->num_cus;
-    sl.scratch = DevBuf(256 + grid * 4 * 64 * (s->max_depth + 2) * sizeof(uint32_t));
+    sl.scratch = DevBuf(256);
 }
 
 void slot_free(Slot& sl) {
@@ -164,15 +160,27 @@ void issue(rq_server* s, Slot& sl) {
     hipStream_t st = sl.stream;
     const bool any = sl.reqs.front()->any != 0;
     uint64_t total = 0;
-    uint32_t depth = 0;
+    uint32_t words = 0;
+    for (Request* r : sl.reqs) words = std::max(words, scene_spill_words(r->scene));
+    const uint32_t pred = sl.reqs.front()->scene->predicate;
+    const uint32_t width = sl.reqs.front()->scene->width;
+    const TraceKind kind = any ? TraceKind::Any : TraceKind::Closest;
+    int grid_max;
+    if (s->cfg.plain) {
+        grid_max = occupancy_blocks_per_cu(RQ_ACCEL_BVH, pred, kind, width, s->dev->kc) * s->dev->num_cus;
+    } else {
+        const int occ = occupancy_bvh_batched(pred, kind, width, s->kc.stack);
+        grid_max = (s->kc.blocks_per_cu > 0 ? std::min(s->kc.blocks_per_cu, occ) : occ) * s->dev->num_cus;
+    }
+    // The slot's stream is idle here (its previous launch was retired), so
+    // its scratch can be replaced.
+    const size_t need = 256 + (size_t)grid_max * 4 * 64 * words * sizeof(uint32_t);
+    if (sl.scratch.bytes < need) sl.scratch = DevBuf(need);
     for (Request* r : sl.reqs) {
         char* base = s->arena.ptr(r->off);
         RQ_HIP(hipMemcpyAsync(base, r->h_rays, (size_t)r->n * sizeof(rq_ray), hipMemcpyHostToDevice, st));
-        depth = std::max(depth, r->scene->stats.max_depth);
         total += r->n;
     }
-    if (depth > s->max_depth) throw Error(RQ_ERR_UNSUPPORTED, "scene deeper than the server's stack budget");
-    const uint32_t pred = sl.reqs.front()->scene->predicate;
     if (s->cfg.plain) {
         Request* r = sl.reqs.front();
         char* base = s->arena.ptr(r->off);
@@ -186,21 +194,18 @@ void issue(rq_server* s, Slot& sl) {
         L.n = r->n;
         L.counter = sl.scratch.as<uint32_t>();
         L.spill = sl.scratch.as<uint32_t>() + 64;
-        L.spill_depth = r->scene->stats.max_depth + 2;
+        L.spill_depth = words;
         L.fetch_threshold = (uint32_t)s->dev->kc.fetch_threshold;
-        const int occ = occupancy_blocks_per_cu(RQ_ACCEL_BVH, pred, any ? TraceKind::Any : TraceKind::Closest,
-                                                s->dev->kc);
-        const int64_t grid = std::max<int64_t>(
-            1, std::min<int64_t>((int64_t)occ * s->dev->num_cus, ((int64_t)r->n + 255) / 256));
+        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(grid_max, ((int64_t)r->n + 255) / 256));
         launch_zero_u32(L.counter, 1, st);
-        launch_bvh(L, pred, any ? TraceKind::Any : TraceKind::Closest, (int)grid, s->dev->kc, st);
+        launch_bvh(L, pred, kind, width, (int)grid, s->dev->kc, st);
     } else {
         uint32_t i = 0;
         uint64_t begin = 0;
         for (Request* r : sl.reqs) {
             char* base = s->arena.ptr(r->off);
             BatchItemDev d{};
-            d.nodes = r->scene->nodes.as<BvhNode>();
+            d.nodes = r->scene->nodes.ptr;
             d.tris = r->scene->tris.as<TriRec>();
             d.rays = reinterpret_cast<const rq_ray*>(base);
             void* out = base + Arena::round((uint64_t)r->n * sizeof(rq_ray));
@@ -216,17 +221,13 @@ void issue(rq_server* s, Slot& sl) {
         L.n = (uint32_t)total;
         L.counter = sl.scratch.as<uint32_t>();
         L.spill = sl.scratch.as<uint32_t>() + 64;
-        L.spill_depth = s->max_depth + 2;
+        L.spill_depth = words;
         L.items = sl.table.as<BatchItemDev>();
         L.nitems = i;
         L.fetch_threshold = (uint32_t)s->kc.fetch_threshold;
-        const TraceKind kind = any ? TraceKind::Any : TraceKind::Closest;
-        const int occ = occupancy_bvh_batched(pred, kind, s->kc.stack);
-        const int per_cu = s->kc.blocks_per_cu > 0 ? std::min(s->kc.blocks_per_cu, occ) : occ;
-        const int64_t grid =
-            std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * s->dev->num_cus, ((int64_t)total + 255) / 256));
+        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(grid_max, ((int64_t)total + 255) / 256));
         launch_zero_u32(L.counter, 1, st);
-        launch_bvh_batched(L, pred, kind, (int)grid, s->kc, st);
+        launch_bvh_batched(L, pred, kind, width, (int)grid, s->kc, st);
     }
     for (Request* r : sl.reqs) {
         char* base = s->arena.ptr(r->off);
@@ -278,11 +279,12 @@ void worker_loop(rq_server* s) {
             if (sl.busy || s->admitted.empty()) continue;
             const uint32_t any = s->admitted.front()->any;
             const uint32_t pred = s->admitted.front()->scene->predicate;
+            const uint32_t width = s->admitted.front()->scene->width;
             uint64_t rays = 0;
             for (auto it = s->admitted.begin(); it != s->admitted.end();) {
                 Request* r = *it;
                 const bool fits = sl.reqs.size() < sl.cap && (sl.reqs.empty() || rays + r->n <= s->cfg.max_rays_per_launch);
-                if (r->any == any && r->scene->predicate == pred && fits) {
+                if (r->any == any && r->scene->predicate == pred && r->scene->width == width && fits) {
                     sl.reqs.push_back(r);
                     rays += r->n;
                     it = s->admitted.erase(it);

@@ -30,8 +30,31 @@ namespace dev {
 constexpr int kWave = 64;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
-__device__ __forceinline__ float4 ldg4(const void* p) { return *reinterpret_cast<const float4*>(p); }
-__device__ __forceinline__ uint4 ldgu4(const void* p) { return *reinterpret_cast<const uint4*>(p); }
+// Loads and stores through the global address space.  Every buffer the
+// kernels touch is device memory, whose generic address equals its global
+// address; going through an integer makes the compiler emit global_load /
+// global_store (which wait on vmcnt alone) even for pointers read from a
+// descriptor table, where it could not prove the address space and would
+// emit flat accesses that also wait on lgkmcnt.
+typedef float F4v __attribute__((ext_vector_type(4)));
+typedef uint32_t U4v __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) F4v GF4v;
+typedef const __attribute__((address_space(1))) U4v GU4v;
+typedef __attribute__((address_space(1))) F4v GF4vOut;
+typedef __attribute__((address_space(1))) uint32_t GU32Out;
+__device__ __forceinline__ float4 ldg4(const void* p) {
+    const F4v v = *(GF4v*)(uintptr_t)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 ldgu4(const void* p) {
+    const U4v v = *(GU4v*)(uintptr_t)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void stg4(void* p, float4 v) {
+    const F4v o = {v.x, v.y, v.z, v.w};
+    *(GF4vOut*)(uintptr_t)p = o;
+}
+__device__ __forceinline__ void stg1(uint32_t* p, uint32_t v) { *(GU32Out*)(uintptr_t)p = v; }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
@@ -77,6 +100,22 @@ struct RingStack {
         return true;
     }
     __device__ __forceinline__ void reset() { sp = lo = 0; }
+    // Makes room for n more entries (spilling the oldest), so that put(i)
+    // for i < n and advance(n) need no check of their own.
+    __device__ __forceinline__ void reserve(int n) {
+        while (sp - lo + n > S) {
+            const int s = lo & (S - 1);
+#pragma unroll
+            for (int w = 0; w < W; ++w) spill[((size_t)lo * W + w) * kWave] = lds[(s * W + w) * kWave];
+            ++lo;
+        }
+    }
+    __device__ __forceinline__ void put(int i, const uint32_t (&v)[W]) {
+        const int s = (sp + i) & (S - 1);
+#pragma unroll
+        for (int w = 0; w < W; ++w) lds[(s * W + w) * kWave] = v[w];
+    }
+    __device__ __forceinline__ void advance(int n) { sp += n; }
 };
 
 // ---------------------------------------------------------------------------
@@ -252,86 +291,270 @@ __device__ __forceinline__ uint32_t find_item(const BatchItemDev* items, uint32_
     return lo;
 }
 
-template <class Pred, bool ANY, int S, bool COUNT = false, bool BATCHED = false>
+// Compare-exchange of (distance, ref) pairs, ascending by distance.
+__device__ __forceinline__ void cswap(float& ka, uint32_t& ra, float& kb, uint32_t& rb) {
+    const bool sw = kb < ka;
+    const float k = sw ? kb : ka;
+    kb = sw ? ka : kb;
+    ka = k;
+    const uint32_t r = sw ? rb : ra;
+    rb = sw ? ra : rb;
+    ra = r;
+}
+
+// Ray staging.  A wave claims rays in chunks of 64 consecutive ids with one
+// atomicAdd, loads them with every lane busy, computes 1/d (and, in a
+// batched launch, the owning item and the item-local index) and parks the
+// records in LDS, [field][slot].  Idle lanes are refilled from there: a
+// ballot, a prefix count and a few LDS reads, with no global memory access
+// and no division on the refill path, so lanes can be refilled as soon as a
+// handful of them are idle.
+constexpr int kStageFields = 13;  // ox oy oz tmin dx dy dz tmax ix iy iz local item
+struct Stage {
+    float* base;  // &lds_stage[wave][0][0]
+    __device__ __forceinline__ float& at(int f, uint32_t slot) { return base[f * kWave + slot]; }
+};
+
+// Item of a batched launch whose rays include global ray index `id` (items
+// sorted by begin): a wave-uniform binary search on the chunk's first id,
+// then a per-lane forward scan (a 64-ray chunk spans few items).
+__device__ __forceinline__ uint32_t chunk_item(const BatchItemDev* items, uint32_t nitems, uint32_t first_id,
+                                               uint32_t id) {
+    uint32_t lo = 0, hi = nitems;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (items[mid].begin <= first_id)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    while (lo + 1 < nitems && items[lo + 1].begin <= id) ++lo;
+    return lo;
+}
+
+// WIDTH 2: BvhNode (both child boxes in one 64-byte node), stack entries are
+// bare refs.  WIDTH 4: BvhNode4 (four child boxes, 128 bytes), the hit
+// children are sorted by entry distance with a 5-comparator network, the
+// nearest is visited next and the others are pushed far to near together
+// with their entry distance; a popped entry whose entry distance is beyond
+// the current closest hit is dropped (the box test would now fail).
+//
+// Lanes leave the trace loop when their ray ends; the wave returns to the
+// refill step once fewer than fetch_threshold lanes are still tracing.
+template <class Pred, bool ANY, int S, bool COUNT = false, bool BATCHED = false, int WIDTH = 2>
 __global__ __launch_bounds__(256) void bvh_trace(TraceLaunch L) {
-    __shared__ uint32_t lds_stack[256 / kWave][S][kWave];
+    constexpr int SW = WIDTH == 4 ? 2 : 1;  // dwords per stack entry
+    __shared__ uint32_t lds_stack[256 / kWave][S * SW][kWave];
+    __shared__ float lds_stage[256 / kWave][kStageFields][kWave];
     const int lane = lane_id();
     const int wave = threadIdx.x / kWave;
     const uint32_t gwave = blockIdx.x * (blockDim.x / kWave) + wave;
-    RingStack<S, 1> stk;
+    RingStack<S, SW> stk;
     stk.lds = &lds_stack[wave][0][lane];
     stk.spill = L.spill + (size_t)gwave * L.spill_depth * kWave + lane;
-    // Per-lane scene and buffers: uniform in a plain launch, per item in a
+    Stage stage{&lds_stage[wave][0][0]};
+    // Per-lane scene and output: uniform in a plain launch, per item in a
     // batched one.
-    const BvhNode* nodes = static_cast<const BvhNode*>(L.nodes);
+    const void* nodes = L.nodes;
     const TriRec* tris = L.tris;
-    rq_hit* hout = L.hits;
-    uint32_t* oout = L.occluded;
-    uint32_t local = 0;
+    void* out = ANY ? static_cast<void*>(L.occluded) : static_cast<void*>(L.hits);
 
     RayState r;
     float tmin = 0, tbest = 0, bu = 0, bv = 0;
     uint32_t bprim = kNone;
-    uint32_t ray = kNone;
+    uint32_t local = kNone;  // item-local ray index; kNone = idle lane
     uint32_t cur = kRefSentinel, pend = kNone;
-    const uint32_t thresh = L.fetch_threshold;
     uint32_t n_nodes = 0, n_tris = 0;  // COUNT only
+    // COUNT only: wave iterations of the node loop, of the triangle loop and
+    // of the refill round, each tallied by the lowest active lane.
+    uint32_t w_node = 0, w_tri = 0, w_round = 0;
+    uint32_t offx = 0, offy = 0, offz = 0;  // WIDTH 4: near-plane rows picked by direction sign
+    // Wave-uniform chunk state.
+    uint32_t ch_base = 0, ch_n = 0, ch_used = 0;
+    bool exhausted = false;
+
+    // Next stacked ref, skipping (WIDTH 4) entries the closest hit has passed.
+    auto pop_next = [&]() -> uint32_t {
+        uint32_t v[SW];
+        while (stk.pop(v)) {
+            if (SW == 1 || __uint_as_float(v[SW - 1]) <= tbest * 1.00000048f) return v[0];
+        }
+        return kRefSentinel;
+    };
 
     for (;;) {
-        ray = fetch_work(ray, L.counter, L.n);
-        if (ray != kNone && ray != kDone && cur == kRefSentinel && pend == kNone) {
-            const rq_ray* rin = L.rays;
-            local = ray;
-            if (BATCHED) {
-                const BatchItemDev* it = L.items + find_item(L.items, L.nitems, ray);
-                nodes = it->nodes;
-                tris = it->tris;
-                rin = it->rays;
-                hout = it->hits;
-                oout = it->occ;
-                local = ray - it->begin;
+        // Refill idle lanes, staging new chunks as needed.
+        for (;;) {
+            const uint64_t idle = __ballot(local == kNone);
+            if (idle == 0) break;
+            if (ch_used == ch_n) {
+                if (exhausted) break;
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(L.counter, (uint32_t)kWave);
+                base = __builtin_amdgcn_readfirstlane(base);
+                if (base >= L.n) {
+                    exhausted = true;
+                    break;
+                }
+                ch_base = base;
+                ch_n = min((uint32_t)kWave, L.n - base);
+                ch_used = 0;
+                if ((uint32_t)lane < ch_n) {
+                    const uint32_t id = base + lane;
+                    const rq_ray* rin = L.rays;
+                    uint32_t loc = id, item = 0;
+                    if (BATCHED) {
+                        item = chunk_item(L.items, L.nitems, base, id);
+                        const BatchItemDev* it = L.items + item;
+                        rin = it->rays;
+                        loc = id - it->begin;
+                    }
+                    const float4 p = ldg4(&rin[loc].ox);
+                    const float4 q = ldg4(&rin[loc].dx);
+                    stage.at(0, lane) = p.x;
+                    stage.at(1, lane) = p.y;
+                    stage.at(2, lane) = p.z;
+                    stage.at(3, lane) = p.w;
+                    stage.at(4, lane) = q.x;
+                    stage.at(5, lane) = q.y;
+                    stage.at(6, lane) = q.z;
+                    stage.at(7, lane) = q.w;
+                    stage.at(8, lane) = 1.0f / q.x;
+                    stage.at(9, lane) = 1.0f / q.y;
+                    stage.at(10, lane) = 1.0f / q.z;
+                    stage.at(11, lane) = __uint_as_float(loc);
+                    stage.at(12, lane) = __uint_as_float(item);
+                }
             }
-            init_ray(r, rin, local, tmin, tbest);
-            Pred::prepare(r);
-            bu = bv = 0.0f;
-            bprim = kNone;
-            cur = 0;
-            stk.reset();
+            const uint32_t take = min((uint32_t)__popcll(idle), ch_n - ch_used);
+            const uint32_t rank = lanes_below(idle);
+            if (local == kNone && rank < take) {
+                const uint32_t slot = ch_used + rank;
+                r.ox = stage.at(0, slot);
+                r.oy = stage.at(1, slot);
+                r.oz = stage.at(2, slot);
+                tmin = stage.at(3, slot);
+                r.dx = stage.at(4, slot);
+                r.dy = stage.at(5, slot);
+                r.dz = stage.at(6, slot);
+                tbest = stage.at(7, slot);
+                r.ix = stage.at(8, slot);
+                r.iy = stage.at(9, slot);
+                r.iz = stage.at(10, slot);
+                local = __float_as_uint(stage.at(11, slot));
+                if (BATCHED) {
+                    const BatchItemDev* it = L.items + __float_as_uint(stage.at(12, slot));
+                    const uint4 pp = ldgu4(&it->nodes);
+                    nodes = reinterpret_cast<const void*>(((uint64_t)pp.y << 32) | pp.x);
+                    tris = reinterpret_cast<const TriRec*>(((uint64_t)pp.w << 32) | pp.z);
+                    const uint4 oo = ldgu4(&it->hits);
+                    out = reinterpret_cast<void*>(ANY ? (((uint64_t)oo.w << 32) | oo.z)
+                                                      : (((uint64_t)oo.y << 32) | oo.x));
+                }
+                Pred::prepare(r);
+                if (WIDTH == 4) {
+                    offx = __builtin_signbit(r.dx) ? 4u : 0u;
+                    offy = __builtin_signbit(r.dy) ? 12u : 8u;
+                    offz = __builtin_signbit(r.dz) ? 20u : 16u;
+                }
+                bu = bv = 0.0f;
+                bprim = kNone;
+                cur = 0;
+                stk.reset();
+            }
+            ch_used += take;
         }
-        if (__ballot(ray != kDone) == 0) break;
+        if (__ballot(local != kNone) == 0) break;  // no work left for this wave
+        const uint32_t thresh = exhausted ? 0u : L.fetch_threshold;
+        if (COUNT && lane == 0) ++w_round;
 
-        // Trace until this lane's ray ends or the wave runs low on active lanes.
-        while (ray != kDone && (cur != kRefSentinel || pend != kNone)) {
-            // Node phase.
+        // One round: node phase, leaf phase, retire.  A lane keeps its
+        // traversal state (cur, pend, stack) across rounds.
+        if (local != kNone) {
+            // Node phase: until every lane still here has a leaf postponed,
+            // or fewer than `thresh` lanes are still traversing (then the
+            // wave goes on to refill the lanes whose rays ended).
             while (cur != kRefSentinel && (int)cur >= 0) {
-                const BvhNode* np = nodes + cur;
-                if (COUNT) ++n_nodes;
-                const float4 nx = ldg4(np->x), ny = ldg4(np->y), nz = ldg4(np->z);
-                const uint4 meta = ldgu4(np->ref);
-                float tn0, tn1;
-                bool h1;
-                const bool h0 = box2(r, nx, ny, nz, tmin, tbest, tn0, tn1, h1);
-                if (h0 && h1) {
-                    const bool first0 = tn0 <= tn1;
-                    const uint32_t nearr = first0 ? meta.x : meta.y;
-                    const uint32_t farr = first0 ? meta.y : meta.x;
-                    uint32_t v[1] = {farr};
-                    stk.push(v);
-                    cur = nearr;
-                } else if (h0) {
-                    cur = meta.x;
-                } else if (h1) {
-                    cur = meta.y;
+                if (COUNT) {
+                    ++n_nodes;
+                    if (lane == __ffsll((unsigned long long)__ballot(true)) - 1) ++w_node;
+                }
+                if constexpr (WIDTH == 2) {
+                    const BvhNode* np = static_cast<const BvhNode*>(nodes) + cur;
+                    const float4 nx = ldg4(np->x), ny = ldg4(np->y), nz = ldg4(np->z);
+                    const uint4 meta = ldgu4(np->ref);
+                    float tn0, tn1;
+                    bool h1;
+                    const bool h0 = box2(r, nx, ny, nz, tmin, tbest, tn0, tn1, h1);
+                    if (h0 && h1) {
+                        const bool first0 = tn0 <= tn1;
+                        const uint32_t nearr = first0 ? meta.x : meta.y;
+                        const uint32_t farr = first0 ? meta.y : meta.x;
+                        uint32_t v[SW] = {farr};
+                        stk.push(v);
+                        cur = nearr;
+                    } else if (h0) {
+                        cur = meta.x;
+                    } else if (h1) {
+                        cur = meta.y;
+                    } else {
+                        cur = pop_next();
+                    }
                 } else {
-                    uint32_t v[1];
-                    cur = stk.pop(v) ? v[0] : kRefSentinel;
+                    const float* np = reinterpret_cast<const float*>(static_cast<const BvhNode4*>(nodes) + cur);
+                    const float4 nx = ldg4(np + offx), fx = ldg4(np + (offx ^ 4u));
+                    const float4 ny = ldg4(np + offy), fy = ldg4(np + (offy ^ 4u));
+                    const float4 nz = ldg4(np + offz), fz = ldg4(np + (offz ^ 4u));
+                    const uint4 rf = ldgu4(np + 24);
+                    const float kInf = __int_as_float(0x7f800000);
+                    float k[4];
+                    uint32_t c[4] = {rf.x, rf.y, rf.z, rf.w};
+                    const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
+                    const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
+                    const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const float tnx = (nxa[j] - r.ox) * r.ix, tfx = (fxa[j] - r.ox) * r.ix;
+                        const float tny = (nya[j] - r.oy) * r.iy, tfy = (fya[j] - r.oy) * r.iy;
+                        const float tnz = (nza[j] - r.oz) * r.iz, tfz = (fza[j] - r.oz) * r.iz;
+                        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+                        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tbest)) * 1.00000048f;
+                        k[j] = (tn <= tf && c[j] != kRefEmpty) ? tn : kInf;
+                    }
+                    cswap(k[0], c[0], k[1], c[1]);
+                    cswap(k[2], c[2], k[3], c[3]);
+                    cswap(k[0], c[0], k[2], c[2]);
+                    cswap(k[1], c[1], k[3], c[3]);
+                    cswap(k[1], c[1], k[2], c[2]);
+                    if (k[0] < kInf) {
+                        // Push children 1..nh-1 far to near: child j lands
+                        // at offset np - j from the current top.
+                        const int np = (k[1] < kInf) + (k[2] < kInf) + (k[3] < kInf);
+                        stk.reserve(np);
+                        if (np >= 3) {
+                            uint32_t v[SW] = {c[3], __float_as_uint(k[3])};
+                            stk.put(0, v);
+                        }
+                        if (np >= 2) {
+                            uint32_t v[SW] = {c[2], __float_as_uint(k[2])};
+                            stk.put(np - 2, v);
+                        }
+                        if (np >= 1) {
+                            uint32_t v[SW] = {c[1], __float_as_uint(k[1])};
+                            stk.put(np - 1, v);
+                        }
+                        stk.advance(np);
+                        cur = c[0];
+                    } else {
+                        cur = pop_next();
+                    }
                 }
                 if ((int)cur < 0 && pend == kNone) {
                     pend = cur;
-                    uint32_t v[1];
-                    cur = stk.pop(v) ? v[0] : kRefSentinel;
+                    cur = pop_next();
                 }
-                if (__ballot(pend == kNone) == 0) break;
+                const uint64_t busy = __ballot(true);
+                if (__ballot(pend == kNone) == 0 || (uint32_t)__popcll(busy) < thresh) break;
             }
             // Leaf phase.
             while (pend != kNone) {
@@ -340,7 +563,10 @@ __global__ __launch_bounds__(256) void bvh_trace(TraceLaunch L) {
                 bool done = false;
                 for (uint32_t i = 0; i < count; ++i) {
                     const TriRec* tp = tris + first + i;
-                    if (COUNT) ++n_tris;
+                    if (COUNT) {
+                        ++n_tris;
+                        if (lane == __ffsll((unsigned long long)__ballot(true)) - 1) ++w_tri;
+                    }
                     const float4 a = ldg4(tp->a), b = ldg4(tp->b), c = ldg4(tp->c);
                     float t, u, v;
                     if (Pred::test(r, a, b, c, tmin, t, u, v) && t < tbest) {
@@ -361,29 +587,32 @@ __global__ __launch_bounds__(256) void bvh_trace(TraceLaunch L) {
                 }
                 if (cur != kRefSentinel && (int)cur < 0) {
                     pend = cur;
-                    uint32_t v[1];
-                    cur = stk.pop(v) ? v[0] : kRefSentinel;
+                    cur = pop_next();
                 } else {
                     pend = kNone;
                 }
             }
-            if ((uint32_t)__popcll(__ballot(true)) < thresh) break;
         }
-        if (ray != kNone && ray != kDone && cur == kRefSentinel && pend == kNone) {
+        if (local != kNone && cur == kRefSentinel && pend == kNone) {
             if (ANY) {
-                oout[local] = (bprim != kNone) ? 1u : 0u;
+                stg1(static_cast<uint32_t*>(out) + local, (bprim != kNone) ? 1u : 0u);
             } else {
                 const bool has = bprim != kNone;
-                *reinterpret_cast<float4*>(&hout[local]) =
-                    make_float4(tbest, has ? bu : 0.0f, has ? bv : 0.0f, __uint_as_float(bprim));
+                stg4(static_cast<rq_hit*>(out) + local,
+                     make_float4(tbest, has ? bu : 0.0f, has ? bv : 0.0f, __uint_as_float(bprim)));
             }
             if (COUNT) {
                 atomicAdd(&L.counts[0], (unsigned long long)n_nodes);
                 atomicAdd(&L.counts[1], (unsigned long long)n_tris);
                 n_nodes = n_tris = 0;
             }
-            ray = kNone;
+            local = kNone;
         }
+    }
+    if (COUNT) {
+        if (w_node) atomicAdd(&L.counts[2], (unsigned long long)w_node);
+        if (w_tri) atomicAdd(&L.counts[3], (unsigned long long)w_tri);
+        if (w_round) atomicAdd(&L.counts[4], (unsigned long long)w_round);
     }
 }
 
@@ -433,7 +662,7 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
     const uint32_t gwave = blockIdx.x * (blockDim.x / kWave) + wave;
     RingStack<S, 3> stk;
     stk.lds = &lds_stack[wave][0][lane];
-    stk.spill = L.spill + (size_t)gwave * L.spill_depth * 3 * kWave + lane;
+    stk.spill = L.spill + (size_t)gwave * L.spill_depth * kWave + lane;
     const KdNodeG* nodes = static_cast<const KdNodeG*>(L.nodes);
     const TriRec* tris = L.tris;
     const float kInf = __int_as_float(0x7f800000);
@@ -591,131 +820,84 @@ __global__ void zero_u32(uint32_t* p, uint32_t n) {
 // ---------------------------------------------------------------------------
 // kd ring: 8 entries x 3 dwords x 64 lanes x 4 waves = 24 KB of LDS per block.
 constexpr int kKdStack = 8;
-template <class Pred, bool ANY>
-static void launch_bvh_t(const TraceLaunch& L, int grid, const KernelConfig& kc, hipStream_t s) {
-    switch (kc.stack) {
+
+template <class Pred, bool ANY, bool COUNT, bool BATCHED, int WIDTH>
+static const void* bvh_fn_s(int stack) {
+    if (COUNT) return (const void*)dev::bvh_trace<Pred, ANY, 16, COUNT, BATCHED, WIDTH>;
+    switch (stack) {
         case 8:
-            hipLaunchKernelGGL((dev::bvh_trace<Pred, ANY, 8>), dim3(grid), dim3(256), 0, s, L);
-            break;
+            return (const void*)dev::bvh_trace<Pred, ANY, 8, COUNT, BATCHED, WIDTH>;
         case 32:
-            hipLaunchKernelGGL((dev::bvh_trace<Pred, ANY, 32>), dim3(grid), dim3(256), 0, s, L);
-            break;
+            return (const void*)dev::bvh_trace<Pred, ANY, 32, COUNT, BATCHED, WIDTH>;
         default:
-            hipLaunchKernelGGL((dev::bvh_trace<Pred, ANY, 16>), dim3(grid), dim3(256), 0, s, L);
-            break;
+            return (const void*)dev::bvh_trace<Pred, ANY, 16, COUNT, BATCHED, WIDTH>;
     }
+}
+
+template <class Pred, bool ANY, bool COUNT, bool BATCHED>
+static const void* bvh_fn_w(uint32_t width, int stack) {
+    return width == 4 ? bvh_fn_s<Pred, ANY, COUNT, BATCHED, 4>(stack) : bvh_fn_s<Pred, ANY, COUNT, BATCHED, 2>(stack);
 }
 
 template <class Pred, bool ANY>
-static void launch_bvh_batched_t(const TraceLaunch& L, int grid, const KernelConfig& kc, hipStream_t s) {
-    switch (kc.stack) {
-        case 8:
-            hipLaunchKernelGGL((dev::bvh_trace<Pred, ANY, 8, false, true>), dim3(grid), dim3(256), 0, s, L);
-            break;
-        case 32:
-            hipLaunchKernelGGL((dev::bvh_trace<Pred, ANY, 32, false, true>), dim3(grid), dim3(256), 0, s, L);
-            break;
-        default:
-            hipLaunchKernelGGL((dev::bvh_trace<Pred, ANY, 16, false, true>), dim3(grid), dim3(256), 0, s, L);
-            break;
-    }
+static const void* bvh_fn_v(bool count, bool batched, uint32_t width, int stack) {
+    if (count) return bvh_fn_w<Pred, ANY, true, false>(width, stack);
+    return batched ? bvh_fn_w<Pred, ANY, false, true>(width, stack) : bvh_fn_w<Pred, ANY, false, false>(width, stack);
 }
 
-void launch_bvh_batched(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, const KernelConfig& kc,
-                        hipStream_t s) {
+// The bvh_trace instantiation for a launch.
+static const void* bvh_fn(uint32_t predicate, TraceKind kind, bool count, bool batched, uint32_t width, int stack) {
     const bool any = kind == TraceKind::Any;
-    if (predicate == RQ_PRED_WATERTIGHT) {
-        if (any)
-            launch_bvh_batched_t<dev::PredWatertight, true>(L, grid, kc, s);
-        else
-            launch_bvh_batched_t<dev::PredWatertight, false>(L, grid, kc, s);
-    } else {
-        if (any)
-            launch_bvh_batched_t<dev::PredEncore, true>(L, grid, kc, s);
-        else
-            launch_bvh_batched_t<dev::PredEncore, false>(L, grid, kc, s);
-    }
-    RQ_HIP(hipGetLastError());
-}
-
-int occupancy_bvh_batched(uint32_t predicate, TraceKind kind, int stack) {
-    const bool any = kind == TraceKind::Any;
-    const void* fn;
-#define RQ_PICKB(P, A)                                                                        \
-    (stack == 8    ? (const void*)dev::bvh_trace<P, A, 8, false, true>                       \
-     : stack == 32 ? (const void*)dev::bvh_trace<P, A, 32, false, true>                      \
-                   : (const void*)dev::bvh_trace<P, A, 16, false, true>)
     if (predicate == RQ_PRED_WATERTIGHT)
-        fn = any ? RQ_PICKB(dev::PredWatertight, true) : RQ_PICKB(dev::PredWatertight, false);
-    else
-        fn = any ? RQ_PICKB(dev::PredEncore, true) : RQ_PICKB(dev::PredEncore, false);
-#undef RQ_PICKB
+        return any ? bvh_fn_v<dev::PredWatertight, true>(count, batched, width, stack)
+                   : bvh_fn_v<dev::PredWatertight, false>(count, batched, width, stack);
+    return any ? bvh_fn_v<dev::PredEncore, true>(count, batched, width, stack)
+               : bvh_fn_v<dev::PredEncore, false>(count, batched, width, stack);
+}
+
+static const void* kd_fn(uint32_t predicate, TraceKind kind, bool count) {
+    const bool any = kind == TraceKind::Any;
+#define RQ_KD(P, A) (count ? (const void*)dev::kd_trace<P, A, kKdStack, true> : (const void*)dev::kd_trace<P, A, kKdStack>)
+    if (predicate == RQ_PRED_WATERTIGHT)
+        return any ? RQ_KD(dev::PredWatertight, true) : RQ_KD(dev::PredWatertight, false);
+    return any ? RQ_KD(dev::PredEncore, true) : RQ_KD(dev::PredEncore, false);
+#undef RQ_KD
+}
+
+static void launch_trace(const void* fn, const TraceLaunch& L, int grid, hipStream_t s) {
+    TraceLaunch arg = L;
+    void* args[] = {&arg};
+    RQ_HIP(hipLaunchKernel(fn, dim3(grid), dim3(256), args, 0, s));
+}
+
+static int occupancy_of(const void* fn) {
     int nb = 0;
     RQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0));
     return nb > 0 ? nb : 1;
 }
 
-template <class Pred, bool ANY>
-static void launch_kd_t(const TraceLaunch& L, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((dev::kd_trace<Pred, ANY, kKdStack>), dim3(grid), dim3(256), 0, s, L);
+void launch_bvh(const TraceLaunch& L, uint32_t predicate, TraceKind kind, uint32_t width, int grid,
+                const KernelConfig& kc, hipStream_t s) {
+    launch_trace(bvh_fn(predicate, kind, false, false, width, kc.stack), L, grid, s);
 }
 
-void launch_bvh(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, const KernelConfig& kc,
-                hipStream_t s) {
-    const bool any = kind == TraceKind::Any;
-    if (predicate == RQ_PRED_WATERTIGHT) {
-        if (any)
-            launch_bvh_t<dev::PredWatertight, true>(L, grid, kc, s);
-        else
-            launch_bvh_t<dev::PredWatertight, false>(L, grid, kc, s);
-    } else {
-        if (any)
-            launch_bvh_t<dev::PredEncore, true>(L, grid, kc, s);
-        else
-            launch_bvh_t<dev::PredEncore, false>(L, grid, kc, s);
-    }
-    RQ_HIP(hipGetLastError());
+void launch_bvh_batched(const TraceLaunch& L, uint32_t predicate, TraceKind kind, uint32_t width, int grid,
+                        const KernelConfig& kc, hipStream_t s) {
+    launch_trace(bvh_fn(predicate, kind, false, true, width, kc.stack), L, grid, s);
 }
 
-void launch_kd(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, const KernelConfig& kc,
-               hipStream_t s) {
-    (void)kc;
-    const bool any = kind == TraceKind::Any;
-    if (predicate == RQ_PRED_WATERTIGHT) {
-        if (any)
-            launch_kd_t<dev::PredWatertight, true>(L, grid, s);
-        else
-            launch_kd_t<dev::PredWatertight, false>(L, grid, s);
-    } else {
-        if (any)
-            launch_kd_t<dev::PredEncore, true>(L, grid, s);
-        else
-            launch_kd_t<dev::PredEncore, false>(L, grid, s);
-    }
-    RQ_HIP(hipGetLastError());
+int occupancy_bvh_batched(uint32_t predicate, TraceKind kind, uint32_t width, int stack) {
+    return occupancy_of(bvh_fn(predicate, kind, false, true, width, stack));
 }
 
-void launch_count(const TraceLaunch& L, uint32_t accel, uint32_t predicate, TraceKind kind, int grid,
+void launch_kd(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, hipStream_t s) {
+    launch_trace(kd_fn(predicate, kind, false), L, grid, s);
+}
+
+void launch_count(const TraceLaunch& L, uint32_t accel, uint32_t predicate, TraceKind kind, uint32_t width, int grid,
                   hipStream_t s) {
-    const bool any = kind == TraceKind::Any;
-#define RQ_CNT(K, P, A) hipLaunchKernelGGL((dev::K<P, A, 16, true>), dim3(grid), dim3(256), 0, s, L)
-#define RQ_CNTK(K, P, A) hipLaunchKernelGGL((dev::K<P, A, kKdStack, true>), dim3(grid), dim3(256), 0, s, L)
-    if (accel == RQ_ACCEL_KD) {
-        if (predicate == RQ_PRED_WATERTIGHT) {
-            if (any) RQ_CNTK(kd_trace, dev::PredWatertight, true); else RQ_CNTK(kd_trace, dev::PredWatertight, false);
-        } else {
-            if (any) RQ_CNTK(kd_trace, dev::PredEncore, true); else RQ_CNTK(kd_trace, dev::PredEncore, false);
-        }
-    } else {
-        if (predicate == RQ_PRED_WATERTIGHT) {
-            if (any) RQ_CNT(bvh_trace, dev::PredWatertight, true); else RQ_CNT(bvh_trace, dev::PredWatertight, false);
-        } else {
-            if (any) RQ_CNT(bvh_trace, dev::PredEncore, true); else RQ_CNT(bvh_trace, dev::PredEncore, false);
-        }
-    }
-#undef RQ_CNT
-#undef RQ_CNTK
-    RQ_HIP(hipGetLastError());
+    launch_trace(accel == RQ_ACCEL_KD ? kd_fn(predicate, kind, true) : bvh_fn(predicate, kind, true, false, width, 16),
+                 L, grid, s);
 }
 
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
@@ -723,32 +905,14 @@ void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
     RQ_HIP(hipGetLastError());
 }
 
-int occupancy_blocks_per_cu(uint32_t accel, uint32_t predicate, TraceKind kind, const KernelConfig& kc) {
-    int nb = 0;
-    const bool any = kind == TraceKind::Any;
-    const void* fn = nullptr;
-    if (accel == RQ_ACCEL_KD) {
-        if (predicate == RQ_PRED_WATERTIGHT)
-            fn = any ? (const void*)dev::kd_trace<dev::PredWatertight, true, kKdStack>
-                     : (const void*)dev::kd_trace<dev::PredWatertight, false, kKdStack>;
-        else
-            fn = any ? (const void*)dev::kd_trace<dev::PredEncore, true, kKdStack>
-                     : (const void*)dev::kd_trace<dev::PredEncore, false, kKdStack>;
-    } else {
-#define RQ_PICK(P, A)                                                               \
-    (kc.stack == 8    ? (const void*)dev::bvh_trace<P, A, 8>                       \
-     : kc.stack == 32 ? (const void*)dev::bvh_trace<P, A, 32>                      \
-                      : (const void*)dev::bvh_trace<P, A, 16>)
-        if (predicate == RQ_PRED_WATERTIGHT)
-            fn = any ? RQ_PICK(dev::PredWatertight, true) : RQ_PICK(dev::PredWatertight, false);
-        else
-            fn = any ? RQ_PICK(dev::PredEncore, true) : RQ_PICK(dev::PredEncore, false);
-#undef RQ_PICK
-    }
-    RQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0));
-    return nb > 0 ? nb : 1;
+int occupancy_blocks_per_cu(uint32_t accel, uint32_t predicate, TraceKind kind, uint32_t width,
+                            const KernelConfig& kc) {
+    return occupancy_of(accel == RQ_ACCEL_KD ? kd_fn(predicate, kind, false)
+                                             : bvh_fn(predicate, kind, false, false, width, kc.stack));
 }
 
-uint32_t spill_entry_words(uint32_t accel) { return accel == RQ_ACCEL_KD ? 3u : 1u; }
+uint32_t stack_entry_words(uint32_t accel, uint32_t width) {
+    return accel == RQ_ACCEL_KD ? 3u : (width == 4 ? 2u : 1u);
+}
 
 }  // namespace rq

@@ -31,14 +31,15 @@ class RQError(RuntimeError):
 class SceneDesc(ctypes.Structure):
     _fields_ = [("accel", ctypes.c_uint32), ("predicate", ctypes.c_uint32), ("leaf_size", ctypes.c_uint32),
                 ("has_bounds", ctypes.c_uint32), ("bounds_min", ctypes.c_float * 3),
-                ("bounds_max", ctypes.c_float * 3)]
+                ("bounds_max", ctypes.c_float * 3), ("bvh_width", ctypes.c_uint32)]
 
 
 class SceneInfo(ctypes.Structure):
     _fields_ = [("accel", ctypes.c_uint32), ("predicate", ctypes.c_uint32), ("ntris", ctypes.c_uint32),
                 ("nnodes", ctypes.c_uint32), ("nleaves", ctypes.c_uint32), ("nrefs", ctypes.c_uint32),
                 ("max_depth", ctypes.c_uint32), ("kd_max_depth_limit", ctypes.c_uint32),
-                ("device_bytes", ctypes.c_uint64), ("build_seconds", ctypes.c_double)]
+                ("device_bytes", ctypes.c_uint64), ("build_seconds", ctypes.c_double),
+                ("bvh_width", ctypes.c_uint32), ("stack_bound", ctypes.c_uint32)]
 
 
 class BatchItem(ctypes.Structure):
@@ -77,9 +78,11 @@ def load():
     global _LIB
     if _LIB is not None:
         return _LIB
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError("libencore_rq.so is not built (run __graft_entry__.build()); no fallback exists")
-    L = ctypes.CDLL(LIB_PATH)
+    # RQ_LIBRARY names another build of the same library (A/B comparisons).
+    path = os.environ.get("RQ_LIBRARY") or LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError("%s is not built (run __graft_entry__.build()); no fallback exists" % path)
+    L = ctypes.CDLL(path)
     L.rq_version.restype = ctypes.c_char_p
     L.rq_last_error.restype = ctypes.c_char_p
     L.rq_device_open.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
@@ -92,6 +95,7 @@ def load():
     L.rq_intersect_closest.argtypes = [_vp, _vp, _u32, _vp, _vp]
     L.rq_occluded.argtypes = [_vp, _vp, _u32, _vp, _vp]
     L.rq_trace_counts.argtypes = [_vp, _vp, _u32, _u32, _vp, _vp]
+    L.rq_trace_profile.argtypes = [_vp, _vp, _u32, _u32, _vp, _vp]
     L.rq_batch_create.argtypes = [_vp, _u32, ctypes.POINTER(_vp)]
     L.rq_batch_set.argtypes = [_vp, _vp, _u32]
     L.rq_batch_launch.argtypes = [_vp, _u32, _vp]
@@ -124,7 +128,7 @@ def load():
     L.rq_host_tree_destroy.restype = None
     for name in ("rq_device_open", "rq_scene_create", "rq_scene_stats", "rq_intersect_closest", "rq_occluded",
                  "rq_host_tree_build", "rq_host_tree_info", "rq_host_tree_copy", "rq_trace_counts",
-                 "rq_photons_create", "rq_photons_order", "rq_photons_balance_host", "rq_knn",
+                 "rq_trace_profile", "rq_photons_create", "rq_photons_order", "rq_photons_balance_host", "rq_knn",
                  "rq_radiance_estimate", "rq_batch_create", "rq_batch_set", "rq_batch_launch", "rq_batch_get_config",
                  "rq_batch_set_config", "rq_batch_autotune", "rq_server_create", "rq_server_submit", "rq_server_wait",
                  "rq_server_poll", "rq_server_cancel", "rq_server_stats"):
@@ -175,19 +179,12 @@ class Device:
 class Scene:
     """An acceleration structure resident in HBM."""
 
-    def __init__(self, device, xyz, idx, accel="bvh", predicate="encore", leaf_size=0, bounds=None):
+    def __init__(self, device, xyz, idx, accel="bvh", predicate="encore", leaf_size=0, bounds=None, width=0):
+        """width: BVH children per node, 2 or 4 (0 = 4)."""
         L = load()
         xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
         idx = np.ascontiguousarray(idx, np.uint32).reshape(-1)
-        d = SceneDesc()
-        d.accel = ACCEL[accel]
-        d.predicate = PRED[predicate]
-        d.leaf_size = leaf_size
-        if bounds is not None:
-            d.has_bounds = 1
-            for k in range(3):
-                d.bounds_min[k] = float(bounds[0][k])
-                d.bounds_max[k] = float(bounds[1][k])
+        d = _desc(accel, predicate, leaf_size, bounds, width)
         h = _vp()
         check(L.rq_scene_create(device.h, xyz.ctypes.data, len(xyz) // 3, idx.ctypes.data, len(idx) // 3,
                                 ctypes.byref(d), ctypes.byref(h)))
@@ -213,6 +210,18 @@ class Scene:
         """rays: (n,8) float32 device tensor; hits: (n,4) float32 device tensor."""
         n = rays.shape[0] if n is None else n
         check(load().rq_intersect_closest(self.h, _ptr(rays), n, _ptr(hits), _stream(stream)))
+
+    def profile(self, rays, any_hit=False, n=None, stream=None):
+        """SIMD-efficiency profile of the BVH kernel (synchronous): dict of
+        rays, nodes, tris, node_iters, tri_iters, rounds and the node / triangle
+        loop lane utilisation."""
+        n = rays.shape[0] if n is None else n
+        c = np.zeros(6, np.uint64)
+        check(load().rq_trace_profile(self.h, _ptr(rays), n, int(any_hit), c.ctypes.data, _stream(stream)))
+        d = dict(zip(("rays", "nodes", "tris", "node_iters", "tri_iters", "rounds"), (int(x) for x in c)))
+        d["node_util"] = d["nodes"] / max(1, 64 * d["node_iters"])
+        d["tri_util"] = d["tris"] / max(1, 64 * d["tri_iters"])
+        return d
 
     def counts(self, rays, any_hit=False, n=None, stream=None):
         """(rays, nodes visited, triangles tested) of one query (synchronous)."""
@@ -383,13 +392,17 @@ def photon_balance_host(pos, layout="encore"):
 
 BVH_NODE_DTYPE = np.dtype([("x", "<f4", 4), ("y", "<f4", 4), ("z", "<f4", 4), ("ref", "<u4", 2),
                            ("pad", "<u4", 2)])
+BVH4_NODE_DTYPE = np.dtype([("lox", "<f4", 4), ("hix", "<f4", 4), ("loy", "<f4", 4), ("hiy", "<f4", 4),
+                            ("loz", "<f4", 4), ("hiz", "<f4", 4), ("ref", "<u4", 4), ("pad", "<u4", 4)])
+REF_EMPTY = 0xFFFFFFFF
 
 
-def _desc(accel, predicate, leaf_size, bounds):
+def _desc(accel, predicate, leaf_size, bounds, width=0):
     d = SceneDesc()
     d.accel = ACCEL[accel]
     d.predicate = PRED[predicate]
     d.leaf_size = leaf_size
+    d.bvh_width = width
     if bounds is not None:
         d.has_bounds = 1
         for k in range(3):
@@ -398,14 +411,15 @@ def _desc(accel, predicate, leaf_size, bounds):
     return d
 
 
-def host_tree(xyz, idx, accel="kd", leaf_size=0, bounds=None):
+def host_tree(xyz, idx, accel="kd", leaf_size=0, bounds=None, width=0):
     """Builds the engine's tree on the host (no GPU) and returns its arrays.
     kd: dict(words, childs, leaf_prims, root_box, info); bvh: dict(nodes,
-    tri_order, info)."""
+    tri_order, info) with BVH_NODE_DTYPE (width 2) or BVH4_NODE_DTYPE
+    (width 4, the default) nodes."""
     L = load()
     xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
     idx = np.ascontiguousarray(idx, np.uint32).reshape(-1)
-    d = _desc(accel, "encore", leaf_size, bounds)
+    d = _desc(accel, "encore", leaf_size, bounds, width)
     h = _vp()
     check(L.rq_host_tree_build(xyz.ctypes.data, len(xyz) // 3, idx.ctypes.data, len(idx) // 3, ctypes.byref(d),
                                ctypes.byref(h)))
@@ -420,7 +434,7 @@ def host_tree(xyz, idx, accel="kd", leaf_size=0, bounds=None):
             check(L.rq_host_tree_copy(h, nodes.ctypes.data, lp.ctypes.data, box.ctypes.data))
             return dict(words=nodes[:, 0].copy(), childs=nodes[:, 1].copy(), leaf_prims=lp[: info.nrefs],
                         root_box=box, info=inf)
-        nodes = np.zeros(info.nnodes, BVH_NODE_DTYPE)
+        nodes = np.zeros(info.nnodes, BVH4_NODE_DTYPE if info.bvh_width == 4 else BVH_NODE_DTYPE)
         order = np.zeros(info.ntris, np.uint32)
         check(L.rq_host_tree_copy(h, nodes.ctypes.data, order.ctypes.data, box.ctypes.data))
         return dict(nodes=nodes, tri_order=order, info=inf)

@@ -93,6 +93,8 @@ typedef struct rq_scene_desc {
                               bounds of the triangle list */
     float bounds_min[3];
     float bounds_max[3];
+    uint32_t bvh_width;    /* BVH only: children per node, 2 (64-byte nodes)
+                              or 4 (128-byte nodes); 0 = 4 */
 } rq_scene_desc;
 
 /* Scene statistics, filled by rq_scene_stats. */
@@ -106,6 +108,8 @@ typedef struct rq_scene_info {
     uint32_t kd_max_depth_limit;  /* KD: 11 + floor(1.3 ln N) (Kdtree.cpp:123) */
     uint64_t device_bytes; /* bytes resident in HBM for this scene */
     double build_seconds;  /* host build time */
+    uint32_t bvh_width;    /* BVH: 2 or 4; KD: 0 */
+    uint32_t stack_bound;  /* traversal stack entries a ray can need */
 } rq_scene_info;
 
 /* Library version string, e.g. "encore-rq 0.1.0 gfx950". */
@@ -284,12 +288,21 @@ rq_status rq_radiance_estimate(rq_photons* photons, const float* d_qpos, const f
 
 /* Traversal work of one query over n rays, counted by an instrumented copy
  * of the same kernel (synchronous): counts[0] = rays, counts[1] = nodes
- * visited (64-byte BVH nodes or 8-byte kd nodes), counts[2] = triangles
- * tested (48-byte records; on the kd path each also reads a 4-byte leaf
- * reference).  These are the N_node and N_tri of the algorithmic-bytes
- * figure used for the roofline.  No counterpart in ENCORE (measurement). */
+ * visited (64-byte BVH2, 128-byte BVH4 or 8-byte kd nodes), counts[2] =
+ * triangles tested (48-byte records, stored in leaf order on both paths).
+ * These are the N_node and N_tri of the algorithmic-bytes figure used for
+ * the roofline.  No counterpart in ENCORE (measurement). */
 rq_status rq_trace_counts(rq_scene* scene, const rq_ray* d_rays, uint32_t n, uint32_t any_hit,
                           uint64_t* counts, void* stream);
+
+/* SIMD efficiency of the BVH kernel on n rays (synchronous, BVH scenes):
+ * counts[0..2] as rq_trace_counts, counts[3] = wave iterations of the node
+ * loop, counts[4] = wave iterations of the triangle loop, counts[5] = refill
+ * rounds.  Lane utilisation of the node loop is counts[1] / (64 counts[3]),
+ * of the triangle loop counts[2] / (64 counts[4]).  No counterpart in ENCORE
+ * (measurement). */
+rq_status rq_trace_profile(rq_scene* scene, const rq_ray* d_rays, uint32_t n, uint32_t any_hit,
+                           uint64_t* counts, void* stream);
 
 /* ---------------------------------------------------------------------
  * Host-side trees.  Builds the same structure rq_scene_create uploads, on
@@ -309,7 +322,7 @@ typedef struct rq_host_tree rq_host_tree;
 rq_status rq_host_tree_build(const float* xyz, uint32_t nverts, const uint32_t* tri_idx, uint32_t ntris,
                              const rq_scene_desc* desc, rq_host_tree** out);
 rq_status rq_host_tree_info(const rq_host_tree* tree, rq_scene_info* info);
-/* nodes: info.nnodes * (8 or 64) bytes; leaf_prims: info.nrefs (KD) or
+/* nodes: info.nnodes * (8 (KD), 64 (BVH2) or 128 (BVH4)) bytes; leaf_prims: info.nrefs (KD) or
  * info.ntris (BVH) entries; root_box: pos[3], size[3] (KD) or zeros. */
 rq_status rq_host_tree_copy(const rq_host_tree* tree, void* nodes, uint32_t* leaf_prims, float* root_box);
 void rq_host_tree_destroy(rq_host_tree* tree);

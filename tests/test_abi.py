@@ -67,29 +67,76 @@ def test_engine_kd_tree_honours_model_bounds():
     assert np.array_equal(e["words"], t["words"])
 
 
+def _children(nodes, ni, width):
+    """(ref, box lo, box hi) of each child slot of node ni."""
+    out = []
+    for c in range(width):
+        ref = int(nodes["ref"][ni, c])
+        if width == 2:
+            blo = np.array([nodes["x"][ni, 2 * c], nodes["y"][ni, 2 * c], nodes["z"][ni, 2 * c]])
+            bhi = np.array([nodes["x"][ni, 2 * c + 1], nodes["y"][ni, 2 * c + 1], nodes["z"][ni, 2 * c + 1]])
+        else:
+            blo = np.array([nodes["lox"][ni, c], nodes["loy"][ni, c], nodes["loz"][ni, c]])
+            bhi = np.array([nodes["hix"][ni, c], nodes["hiy"][ni, c], nodes["hiz"][ni, c]])
+        out.append((ref, blo, bhi))
+    return out
+
+
+@pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("leaf", [1, 4, 8])
-def test_bvh_structure(leaf):
+def test_bvh_structure(leaf, width):
+    """Every triangle sits in exactly one leaf of at most `leaf` triangles;
+    every child box contains every triangle of its subtree; inner nodes are
+    referenced exactly once (a tree, parents before children); BVH4 empty
+    slots carry an inverted box that no slab test can enter."""
     xyz, idx = scenes.displaced_sphere(level=4)
-    t = rq.host_tree(xyz, idx, "bvh", leaf_size=leaf)
+    t = rq.host_tree(xyz, idx, "bvh", leaf_size=leaf, width=width)
+    assert t["info"]["bvh_width"] == width
     nodes, order = t["nodes"], t["tri_order"]
     n = len(idx)
     assert sorted(order.tolist()) == list(range(n))
     tri = xyz[idx[order]]  # leaf order
     lo, hi = tri.min(1), tri.max(1)
     seen = np.zeros(n, np.int64)
-    for ni in range(len(nodes)):
-        for c in range(2):
-            ref = int(nodes["ref"][ni, c])
-            blo = np.array([nodes["x"][ni, 2 * c], nodes["y"][ni, 2 * c], nodes["z"][ni, 2 * c]])
-            bhi = np.array([nodes["x"][ni, 2 * c + 1], nodes["y"][ni, 2 * c + 1], nodes["z"][ni, 2 * c + 1]])
+    parents = np.zeros(len(nodes), np.int64)
+
+    def subtree(ni, depth):
+        """Leaf-order triangle slots under node ni."""
+        assert depth <= t["info"]["max_depth"] + 1
+        slots = []
+        for ref, blo, bhi in _children(nodes, ni, width):
+            if ref == rq.REF_EMPTY:
+                assert width == 4 and np.all(blo == np.inf) and np.all(bhi == -np.inf)
+                continue
             if ref & 0x80000000:
                 first, cnt = ref & 0x07FFFFFF, ((ref >> 27) & 15) + 1
                 assert cnt <= leaf
                 seen[first:first + cnt] += 1
-                assert np.all(lo[first:first + cnt] >= blo) and np.all(hi[first:first + cnt] <= bhi)
+                sub = list(range(first, first + cnt))
             else:
                 assert ni < ref < len(nodes)
+                parents[ref] += 1
+                sub = subtree(ref, depth + 1)
+            sub = np.array(sub)
+            assert np.all(lo[sub] >= blo) and np.all(hi[sub] <= bhi)
+            slots += sub.tolist()
+        return slots
+
+    assert sorted(subtree(0, 1)) == list(range(n))
     assert np.all(seen == 1)
+    assert np.all(parents[1:] == 1) and parents[0] == 0
+    if width == 4:
+        # The collapse fills nodes: few inner nodes keep fewer than 3 children.
+        used = (nodes["ref"] != rq.REF_EMPTY).sum(1)
+        assert np.mean(used) > 3.0
+        assert t["info"]["stack_bound"] >= 3 * t["info"]["max_depth"]
+
+
+def test_bvh4_single_leaf_scene():
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    t = rq.host_tree(xyz, np.array([[0, 1, 2]], np.uint32), "bvh", width=4)
+    ch = _children(t["nodes"], 0, 4)
+    assert ch[0][0] & 0x80000000 and all(c[0] == rq.REF_EMPTY for c in ch[1:])
 
 
 REF_SRC = "/root/reference/ENCORE/src"

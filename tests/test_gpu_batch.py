@@ -11,12 +11,12 @@ from helpers import log
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def two_scenes(device):
+@pytest.fixture(scope="module", params=[2, 4], ids=["bvh2", "bvh4"])
+def two_scenes(device, request):
     a = scenes.displaced_sphere(level=4)
     b = scenes.cornell()
-    sa = rq.Scene(device, *a, accel="bvh")
-    sb = rq.Scene(device, *b, accel="bvh")
+    sa = rq.Scene(device, *a, accel="bvh", width=request.param)
+    sb = rq.Scene(device, *b, accel="bvh", width=request.param)
     kda = oracle.KdTree(*a)
     prim = scenes.config2_primary(cols=160, rows=90)
     ray_a, _ = scenes.config3_rays(a[0], a[1], prim, kda.intersect(prim), count=30000, nbatches=2)
@@ -88,6 +88,21 @@ def test_batched_any_hit(device, two_scenes):
     torch.cuda.synchronize()
     assert torch.equal(out, plain)
     b.destroy()
+
+
+def test_batch_rejects_mixed_widths(device):
+    xyz, idx = scenes.cornell()
+    s2 = rq.Scene(device, xyz, idx, accel="bvh", width=2)
+    s4 = rq.Scene(device, xyz, idx, accel="bvh", width=4)
+    b = rq.Batch(device)
+    d = torch.zeros((4, 8), device="cuda")
+    o = torch.zeros((4, 4), device="cuda")
+    with pytest.raises(rq.RQError) as e:
+        b.set([(s2, d, o), (s4, d, o)])
+    assert e.value.status == 4
+    b.destroy()
+    s2.destroy()
+    s4.destroy()
 
 
 def test_batch_rejects_kd_scene(device):

@@ -15,8 +15,11 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def resident(device):
     out = []
-    for xyz, idx in (scenes.displaced_sphere(level=4), scenes.cornell(), scenes.displaced_sphere(freq=30)):
-        sc = rq.Scene(device, xyz, idx, accel="bvh")
+    # The third scene is a BVH2: requests on it are never batched with the
+    # BVH4 ones.
+    for (xyz, idx), width in ((scenes.displaced_sphere(level=4), 4), (scenes.cornell(), 4),
+                              (scenes.displaced_sphere(freq=30), 2)):
+        sc = rq.Scene(device, xyz, idx, accel="bvh", width=width)
         prim = scenes.config2_primary(cols=200, rows=100) if len(idx) > 100 else scenes.config1(res=256)[2]
         kd = oracle.KdTree(xyz, idx)
         bounce, _ = scenes.config3_rays(xyz, idx, prim, kd.intersect(prim), count=60000, nbatches=2)

@@ -70,10 +70,11 @@ def test_config1_kd_path_is_bit_exact(device, cornell):
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
-def test_config1_bvh_closest(device, cornell):
+@pytest.mark.parametrize("width", [2, 4])
+def test_config1_bvh_closest(device, cornell, width):
     xyz, idx, rays, ref = cornell
-    log("case: config1 512x512 bvh/encore closest")
-    sc = rq.Scene(device, xyz, idx, accel="bvh")
+    log("case: config1 512x512 bvh%d/encore closest" % width)
+    sc = rq.Scene(device, xyz, idx, accel="bvh", width=width)
     got = _trace(sc, rays)
     c = _check_vs_oracle(xyz, idx, rays, got, ref)
     log("config1 bvh:", {k: v for k, v in c.items() if k != "bad_idx"})
@@ -81,11 +82,12 @@ def test_config1_bvh_closest(device, cornell):
     assert c["exact"] + c["tie"] + c["ref_defect"] == c["n"]
 
 
+@pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("leaf", [1, 4, 8])
-def test_sphere_bvh_closest(device, sphere, leaf):
+def test_sphere_bvh_closest(device, sphere, leaf, width):
     s = sphere
-    log("case: sphere L5 bvh/encore leaf", leaf)
-    sc = rq.Scene(device, s["xyz"], s["idx"], accel="bvh", leaf_size=leaf)
+    log("case: sphere L5 bvh%d/encore leaf" % width, leaf)
+    sc = rq.Scene(device, s["xyz"], s["idx"], accel="bvh", leaf_size=leaf, width=width)
     for name in ("primary", "bounce"):
         rays, ref = s[name], s[name + "_ref"]
         got = _trace(sc, rays)
@@ -105,10 +107,11 @@ def test_sphere_kd_path_is_bit_exact(device, sphere):
     assert np.array_equal(occ.astype(np.uint32), s["shadow_ref"])
 
 
-def test_sphere_bvh_occluded(device, sphere):
+@pytest.mark.parametrize("width", [2, 4])
+def test_sphere_bvh_occluded(device, sphere, width):
     s = sphere
-    log("case: sphere L5 bvh/encore occluded")
-    sc = rq.Scene(device, s["xyz"], s["idx"], accel="bvh")
+    log("case: sphere L5 bvh%d/encore occluded" % width)
+    sc = rq.Scene(device, s["xyz"], s["idx"], accel="bvh", width=width)
     occ = _trace(sc, s["shadow"], "any").astype(np.uint32)
     diff = np.nonzero(occ != s["shadow_ref"])[0]
     if len(diff):
@@ -118,7 +121,8 @@ def test_sphere_bvh_occluded(device, sphere):
     assert occ.mean() > 0.01
 
 
-def test_watertight_bvh(device, cornell, sphere):
+@pytest.mark.parametrize("width", [2, 4])
+def test_watertight_bvh(device, cornell, sphere, width):
     """Watertight predicate: same closest hits within 1e-5 relative in t and
     the barycentrics, except ties, rays that graze triangle edges, and rays
     where the FP64 brute-force closest hit sides with the engine (ENCORE's
@@ -127,8 +131,8 @@ def test_watertight_bvh(device, cornell, sphere):
     for name, (xyz, idx, rays, ref) in (("cornell", cornell),
                                        ("sphere", (sphere["xyz"], sphere["idx"], sphere["bounce"],
                                                    sphere["bounce_ref"]))):
-        log("case: watertight bvh", name)
-        sc = rq.Scene(device, xyz, idx, accel="bvh", predicate="watertight")
+        log("case: watertight bvh%d" % width, name)
+        sc = rq.Scene(device, xyz, idx, accel="bvh", predicate="watertight", width=width)
         got = _trace(sc, rays)
         c = classify_closest(xyz, idx, rays, got, ref, bit_exact=False)
         c["fp64_agrees"], c["bad_idx"] = fp64_adjudicate(xyz, idx, rays, got, c["bad_idx"])
@@ -161,11 +165,12 @@ def test_fp64_error_report(device, sphere):
     assert max(err_u.max(), err_v.max()) < 1e-4
 
 
-def test_edge_cases(device):
+@pytest.mark.parametrize("width", [2, 4])
+def test_edge_cases(device, width):
     xyz, idx = scenes.cornell()
-    sc = rq.Scene(device, xyz, idx, accel="bvh")
+    sc = rq.Scene(device, xyz, idx, accel="bvh", width=width)
     kd = oracle.KdTree(xyz, idx)
-    log("case: edge cases")
+    log("case: edge cases bvh%d" % width)
     # n = 0 is a no-op
     out = torch.full((1, 4), 3.0, device="cuda")
     sc.intersect(torch.zeros((1, 8), device="cuda"), out, n=0)
