@@ -50,9 +50,9 @@ static bool prepare(rq_batch* b) {
     const int occ = occupancy_bvh_batched(b->predicate, kind_of(b), b->width, b->kc.stack);
     if (per_cu <= 0 || per_cu > occ) per_cu = occ;
     int64_t grid = (int64_t)per_cu * b->dev->num_cus;
-    grid = std::max<int64_t>(1, std::min<int64_t>(grid, ((int64_t)b->total + 255) / 256));
+    grid = std::max<int64_t>(1, std::min<int64_t>(grid, ((int64_t)b->total + kBvhBlock - 1) / kBvhBlock));
     b->grid = (int)grid;
-    const size_t need = 256 + (size_t)b->grid * 4 * 64 * b->spill_words * sizeof(uint32_t);
+    const size_t need = 256 + (size_t)b->grid * kBvhBlock * b->spill_words * sizeof(uint32_t);
     if (b->scratch.bytes < need) {
         RQ_HIP(hipDeviceSynchronize());
         b->scratch = DevBuf(need);
@@ -75,7 +75,7 @@ static TraceLaunch launch_params(const rq_batch* b) {
 
 static void enqueue(rq_batch* b, hipStream_t s) {
     const TraceLaunch L = launch_params(b);
-    launch_zero_u32(L.counter, 1, s);
+    launch_zero_u32(L.counter, 2, s);
     launch_bvh_batched(L, b->predicate, kind_of(b), b->width, b->grid, b->kc, s);
 }
 

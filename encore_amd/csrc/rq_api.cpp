@@ -89,7 +89,8 @@ static int grid_for(rq_device* dev, uint32_t accel, uint32_t pred, TraceKind kin
     const KernelConfig& kc = dev->kc;
     int per_cu = kc.blocks_per_cu > 0 ? kc.blocks_per_cu : occupancy_blocks_per_cu(accel, pred, kind, width, kc);
     int64_t grid = (int64_t)per_cu * dev->num_cus;
-    const int64_t need = ((int64_t)n + 255) / 256;
+    const int64_t bt = block_threads(accel);
+    const int64_t need = ((int64_t)n + bt - 1) / bt;
     grid = std::max<int64_t>(1, std::min(grid, need));
     return (int)grid;
 }
@@ -101,7 +102,7 @@ static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, ui
     RQ_HIP(hipSetDevice(dev->ordinal));
     const int grid = grid_for(dev, sc->accel, sc->predicate, kind, sc->width, n);
     const uint32_t words = scene_spill_words(sc);
-    const size_t spill_bytes = (size_t)grid * 4 * 64 * words * sizeof(uint32_t);
+    const size_t spill_bytes = (size_t)grid * block_threads(sc->accel) * words * sizeof(uint32_t);
     uint32_t* scratch = stream_scratch(dev, s, spill_bytes);
     TraceLaunch L{};
     L.nodes = sc->nodes.ptr;
@@ -120,7 +121,7 @@ static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, ui
     }
     L.counts = counts;
     L.fetch_threshold = (uint32_t)(sc->accel == RQ_ACCEL_KD ? dev->kc.kd_fetch_threshold : dev->kc.fetch_threshold);
-    launch_zero_u32(scratch, 1, s);
+    launch_zero_u32(scratch, 2, s);
     if (counts)
         launch_count(L, sc->accel, sc->predicate, kind, sc->width, grid, s);
     else if (sc->accel == RQ_ACCEL_KD)

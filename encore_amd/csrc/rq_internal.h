@@ -261,6 +261,13 @@ struct KernelConfig {
     int kd_fetch_threshold = 16;  // kd path (its refill claims rays with an atomic per round)
 };
 
+// Threads per block: the BVH kernels run one wave per block (their LDS use
+// per wave is large, so single-wave blocks pack a CU best); the kd kernel
+// four.
+constexpr int kBvhBlock = 64;
+constexpr int kKdBlock = 256;
+inline int block_threads(uint32_t accel) { return accel == RQ_ACCEL_KD ? kKdBlock : kBvhBlock; }
+
 // Launchers (traverse.hip).
 void launch_bvh(const TraceLaunch& L, uint32_t predicate, TraceKind kind, uint32_t width, int grid,
                 const KernelConfig& kc, hipStream_t s);

@@ -174,7 +174,7 @@ void issue(rq_server* s, Slot& sl) {
     }
     // The slot's stream is idle here (its previous launch was retired), so
     // its scratch can be replaced.
-    const size_t need = 256 + (size_t)grid_max * 4 * 64 * words * sizeof(uint32_t);
+    const size_t need = 256 + (size_t)grid_max * kBvhBlock * words * sizeof(uint32_t);
     if (sl.scratch.bytes < need) sl.scratch = DevBuf(need);
     for (Request* r : sl.reqs) {
         char* base = s->arena.ptr(r->off);
@@ -196,8 +196,8 @@ void issue(rq_server* s, Slot& sl) {
         L.spill = sl.scratch.as<uint32_t>() + 64;
         L.spill_depth = words;
         L.fetch_threshold = (uint32_t)s->dev->kc.fetch_threshold;
-        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(grid_max, ((int64_t)r->n + 255) / 256));
-        launch_zero_u32(L.counter, 1, st);
+        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(grid_max, ((int64_t)r->n + kBvhBlock - 1) / kBvhBlock));
+        launch_zero_u32(L.counter, 2, st);
         launch_bvh(L, pred, kind, width, (int)grid, s->dev->kc, st);
     } else {
         uint32_t i = 0;
@@ -225,8 +225,8 @@ void issue(rq_server* s, Slot& sl) {
         L.items = sl.table.as<BatchItemDev>();
         L.nitems = i;
         L.fetch_threshold = (uint32_t)s->kc.fetch_threshold;
-        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(grid_max, ((int64_t)total + 255) / 256));
-        launch_zero_u32(L.counter, 1, st);
+        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(grid_max, ((int64_t)total + kBvhBlock - 1) / kBvhBlock));
+        launch_zero_u32(L.counter, 2, st);
         launch_bvh_batched(L, pred, kind, width, (int)grid, s->kc, st);
     }
     for (Request* r : sl.reqs) {

@@ -332,6 +332,15 @@ __device__ __forceinline__ uint32_t chunk_item(const BatchItemDev* items, uint32
     return lo;
 }
 
+// One traversal step per lane per iteration, through shared loads.  Every
+// busy lane fetches one record with the same NL 16-byte load instructions --
+// a node (BVH4: the sign-selected near/far rows and the refs, 7 loads; BVH2:
+// the 64-byte node, 4 loads) or the next TPS triangles of its current leaf
+// (BVH4: 2 triangles in 6 loads; BVH2: 1 in 3) -- and then runs the node or
+// the triangle arithmetic.  Node and leaf lanes share the load instructions
+// and the loop, so there are no separate node and leaf phases whose lanes
+// wait for each other.
+//
 // WIDTH 2: BvhNode (both child boxes in one 64-byte node), stack entries are
 // bare refs.  WIDTH 4: BvhNode4 (four child boxes, 128 bytes), the hit
 // children are sorted by entry distance with a 5-comparator network, the
@@ -339,38 +348,43 @@ __device__ __forceinline__ uint32_t chunk_item(const BatchItemDev* items, uint32
 // with their entry distance; a popped entry whose entry distance is beyond
 // the current closest hit is dropped (the box test would now fail).
 //
-// Lanes leave the trace loop when their ray ends; the wave returns to the
-// refill step once fewer than fetch_threshold lanes are still tracing.
+// One wave per block.  Idle lanes are refilled from the staged chunk
+// whenever fewer than fetch_threshold lanes are busy.
 template <class Pred, bool ANY, int S, bool COUNT = false, bool BATCHED = false, int WIDTH = 2>
-__global__ __launch_bounds__(256) void bvh_trace(TraceLaunch L) {
-    constexpr int SW = WIDTH == 4 ? 2 : 1;  // dwords per stack entry
-    __shared__ uint32_t lds_stack[256 / kWave][S * SW][kWave];
-    __shared__ float lds_stage[256 / kWave][kStageFields][kWave];
+__global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
+    constexpr int SW = WIDTH == 4 ? 2 : 1;      // dwords per stack entry
+    constexpr int NL = WIDTH == 4 ? 7 : 4;      // 16-byte rows a lane reads from its record
+    constexpr int TPS = WIDTH == 4 ? 2 : 1;     // triangles per leaf step
+    __shared__ uint32_t lds_stack[S * SW][kWave];
+    __shared__ float lds_stage[kStageFields][kWave];
     const int lane = lane_id();
-    const int wave = threadIdx.x / kWave;
-    const uint32_t gwave = blockIdx.x * (blockDim.x / kWave) + wave;
+    const uint32_t gwave = blockIdx.x;
     RingStack<S, SW> stk;
-    stk.lds = &lds_stack[wave][0][lane];
+    stk.lds = &lds_stack[0][lane];
     stk.spill = L.spill + (size_t)gwave * L.spill_depth * kWave + lane;
-    Stage stage{&lds_stage[wave][0][0]};
+    Stage stage{&lds_stage[0][0]};
     // Per-lane scene and output: uniform in a plain launch, per item in a
     // batched one.
     const void* nodes = L.nodes;
     const TriRec* tris = L.tris;
     void* out = ANY ? static_cast<void*>(L.occluded) : static_cast<void*>(L.hits);
+    const float kInf = __int_as_float(0x7f800000);
 
     RayState r;
     float tmin = 0, tbest = 0, bu = 0, bv = 0;
     uint32_t bprim = kNone;
     uint32_t local = kNone;  // item-local ray index; kNone = idle lane
-    uint32_t cur = kRefSentinel, pend = kNone;
+    // cur: next record (inner node, leaf ref or kRefSentinel); [lpos, lend):
+    // triangles of the leaf in progress.
+    uint32_t cur = kRefSentinel, lpos = 0, lend = 0;
     uint32_t n_nodes = 0, n_tris = 0;  // COUNT only
-    // COUNT only: wave iterations of the node loop, of the triangle loop and
-    // of the refill round, each tallied by the lowest active lane.
+    // COUNT only: iterations, iterations with a leaf lane, refill rounds
+    // (tallied by lane 0).
     uint32_t w_node = 0, w_tri = 0, w_round = 0;
-    uint32_t offx = 0, offy = 0, offz = 0;  // WIDTH 4: near-plane rows picked by direction sign
-    // Wave-uniform chunk state.
-    uint32_t ch_base = 0, ch_n = 0, ch_used = 0;
+    uint32_t offx = 0, offy = 0, offz = 0;  // near-plane rows picked by direction sign
+    // Wave-uniform chunk state: the claimed id range [cl_base, cl_end) and
+    // the staged chunk (ch_n rays from ch_base, ch_used handed out).
+    uint32_t cl_base = 0, cl_end = 0, ch_base = 0, ch_n = 0, ch_used = 0;
     bool exhausted = false;
 
     // Next stacked ref, skipping (WIDTH 4) entries the closest hit has passed.
@@ -383,116 +397,182 @@ __global__ __launch_bounds__(256) void bvh_trace(TraceLaunch L) {
     };
 
     for (;;) {
-        // Refill idle lanes, staging new chunks as needed.
-        for (;;) {
-            const uint64_t idle = __ballot(local == kNone);
-            if (idle == 0) break;
-            if (ch_used == ch_n) {
-                if (exhausted) break;
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(L.counter, (uint32_t)kWave);
-                base = __builtin_amdgcn_readfirstlane(base);
-                if (base >= L.n) {
-                    exhausted = true;
-                    break;
-                }
-                ch_base = base;
-                ch_n = min((uint32_t)kWave, L.n - base);
-                ch_used = 0;
-                if ((uint32_t)lane < ch_n) {
-                    const uint32_t id = base + lane;
-                    const rq_ray* rin = L.rays;
-                    uint32_t loc = id, item = 0;
-                    if (BATCHED) {
-                        item = chunk_item(L.items, L.nitems, base, id);
-                        const BatchItemDev* it = L.items + item;
-                        rin = it->rays;
-                        loc = id - it->begin;
+        // Refill idle lanes once fewer than fetch_threshold are busy (or the
+        // wave is empty), staging new chunks as needed.  Ids are claimed in
+        // guided ranges -- a quarter of this wave's share of what was left at
+        // its last claim, 64..2048 ids -- so the shared counter sees a few
+        // atomics per wave rather than one per 64 rays.
+        const uint64_t idle0 = __ballot(local == kNone);
+        if (!exhausted && (idle0 == ~0ull || 64u - (uint32_t)__popcll(idle0) < L.fetch_threshold)) {
+            if (COUNT && lane == 0) ++w_round;
+            for (;;) {
+                const uint64_t idle = __ballot(local == kNone);
+                if (idle == 0) break;
+                if (ch_used == ch_n) {
+                    if (cl_base >= cl_end) {
+                        const uint32_t left = L.n - min(L.n, cl_end);
+                        const uint32_t want =
+                            min(2048u, max((uint32_t)kWave, (left / (4u * gridDim.x)) & ~(uint32_t)(kWave - 1)));
+                        // 64-bit counter: it runs past n by up to a claim per
+                        // wave, which must not wrap for n close to 2^32.
+                        unsigned long long b = 0;
+                        if (lane == 0)
+                            b = atomicAdd(reinterpret_cast<unsigned long long*>(L.counter), (unsigned long long)want);
+                        b = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(b >> 32)) << 32) |
+                            __builtin_amdgcn_readfirstlane((uint32_t)b);
+                        if (b >= L.n) {
+                            exhausted = true;
+                            break;
+                        }
+                        cl_base = (uint32_t)b;
+                        cl_end = (uint32_t)min((unsigned long long)L.n, b + want);
                     }
-                    const float4 p = ldg4(&rin[loc].ox);
-                    const float4 q = ldg4(&rin[loc].dx);
-                    stage.at(0, lane) = p.x;
-                    stage.at(1, lane) = p.y;
-                    stage.at(2, lane) = p.z;
-                    stage.at(3, lane) = p.w;
-                    stage.at(4, lane) = q.x;
-                    stage.at(5, lane) = q.y;
-                    stage.at(6, lane) = q.z;
-                    stage.at(7, lane) = q.w;
-                    stage.at(8, lane) = 1.0f / q.x;
-                    stage.at(9, lane) = 1.0f / q.y;
-                    stage.at(10, lane) = 1.0f / q.z;
-                    stage.at(11, lane) = __uint_as_float(loc);
-                    stage.at(12, lane) = __uint_as_float(item);
+                    const uint32_t base = cl_base;
+                    ch_base = base;
+                    ch_n = min((uint32_t)kWave, cl_end - base);
+                    cl_base += ch_n;
+                    ch_used = 0;
+                    if ((uint32_t)lane < ch_n) {
+                        const uint32_t id = base + lane;
+                        const rq_ray* rin = L.rays;
+                        uint32_t loc = id, item = 0;
+                        if (BATCHED) {
+                            item = chunk_item(L.items, L.nitems, base, id);
+                            const BatchItemDev* it = L.items + item;
+                            rin = it->rays;
+                            loc = id - it->begin;
+                        }
+                        const float4 p = ldg4(&rin[loc].ox);
+                        const float4 q = ldg4(&rin[loc].dx);
+                        stage.at(0, lane) = p.x;
+                        stage.at(1, lane) = p.y;
+                        stage.at(2, lane) = p.z;
+                        stage.at(3, lane) = p.w;
+                        stage.at(4, lane) = q.x;
+                        stage.at(5, lane) = q.y;
+                        stage.at(6, lane) = q.z;
+                        stage.at(7, lane) = q.w;
+                        stage.at(8, lane) = 1.0f / q.x;
+                        stage.at(9, lane) = 1.0f / q.y;
+                        stage.at(10, lane) = 1.0f / q.z;
+                        stage.at(11, lane) = __uint_as_float(loc);
+                        stage.at(12, lane) = __uint_as_float(item);
+                    }
                 }
+                const uint32_t take = min((uint32_t)__popcll(idle), ch_n - ch_used);
+                const uint32_t rank = lanes_below(idle);
+                if (local == kNone && rank < take) {
+                    const uint32_t slot = ch_used + rank;
+                    r.ox = stage.at(0, slot);
+                    r.oy = stage.at(1, slot);
+                    r.oz = stage.at(2, slot);
+                    tmin = stage.at(3, slot);
+                    r.dx = stage.at(4, slot);
+                    r.dy = stage.at(5, slot);
+                    r.dz = stage.at(6, slot);
+                    tbest = stage.at(7, slot);
+                    r.ix = stage.at(8, slot);
+                    r.iy = stage.at(9, slot);
+                    r.iz = stage.at(10, slot);
+                    local = __float_as_uint(stage.at(11, slot));
+                    if (BATCHED) {
+                        const BatchItemDev* it = L.items + __float_as_uint(stage.at(12, slot));
+                        const uint4 pp = ldgu4(&it->nodes);
+                        nodes = reinterpret_cast<const void*>(((uint64_t)pp.y << 32) | pp.x);
+                        tris = reinterpret_cast<const TriRec*>(((uint64_t)pp.w << 32) | pp.z);
+                        const uint4 oo = ldgu4(&it->hits);
+                        out = reinterpret_cast<void*>(ANY ? (((uint64_t)oo.w << 32) | oo.z)
+                                                          : (((uint64_t)oo.y << 32) | oo.x));
+                    }
+                    Pred::prepare(r);
+                    if (WIDTH == 4) {
+                        offx = __builtin_signbit(r.dx) ? 4u : 0u;
+                        offy = __builtin_signbit(r.dy) ? 12u : 8u;
+                        offz = __builtin_signbit(r.dz) ? 20u : 16u;
+                    } else {
+                        offx = offy = offz = 0u;
+                    }
+                    bu = bv = 0.0f;
+                    bprim = kNone;
+                    cur = 0;
+                    lpos = lend = 0;
+                    stk.reset();
+                }
+                ch_used += take;
             }
-            const uint32_t take = min((uint32_t)__popcll(idle), ch_n - ch_used);
-            const uint32_t rank = lanes_below(idle);
-            if (local == kNone && rank < take) {
-                const uint32_t slot = ch_used + rank;
-                r.ox = stage.at(0, slot);
-                r.oy = stage.at(1, slot);
-                r.oz = stage.at(2, slot);
-                tmin = stage.at(3, slot);
-                r.dx = stage.at(4, slot);
-                r.dy = stage.at(5, slot);
-                r.dz = stage.at(6, slot);
-                tbest = stage.at(7, slot);
-                r.ix = stage.at(8, slot);
-                r.iy = stage.at(9, slot);
-                r.iz = stage.at(10, slot);
-                local = __float_as_uint(stage.at(11, slot));
-                if (BATCHED) {
-                    const BatchItemDev* it = L.items + __float_as_uint(stage.at(12, slot));
-                    const uint4 pp = ldgu4(&it->nodes);
-                    nodes = reinterpret_cast<const void*>(((uint64_t)pp.y << 32) | pp.x);
-                    tris = reinterpret_cast<const TriRec*>(((uint64_t)pp.w << 32) | pp.z);
-                    const uint4 oo = ldgu4(&it->hits);
-                    out = reinterpret_cast<void*>(ANY ? (((uint64_t)oo.w << 32) | oo.z)
-                                                      : (((uint64_t)oo.y << 32) | oo.x));
-                }
-                Pred::prepare(r);
-                if (WIDTH == 4) {
-                    offx = __builtin_signbit(r.dx) ? 4u : 0u;
-                    offy = __builtin_signbit(r.dy) ? 12u : 8u;
-                    offz = __builtin_signbit(r.dz) ? 20u : 16u;
-                }
-                bu = bv = 0.0f;
-                bprim = kNone;
-                cur = 0;
-                stk.reset();
-            }
-            ch_used += take;
         }
-        if (__ballot(local != kNone) == 0) break;  // no work left for this wave
-        const uint32_t thresh = exhausted ? 0u : L.fetch_threshold;
-        if (COUNT && lane == 0) ++w_round;
+        if (__ballot(local != kNone) == 0) {
+            if (exhausted) break;  // no work left for this wave
+            continue;
+        }
+        if (COUNT && lane == 0) ++w_node;
 
-        // One round: node phase, leaf phase, retire.  A lane keeps its
-        // traversal state (cur, pend, stack) across rounds.
+        bool did_leaf = false;  // COUNT only
+
         if (local != kNone) {
-            // Node phase: until every lane still here has a leaf postponed,
-            // or fewer than `thresh` lanes are still traversing (then the
-            // wave goes on to refill the lanes whose rays ended).
-            while (cur != kRefSentinel && (int)cur >= 0) {
-                if (COUNT) {
-                    ++n_nodes;
-                    if (lane == __ffsll((unsigned long long)__ballot(true)) - 1) ++w_node;
+            // A leaf ref in cur becomes the leaf in progress.
+            if (lpos >= lend && (int)cur < 0 && cur != kRefSentinel) {
+                lpos = cur & 0x07FFFFFFu;
+                lend = lpos + ((cur >> 27) & 15u) + 1u;
+                cur = kNone;  // popped when the leaf is done
+            }
+            const bool leaf = lpos < lend;
+            // Record addresses: node rows, or the leaf's next TPS triangles
+            // (a missing second triangle repeats the first).
+            const float* np = WIDTH == 4
+                                  ? reinterpret_cast<const float*>(static_cast<const BvhNode4*>(nodes) + cur)
+                                  : reinterpret_cast<const float*>(static_cast<const BvhNode*>(nodes) + cur);
+            const float* tp = reinterpret_cast<const float*>(tris + lpos);
+            const bool two = TPS == 2 && lpos + 1 < lend;
+            float4 d[NL];
+            if (WIDTH == 4) {
+                const uint32_t o2 = two ? 12u : 0u;
+                d[0] = ldg4(leaf ? tp : np + offx);
+                d[1] = ldg4(leaf ? tp + 4 : np + (offx ^ 4u));
+                d[2] = ldg4(leaf ? tp + 8 : np + offy);
+                d[3] = ldg4(leaf ? tp + o2 : np + (offy ^ 4u));
+                d[4] = ldg4(leaf ? tp + o2 + 4 : np + offz);
+                d[5] = ldg4(leaf ? tp + o2 + 8 : np + (offz ^ 4u));
+                d[6] = ldg4(leaf ? tp : np + 24);
+            } else {
+                d[0] = ldg4(leaf ? tp : np);
+                d[1] = ldg4(leaf ? tp + 4 : np + 4);
+                d[2] = ldg4(leaf ? tp + 8 : np + 8);
+                d[3] = ldg4(leaf ? tp : np + 12);
+            }
+            bool finished = false;
+            did_leaf = leaf;
+            if (leaf) {
+#pragma unroll
+                for (int k = 0; k < TPS; ++k) {
+                    if (k == 0 || two) {
+                        if (COUNT) ++n_tris;
+                        const float4 a = d[3 * k], b = d[3 * k + 1], c = d[3 * k + 2];
+                        float t, u, v;
+                        if (Pred::test(r, a, b, c, tmin, t, u, v) && t < tbest) {
+                            tbest = t;
+                            bu = u;
+                            bv = v;
+                            bprim = __float_as_uint(a.w);
+                            if (ANY) finished = true;
+                        }
+                    }
                 }
+                lpos += TPS;
+                if (!finished && lpos >= lend) cur = pop_next();
+            } else {
+                if (COUNT) ++n_nodes;
                 if constexpr (WIDTH == 2) {
-                    const BvhNode* np = static_cast<const BvhNode*>(nodes) + cur;
-                    const float4 nx = ldg4(np->x), ny = ldg4(np->y), nz = ldg4(np->z);
-                    const uint4 meta = ldgu4(np->ref);
+                    const uint4 meta = make_uint4(__float_as_uint(d[3].x), __float_as_uint(d[3].y),
+                                                  __float_as_uint(d[3].z), __float_as_uint(d[3].w));
                     float tn0, tn1;
                     bool h1;
-                    const bool h0 = box2(r, nx, ny, nz, tmin, tbest, tn0, tn1, h1);
+                    const bool h0 = box2(r, d[0], d[1], d[2], tmin, tbest, tn0, tn1, h1);
                     if (h0 && h1) {
                         const bool first0 = tn0 <= tn1;
-                        const uint32_t nearr = first0 ? meta.x : meta.y;
-                        const uint32_t farr = first0 ? meta.y : meta.x;
-                        uint32_t v[SW] = {farr};
+                        uint32_t v[SW] = {first0 ? meta.y : meta.x};
                         stk.push(v);
-                        cur = nearr;
+                        cur = first0 ? meta.x : meta.y;
                     } else if (h0) {
                         cur = meta.x;
                     } else if (h1) {
@@ -501,17 +581,12 @@ __global__ __launch_bounds__(256) void bvh_trace(TraceLaunch L) {
                         cur = pop_next();
                     }
                 } else {
-                    const float* np = reinterpret_cast<const float*>(static_cast<const BvhNode4*>(nodes) + cur);
-                    const float4 nx = ldg4(np + offx), fx = ldg4(np + (offx ^ 4u));
-                    const float4 ny = ldg4(np + offy), fy = ldg4(np + (offy ^ 4u));
-                    const float4 nz = ldg4(np + offz), fz = ldg4(np + (offz ^ 4u));
-                    const uint4 rf = ldgu4(np + 24);
-                    const float kInf = __int_as_float(0x7f800000);
                     float k[4];
-                    uint32_t c[4] = {rf.x, rf.y, rf.z, rf.w};
-                    const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
-                    const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
-                    const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
+                    uint32_t c[4] = {__float_as_uint(d[6].x), __float_as_uint(d[6].y), __float_as_uint(d[6].z),
+                                     __float_as_uint(d[6].w)};
+                    const float nxa[4] = {d[0].x, d[0].y, d[0].z, d[0].w}, fxa[4] = {d[1].x, d[1].y, d[1].z, d[1].w};
+                    const float nya[4] = {d[2].x, d[2].y, d[2].z, d[2].w}, fya[4] = {d[3].x, d[3].y, d[3].z, d[3].w};
+                    const float nza[4] = {d[4].x, d[4].y, d[4].z, d[4].w}, fza[4] = {d[5].x, d[5].y, d[5].z, d[5].w};
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const float tnx = (nxa[j] - r.ox) * r.ix, tfx = (fxa[j] - r.ox) * r.ix;
@@ -528,86 +603,50 @@ __global__ __launch_bounds__(256) void bvh_trace(TraceLaunch L) {
                     cswap(k[1], c[1], k[2], c[2]);
                     if (k[0] < kInf) {
                         // Push children 1..nh-1 far to near: child j lands
-                        // at offset np - j from the current top.
-                        const int np = (k[1] < kInf) + (k[2] < kInf) + (k[3] < kInf);
-                        stk.reserve(np);
-                        if (np >= 3) {
+                        // at offset npush - j from the current top.
+                        const int npush = (k[1] < kInf) + (k[2] < kInf) + (k[3] < kInf);
+                        stk.reserve(npush);
+                        if (npush >= 3) {
                             uint32_t v[SW] = {c[3], __float_as_uint(k[3])};
                             stk.put(0, v);
                         }
-                        if (np >= 2) {
+                        if (npush >= 2) {
                             uint32_t v[SW] = {c[2], __float_as_uint(k[2])};
-                            stk.put(np - 2, v);
+                            stk.put(npush - 2, v);
                         }
-                        if (np >= 1) {
+                        if (npush >= 1) {
                             uint32_t v[SW] = {c[1], __float_as_uint(k[1])};
-                            stk.put(np - 1, v);
+                            stk.put(npush - 1, v);
                         }
-                        stk.advance(np);
+                        stk.advance(npush);
                         cur = c[0];
                     } else {
                         cur = pop_next();
                     }
                 }
-                if ((int)cur < 0 && pend == kNone) {
-                    pend = cur;
-                    cur = pop_next();
-                }
-                const uint64_t busy = __ballot(true);
-                if (__ballot(pend == kNone) == 0 || (uint32_t)__popcll(busy) < thresh) break;
             }
-            // Leaf phase.
-            while (pend != kNone) {
-                const uint32_t first = pend & 0x07FFFFFFu;
-                const uint32_t count = ((pend >> 27) & 15u) + 1u;
-                bool done = false;
-                for (uint32_t i = 0; i < count; ++i) {
-                    const TriRec* tp = tris + first + i;
-                    if (COUNT) {
-                        ++n_tris;
-                        if (lane == __ffsll((unsigned long long)__ballot(true)) - 1) ++w_tri;
-                    }
-                    const float4 a = ldg4(tp->a), b = ldg4(tp->b), c = ldg4(tp->c);
-                    float t, u, v;
-                    if (Pred::test(r, a, b, c, tmin, t, u, v) && t < tbest) {
-                        tbest = t;
-                        bu = u;
-                        bv = v;
-                        bprim = __float_as_uint(a.w);
-                        if (ANY) {
-                            done = true;
-                            break;
-                        }
-                    }
-                }
-                if (ANY && done) {
-                    pend = kNone;
-                    cur = kRefSentinel;
-                    break;
-                }
-                if (cur != kRefSentinel && (int)cur < 0) {
-                    pend = cur;
-                    cur = pop_next();
+            if (ANY && finished) {
+                cur = kRefSentinel;
+                lpos = lend;
+            }
+            // Retire a finished ray.
+            if (cur == kRefSentinel && lpos >= lend) {
+                if (ANY) {
+                    stg1(static_cast<uint32_t*>(out) + local, (bprim != kNone) ? 1u : 0u);
                 } else {
-                    pend = kNone;
+                    const bool has = bprim != kNone;
+                    stg4(static_cast<rq_hit*>(out) + local,
+                         make_float4(tbest, has ? bu : 0.0f, has ? bv : 0.0f, __uint_as_float(bprim)));
                 }
+                if (COUNT) {
+                    atomicAdd(&L.counts[0], (unsigned long long)n_nodes);
+                    atomicAdd(&L.counts[1], (unsigned long long)n_tris);
+                    n_nodes = n_tris = 0;
+                }
+                local = kNone;
             }
         }
-        if (local != kNone && cur == kRefSentinel && pend == kNone) {
-            if (ANY) {
-                stg1(static_cast<uint32_t*>(out) + local, (bprim != kNone) ? 1u : 0u);
-            } else {
-                const bool has = bprim != kNone;
-                stg4(static_cast<rq_hit*>(out) + local,
-                     make_float4(tbest, has ? bu : 0.0f, has ? bv : 0.0f, __uint_as_float(bprim)));
-            }
-            if (COUNT) {
-                atomicAdd(&L.counts[0], (unsigned long long)n_nodes);
-                atomicAdd(&L.counts[1], (unsigned long long)n_tris);
-                n_nodes = n_tris = 0;
-            }
-            local = kNone;
-        }
+        if (COUNT && __ballot(did_leaf) != 0 && lane == 0) ++w_tri;
     }
     if (COUNT) {
         if (w_node) atomicAdd(&L.counts[2], (unsigned long long)w_node);
@@ -864,40 +903,40 @@ static const void* kd_fn(uint32_t predicate, TraceKind kind, bool count) {
 #undef RQ_KD
 }
 
-static void launch_trace(const void* fn, const TraceLaunch& L, int grid, hipStream_t s) {
+static void launch_trace(const void* fn, const TraceLaunch& L, int grid, int block, hipStream_t s) {
     TraceLaunch arg = L;
     void* args[] = {&arg};
-    RQ_HIP(hipLaunchKernel(fn, dim3(grid), dim3(256), args, 0, s));
+    RQ_HIP(hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, s));
 }
 
-static int occupancy_of(const void* fn) {
+static int occupancy_of(const void* fn, int block) {
     int nb = 0;
-    RQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0));
+    RQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0));
     return nb > 0 ? nb : 1;
 }
 
 void launch_bvh(const TraceLaunch& L, uint32_t predicate, TraceKind kind, uint32_t width, int grid,
                 const KernelConfig& kc, hipStream_t s) {
-    launch_trace(bvh_fn(predicate, kind, false, false, width, kc.stack), L, grid, s);
+    launch_trace(bvh_fn(predicate, kind, false, false, width, kc.stack), L, grid, kBvhBlock, s);
 }
 
 void launch_bvh_batched(const TraceLaunch& L, uint32_t predicate, TraceKind kind, uint32_t width, int grid,
                         const KernelConfig& kc, hipStream_t s) {
-    launch_trace(bvh_fn(predicate, kind, false, true, width, kc.stack), L, grid, s);
+    launch_trace(bvh_fn(predicate, kind, false, true, width, kc.stack), L, grid, kBvhBlock, s);
 }
 
 int occupancy_bvh_batched(uint32_t predicate, TraceKind kind, uint32_t width, int stack) {
-    return occupancy_of(bvh_fn(predicate, kind, false, true, width, stack));
+    return occupancy_of(bvh_fn(predicate, kind, false, true, width, stack), kBvhBlock);
 }
 
 void launch_kd(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, hipStream_t s) {
-    launch_trace(kd_fn(predicate, kind, false), L, grid, s);
+    launch_trace(kd_fn(predicate, kind, false), L, grid, kKdBlock, s);
 }
 
 void launch_count(const TraceLaunch& L, uint32_t accel, uint32_t predicate, TraceKind kind, uint32_t width, int grid,
                   hipStream_t s) {
     launch_trace(accel == RQ_ACCEL_KD ? kd_fn(predicate, kind, true) : bvh_fn(predicate, kind, true, false, width, 16),
-                 L, grid, s);
+                 L, grid, block_threads(accel), s);
 }
 
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
@@ -908,7 +947,8 @@ void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
 int occupancy_blocks_per_cu(uint32_t accel, uint32_t predicate, TraceKind kind, uint32_t width,
                             const KernelConfig& kc) {
     return occupancy_of(accel == RQ_ACCEL_KD ? kd_fn(predicate, kind, false)
-                                             : bvh_fn(predicate, kind, false, false, width, kc.stack));
+                                             : bvh_fn(predicate, kind, false, false, width, kc.stack),
+                        block_threads(accel));
 }
 
 uint32_t stack_entry_words(uint32_t accel, uint32_t width) {
