@@ -124,10 +124,28 @@ struct RingStack {
 struct RayState {
     float ox, oy, oz, dx, dy, dz;
     float ix, iy, iz;  // 1/d
+    // BVH slab offsets: a plane b is entered at fma(b, i, cn) or left at
+    // fma(b, i, cf) (see slab_offsets)
+    float cnx, cny, cnz, cfx, cfy, cfz;
     // watertight shear
     int kx, ky, kz;
     float sx, sy, sz;
 };
+
+// Conservative slab distances with one fma per plane.  The exact distance
+// to plane b is (b - o) i = b i - P with P = o i.  With p = rn(P) and
+// D = |p| 2^-22 (at least two ulps of P), cn = -p - D and cf = -p + D keep
+// fma(b, i, cn) at or below, and fma(b, i, cf) at or above, the exact
+// distance up to the final rounding of the fma (half an ulp of the result,
+// covered by the relative widening applied to the far distance).  For d = 0
+// (i infinite) the fma yields NaN or an infinity of the side that does not
+// cull, so such a slab never rejects a box.
+__device__ __forceinline__ void slab_offsets(float o, float i, float& cn, float& cf) {
+    const float p = o * i;
+    const float D = fabsf(p) * 2.38418579e-07f;
+    cn = -p - D;
+    cf = -p + D;
+}
 
 __device__ __forceinline__ float flush_small(float x) { return (x > -0.001f && x < 0.001f) ? 0.0f : x; }
 
@@ -256,19 +274,19 @@ __device__ __forceinline__ bool box2(const RayState& r, const float4 nx, const f
                                      float tbest, float& tn0, float& tn1, bool& h1) {
     const bool sx = __builtin_signbit(r.dx), sy = __builtin_signbit(r.dy), sz = __builtin_signbit(r.dz);
     // child 0
-    const float n0x = ((sx ? nx.y : nx.x) - r.ox) * r.ix;
-    const float f0x = ((sx ? nx.x : nx.y) - r.ox) * r.ix;
-    const float n0y = ((sy ? ny.y : ny.x) - r.oy) * r.iy;
-    const float f0y = ((sy ? ny.x : ny.y) - r.oy) * r.iy;
-    const float n0z = ((sz ? nz.y : nz.x) - r.oz) * r.iz;
-    const float f0z = ((sz ? nz.x : nz.y) - r.oz) * r.iz;
+    const float n0x = __builtin_fmaf(sx ? nx.y : nx.x, r.ix, r.cnx);
+    const float f0x = __builtin_fmaf(sx ? nx.x : nx.y, r.ix, r.cfx);
+    const float n0y = __builtin_fmaf(sy ? ny.y : ny.x, r.iy, r.cny);
+    const float f0y = __builtin_fmaf(sy ? ny.x : ny.y, r.iy, r.cfy);
+    const float n0z = __builtin_fmaf(sz ? nz.y : nz.x, r.iz, r.cnz);
+    const float f0z = __builtin_fmaf(sz ? nz.x : nz.y, r.iz, r.cfz);
     // child 1
-    const float n1x = ((sx ? nx.w : nx.z) - r.ox) * r.ix;
-    const float f1x = ((sx ? nx.z : nx.w) - r.ox) * r.ix;
-    const float n1y = ((sy ? ny.w : ny.z) - r.oy) * r.iy;
-    const float f1y = ((sy ? ny.z : ny.w) - r.oy) * r.iy;
-    const float n1z = ((sz ? nz.w : nz.z) - r.oz) * r.iz;
-    const float f1z = ((sz ? nz.z : nz.w) - r.oz) * r.iz;
+    const float n1x = __builtin_fmaf(sx ? nx.w : nx.z, r.ix, r.cnx);
+    const float f1x = __builtin_fmaf(sx ? nx.z : nx.w, r.ix, r.cfx);
+    const float n1y = __builtin_fmaf(sy ? ny.w : ny.z, r.iy, r.cny);
+    const float f1y = __builtin_fmaf(sy ? ny.z : ny.w, r.iy, r.cfy);
+    const float n1z = __builtin_fmaf(sz ? nz.w : nz.z, r.iz, r.cnz);
+    const float f1z = __builtin_fmaf(sz ? nz.z : nz.w, r.iz, r.cfz);
     tn0 = fmaxf(fmaxf(n0x, n0y), fmaxf(n0z, tmin));
     const float tf0 = fminf(fminf(f0x, f0y), fminf(f0z, tbest)) * 1.00000048f;
     tn1 = fmaxf(fmaxf(n1x, n1y), fmaxf(n1z, tmin));
@@ -474,6 +492,9 @@ __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
                     r.ix = stage.at(8, slot);
                     r.iy = stage.at(9, slot);
                     r.iz = stage.at(10, slot);
+                    slab_offsets(r.ox, r.ix, r.cnx, r.cfx);
+                    slab_offsets(r.oy, r.iy, r.cny, r.cfy);
+                    slab_offsets(r.oz, r.iz, r.cnz, r.cfz);
                     local = __float_as_uint(stage.at(11, slot));
                     if (BATCHED) {
                         const BatchItemDev* it = L.items + __float_as_uint(stage.at(12, slot));
@@ -589,9 +610,9 @@ __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
                     const float nza[4] = {d[4].x, d[4].y, d[4].z, d[4].w}, fza[4] = {d[5].x, d[5].y, d[5].z, d[5].w};
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        const float tnx = (nxa[j] - r.ox) * r.ix, tfx = (fxa[j] - r.ox) * r.ix;
-                        const float tny = (nya[j] - r.oy) * r.iy, tfy = (fya[j] - r.oy) * r.iy;
-                        const float tnz = (nza[j] - r.oz) * r.iz, tfz = (fza[j] - r.oz) * r.iz;
+                        const float tnx = __builtin_fmaf(nxa[j], r.ix, r.cnx), tfx = __builtin_fmaf(fxa[j], r.ix, r.cfx);
+                        const float tny = __builtin_fmaf(nya[j], r.iy, r.cny), tfy = __builtin_fmaf(fya[j], r.iy, r.cfy);
+                        const float tnz = __builtin_fmaf(nza[j], r.iz, r.cnz), tfz = __builtin_fmaf(fza[j], r.iz, r.cfz);
                         const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
                         const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tbest)) * 1.00000048f;
                         k[j] = (tn <= tf && c[j] != kRefEmpty) ? tn : kInf;
