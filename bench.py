@@ -222,10 +222,9 @@ def main():
         tf = os.path.join(ROOT, "profiles", "bvh_trace_config3_traffic.json")
         if os.path.exists(tf):
             with open(tf) as f:
-                tj = json.load(f)
-            # The counters were read for one node width; another width's
-            # traffic is unmeasured.
-            if tj.get("bvh_width", 2) == args.width and args.mode == "batched":
+                tj = json.load(f).get("by_width", {}).get(str(args.width))
+            # Counters were read for the batched launch of each node width.
+            if tj and args.mode == "batched":
                 traffic = tj.get("hbm_bytes_per_launch")
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -380,7 +379,7 @@ def _time_gpu(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps, e0.elapsed_time(e1) / steps
 
 
-def _adjudicate(xyz, idx, rays, g, ref, thr, limit=600):
+def _adjudicate(xyz, idx, rays, g, ref, thr, limit=600, dump=""):
     """Closest-hit differences that are not ties, decided by brute force over
     every triangle (ENCORE's predicate) and FP64: counts of kd misses the GPU
     finds, reference flush phantoms (the reference's triangle is not crossed
@@ -407,6 +406,7 @@ def _adjudicate(xyz, idx, rays, g, ref, thr, limit=600):
 
     ref_off = (rp != 0xFFFFFFFF) & ~crossed(fr)
     gpu_ok = (gp == 0xFFFFFFFF) | crossed(fg)
+    unexplained = []
     for i in range(len(rays)):
         if gp[i] == bp[i] or (gp[i] != 0xFFFFFFFF and bp[i] != 0xFFFFFFFF and
                               abs(g[i, 0] - br[i, 0]) <= 1e-6 * abs(br[i, 0])):
@@ -417,11 +417,15 @@ def _adjudicate(xyz, idx, rays, g, ref, thr, limit=600):
             out["reference_flush_phantom_fp64_off_triangle"] += 1
         else:
             out["unexplained"] += 1
+            unexplained.append(i)
     out["adjudicated"] = int(len(rays))
+    if dump and unexplained:
+        u = np.array(unexplained)
+        np.savez(dump, rays=rays[u], gpu=g[u], kd=ref[u], brute=br[u])
     return out
 
 
-def _classify(g, ref, xyz=None, idx=None, rays=None, thr=1):
+def _classify(g, ref, xyz=None, idx=None, rays=None, thr=1, dump=""):
     import numpy as np
 
     gp = np.ascontiguousarray(g[:, 3]).view(np.uint32)
@@ -434,7 +438,7 @@ def _classify(g, ref, xyz=None, idx=None, rays=None, thr=1):
            "other": int(len(diff) - is_tie.sum())}
     if xyz is not None and out["other"]:
         o = diff[~is_tie]
-        out.update(_adjudicate(xyz, idx, rays[o], g[o], ref[o], thr))
+        out.update(_adjudicate(xyz, idx, rays[o], g[o], ref[o], thr, dump=dump))
     return out
 
 
@@ -507,8 +511,12 @@ def run_workload(args):
                         f, fp = oracle.fp64_closest(xyz, idx, samp[dif], nthreads=thr)
                         real = ((fp != 0xFFFFFFFF) & (f[:, 0] < samp[dif, 7])).astype(np.uint32)
                         par["gpu_matches_fp64"] = int((real == go[dif]).sum())
+                        if args.dump:
+                            np.savez("%s_%s_%s.npz" % (args.dump, accel, label), rays=samp[dif], gpu=go[dif], kd=ref[dif],
+                                     brute=bo, fp64=real, fp64_t=f[:, 0])
                 else:
-                    par = _classify(got[::stride], ref, xyz, idx, samp, thr) if accel == "bvh" else \
+                    par = _classify(got[::stride], ref, xyz, idx, samp, thr,
+                                    dump=args.dump and "%s_%s_%s.npz" % (args.dump, accel, label)) if accel == "bvh" else \
                         _classify(got[::stride], ref)
                 cnt = sc.counts(d, any_hit=any_hit) if accel == "bvh" else None
                 results["%s_%s" % (accel, label)] = {

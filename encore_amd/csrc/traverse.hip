@@ -137,12 +137,21 @@ struct RayState {
 // D = |p| 2^-22 (at least two ulps of P), cn = -p - D and cf = -p + D keep
 // fma(b, i, cn) at or below, and fma(b, i, cf) at or above, the exact
 // distance up to the final rounding of the fma (half an ulp of the result,
-// covered by the relative widening applied to the far distance).  For d = 0
-// (i infinite) the fma yields NaN or an infinity of the side that does not
-// cull, so such a slab never rejects a box.
-__device__ __forceinline__ void slab_offsets(float o, float i, float& cn, float& cf) {
+// covered by the relative widening applied to the far distance).
+//
+// A zero direction component (i infinite) would turn the slab into NaN and
+// stop it culling anything, so such a ray would visit every box crossing
+// its plane.  There i becomes +-2^80 (its sign kept) and D at least 2^40:
+// the slab then rejects exactly the boxes whose extent on that axis misses
+// o by more than 2^-40 and keeps (conservatively) those within 2^-40.
+__device__ __forceinline__ void slab_offsets(float o, float& i, float& cn, float& cf) {
+    float dmin = 0.0f;
+    if (__builtin_isinf(i)) {
+        i = __builtin_copysignf(0x1p80f, i);
+        dmin = 0x1p40f;
+    }
     const float p = o * i;
-    const float D = fabsf(p) * 2.38418579e-07f;
+    const float D = fmaxf(fabsf(p) * 2.38418579e-07f, dmin);
     cn = -p - D;
     cf = -p + D;
 }

@@ -95,7 +95,9 @@ def load():
     L.rq_intersect_closest.argtypes = [_vp, _vp, _u32, _vp, _vp]
     L.rq_occluded.argtypes = [_vp, _vp, _u32, _vp, _vp]
     L.rq_trace_counts.argtypes = [_vp, _vp, _u32, _u32, _vp, _vp]
-    L.rq_trace_profile.argtypes = [_vp, _vp, _u32, _u32, _vp, _vp]
+    if hasattr(L, "rq_trace_profile"):
+        L.rq_trace_profile.argtypes = [_vp, _vp, _u32, _u32, _vp, _vp]
+        L.rq_trace_profile.restype = ctypes.c_int
     L.rq_batch_create.argtypes = [_vp, _u32, ctypes.POINTER(_vp)]
     L.rq_batch_set.argtypes = [_vp, _vp, _u32]
     L.rq_batch_launch.argtypes = [_vp, _u32, _vp]
@@ -128,7 +130,7 @@ def load():
     L.rq_host_tree_destroy.restype = None
     for name in ("rq_device_open", "rq_scene_create", "rq_scene_stats", "rq_intersect_closest", "rq_occluded",
                  "rq_host_tree_build", "rq_host_tree_info", "rq_host_tree_copy", "rq_trace_counts",
-                 "rq_trace_profile", "rq_photons_create", "rq_photons_order", "rq_photons_balance_host", "rq_knn",
+                 "rq_photons_create", "rq_photons_order", "rq_photons_balance_host", "rq_knn",
                  "rq_radiance_estimate", "rq_batch_create", "rq_batch_set", "rq_batch_launch", "rq_batch_get_config",
                  "rq_batch_set_config", "rq_batch_autotune", "rq_server_create", "rq_server_submit", "rq_server_wait",
                  "rq_server_poll", "rq_server_cancel", "rq_server_stats"):

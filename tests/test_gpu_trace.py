@@ -192,3 +192,39 @@ def test_edge_cases(device, width):
     g2 = _trace(sc, short)
     assert np.all(split_hits(g2)[3] == oracle.NO_HIT)
     assert np.all(g2[:, 0] == np.float32(1e-3))
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_axis_parallel_rays_keep_culling(device, width):
+    """Rays with an exactly zero direction component (ENCORE's eye rays have
+    thousands: Normalize flushes small components) still cull on that axis:
+    they visit about as many nodes as the same rays tilted by 1e-6, and
+    return the oracle's hits."""
+    xyz, idx = scenes.displaced_sphere(level=5)
+    lo, hi = xyz.min(0), xyz.max(0)
+    rng = np.random.default_rng(11)
+    n = 3 * 2048
+    o = rng.uniform(lo - 0.2 * (hi - lo), hi + 0.2 * (hi - lo), (n, 3))
+    d = rng.normal(size=(n, 3))
+    for k in range(3):
+        d[k * 2048:(k + 1) * 2048, k] = 0.0
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 4:7] = d
+    rays[:, 7] = np.inf
+    assert (rays[:, 4:7] == 0).sum() == n
+    tilted = rays.copy()
+    tilted[:, 4:7][rays[:, 4:7] == 0] = 1e-6
+    log("case: axis-parallel rays bvh%d" % width)
+    sc = rq.Scene(device, xyz, idx, accel="bvh", width=width)
+    dr = torch.from_numpy(rays).cuda()
+    dt = torch.from_numpy(tilted).cuda()
+    _, nodes0, _ = sc.counts(dr)
+    _, nodes1, _ = sc.counts(dt)
+    log("nodes/ray: axis-parallel %.2f, tilted %.2f" % (nodes0 / n, nodes1 / n))
+    assert nodes0 <= 1.25 * nodes1 + 64
+    kd = oracle.KdTree(xyz, idx)
+    got = _trace(sc, rays)
+    c = _check_vs_oracle(xyz, idx, rays, got, kd.intersect(rays))
+    assert c["bad"] == 0, c["bad_idx"][:10]
