@@ -379,6 +379,36 @@ def _time_gpu(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps, e0.elapsed_time(e1) / steps
 
 
+def _outside_box(xyz, idx, rays, t, prim):
+    """Distance (world units) by which each hit point o + t d lies outside the
+    bounding box of triangle `prim`, less a rounding allowance of 1e-4 of the
+    point's magnitude; > 0 means a BVH, whose boxes enclose their triangles,
+    cannot reach that hit."""
+    import numpy as np
+
+    tri = xyz[idx[prim]].astype(np.float64)
+    P = rays[:, :3].astype(np.float64) + t.astype(np.float64)[:, None] * rays[:, 4:7].astype(np.float64)
+    out = (np.maximum(tri.min(1) - P, 0) + np.maximum(P - tri.max(1), 0)).max(1)
+    return out - 1e-4 * np.maximum(1.0, np.abs(P).max(1))
+
+
+def _predicate_hit(xyz, idx, rays, g):
+    """True where g's (t, prim) is exactly what ENCORE's predicate returns for
+    that ray and triangle."""
+    import numpy as np
+
+    import oracle
+
+    gp = np.ascontiguousarray(g[:, 3]).view(np.uint32)
+    ok = gp != 0xFFFFFFFF
+    res = np.zeros(len(g), bool)
+    if ok.any():
+        tri = np.ascontiguousarray(xyz[idx[gp[ok]]].reshape(-1, 9), np.float32)
+        h = oracle.tri_intersect_pairs(np.ascontiguousarray(rays[ok]), tri)
+        res[ok] = (h[:, 3] != 0) & (h[:, 0].view(np.uint32) == np.ascontiguousarray(g[ok, 0]).view(np.uint32))
+    return res
+
+
 def _adjudicate(xyz, idx, rays, g, ref, thr, limit=600, dump=""):
     """Closest-hit differences that are not ties, decided by brute force over
     every triangle (ENCORE's predicate) and FP64: counts of kd misses the GPU
@@ -388,8 +418,8 @@ def _adjudicate(xyz, idx, rays, g, ref, thr, limit=600, dump=""):
 
     import oracle
 
-    out = {"kd_miss_gpu_matches_brute": 0, "reference_flush_phantom_fp64_off_triangle": 0, "unexplained": 0,
-           "adjudicated": 0}
+    out = {"kd_miss_gpu_matches_brute": 0, "reference_flush_phantom_fp64_off_triangle": 0,
+           "reference_phantom_outside_its_triangle_box": 0, "unexplained": 0, "adjudicated": 0}
     if len(rays) == 0:
         return out
     sel = np.arange(min(len(rays), limit))
@@ -406,6 +436,11 @@ def _adjudicate(xyz, idx, rays, g, ref, thr, limit=600, dump=""):
 
     ref_off = (rp != 0xFFFFFFFF) & ~crossed(fr)
     gpu_ok = (gp == 0xFFFFFFFF) | crossed(fg)
+    # The reference's hit is a phantom lying outside its own triangle's box
+    # (so no BVH can reach it) and the GPU's answer is an exact predicate hit
+    # farther along, or no hit.
+    ref_box_out = (rp != 0xFFFFFFFF) & (_outside_box(xyz, idx, rays, ref[:, 0], np.where(rp == 0xFFFFFFFF, 0, rp)) > 0)
+    gpu_exact = (gp == 0xFFFFFFFF) | _predicate_hit(xyz, idx, rays, g)
     unexplained = []
     for i in range(len(rays)):
         if gp[i] == bp[i] or (gp[i] != 0xFFFFFFFF and bp[i] != 0xFFFFFFFF and
@@ -415,6 +450,8 @@ def _adjudicate(xyz, idx, rays, g, ref, thr, limit=600, dump=""):
             # the reference (and brute force, same predicate) report a hit on a
             # triangle the ray does not cross; the GPU's answer is genuine
             out["reference_flush_phantom_fp64_off_triangle"] += 1
+        elif ref_off[i] and ref_box_out[i] and gpu_exact[i]:
+            out["reference_phantom_outside_its_triangle_box"] += 1
         else:
             out["unexplained"] += 1
             unexplained.append(i)
@@ -511,6 +548,23 @@ def run_workload(args):
                         f, fp = oracle.fp64_closest(xyz, idx, samp[dif], nthreads=thr)
                         real = ((fp != 0xFFFFFFFF) & (f[:, 0] < samp[dif, 7])).astype(np.uint32)
                         par["gpu_matches_fp64"] = int((real == go[dif]).sum())
+                        # Rows neither explains: the GPU finds no hit while the
+                        # reference's predicate hits.  Enumerate every predicate
+                        # hit below tmax; if each lies outside its own triangle's
+                        # box (a flush phantom no BVH reaches), the GPU answer is
+                        # the predicate over the boxes the ray enters.
+                        rest = dif[(bo != go[dif]) & (real != go[dif])]
+                        par["reference_phantom_outside_its_triangle_box"] = 0
+                        T = len(idx)
+                        tri9 = np.ascontiguousarray(xyz[idx].reshape(T, 9), np.float32)
+                        for j in rest[:16]:
+                            if go[j] != 0:
+                                continue
+                            hh = oracle.tri_intersect_pairs(np.repeat(samp[j:j + 1], T, axis=0), tri9)
+                            hit = np.nonzero((hh[:, 3] != 0) & (hh[:, 0] > 0) & (hh[:, 0] < samp[j, 7]))[0]
+                            if len(hit) and np.all(_outside_box(xyz, idx, np.repeat(samp[j:j + 1], len(hit), axis=0),
+                                                                hh[hit, 0], hit) > 0):
+                                par["reference_phantom_outside_its_triangle_box"] += 1
                         if args.dump:
                             np.savez("%s_%s_%s.npz" % (args.dump, accel, label), rays=samp[dif], gpu=go[dif], kd=ref[dif],
                                      brute=bo, fp64=real, fp64_t=f[:, 0])

@@ -18,7 +18,7 @@ using encore::Point3;
 using encore::Vector3;
 
 HipAccel::HipAccel()
-    : m_Kind(RQ_ACCEL_BVH), m_Predicate(RQ_PRED_ENCORE), m_Ordinal(0), m_Device(0), m_Scene(0), m_dRays(0),
+    : m_Kind(RQ_ACCEL_BVH), m_Predicate(RQ_PRED_ENCORE), m_Width(4), m_Ordinal(0), m_Device(0), m_Scene(0), m_dRays(0),
       m_dOut(0), m_Capacity(0) {}
 
 HipAccel::~HipAccel() {
@@ -39,6 +39,7 @@ void HipAccel::configure(Options* o) {
     if (o->doesOptionExist("HipPredicate"))
         m_Predicate = (o->getOption("HipPredicate") == "Watertight") ? RQ_PRED_WATERTIGHT : RQ_PRED_ENCORE;
     if (o->doesOptionExist("HipDevice")) m_Ordinal = atoi(o->getOption("HipDevice").c_str());
+    if (o->doesOptionExist("HipBvhWidth")) m_Width = (atoi(o->getOption("HipBvhWidth").c_str()) == 2) ? 2u : 4u;
 }
 
 bool HipAccel::ok(rq_status s) {
@@ -97,6 +98,7 @@ void HipAccel::build(std::list<IModel*>& modelList) {
     memset(&d, 0, sizeof(d));
     d.accel = m_Kind;
     d.predicate = m_Predicate;
+    d.bvh_width = m_Width;
     d.has_bounds = 1;
     for (int k = 0; k < 3; ++k) {
         d.bounds_min[k] = bmin[k];

@@ -69,6 +69,7 @@ private:
 
     uint32_t m_Kind;
     uint32_t m_Predicate;
+    uint32_t m_Width;  // BVH children per node: 2 or 4
     int m_Ordinal;
     rq_device* m_Device;
     rq_scene* m_Scene;
