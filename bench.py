@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", choices=["plain", "batched"], default="batched")
     ap.add_argument("--width", type=int, choices=[2, 4], default=4, help="BVH children per node")
+    ap.add_argument("--leaf", type=int, default=0, help="BVH leaf size 1..16 (0 = the builder default, 4)")
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--rays", type=int, default=8388608)
     ap.add_argument("--batches", type=int, default=64)
@@ -47,7 +48,7 @@ def parse():
     ap.add_argument("--no-counts", action="store_true")
     ap.add_argument("--no-exact", action="store_true", help="skip the bit-exact kd-path measurement")
     ap.add_argument("--dump", default="", help="write the adjudicated parity differences to this .npz")
-    ap.add_argument("--workload", choices=["c3", "c1", "c2", "c4", "c5"], default="c3",
+    ap.add_argument("--workload", choices=["c3", "c1", "c2", "c4", "c5", "fg"], default="c3",
                     help="c3 (default): 8M incoherent rays at 1.31M triangles; c1: Cornell box eye rays; c2: "
                          "1080p eye + shadow rays at 1.31M triangles; c4: photon gather; c5: serving")
     ap.add_argument("--requests", type=int, default=160, help="c5: requests per arrival-rate point")
@@ -86,7 +87,7 @@ def main():
     xyz, idx = scenes.config2_scene(level=args.level)
     log("scene: %d triangles (%.1fs)" % (len(idx), time.time() - t))
     t = time.time()
-    scene = rq.Scene(dev, xyz, idx, accel="bvh", predicate="encore", width=args.width)
+    scene = rq.Scene(dev, xyz, idx, accel="bvh", predicate="encore", width=args.width, leaf_size=args.leaf)
     st = scene.stats()
     log("bvh%d: %d nodes, depth %d, %.1f MB, build %.2fs" % (args.width, st["nnodes"], st["max_depth"],
                                                             st["device_bytes"] / 1e6, st["build_seconds"]))
@@ -479,6 +480,78 @@ def _classify(g, ref, xyz=None, idx=None, rays=None, thr=1, dump=""):
     return out
 
 
+def _final_gather_workload(args, dev, thr, trace_fn):
+    """Final gather (PhotonMapperCPU.cpp:1110-1167) with ENCORE's shipped
+    settings scaled to the 1/20 Cornell box: 2M photons split into the direct
+    map (first hits) and the indirect map, 8x8 jittered hemisphere samples
+    per shading point, indirect estimate (100 photons, r = 3/20), direct
+    estimate (100, r = 2/20)."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from encore_amd import rq, scenes
+
+    xyz, idx = scenes.config4_scene()
+    bvh = rq.Scene(dev, xyz, idx, accel="bvh")
+    t = time.time()
+    P, N, W, B = scenes.trace_photons(xyz, idx, trace_fn(bvh), 2000000, batch=1 << 18, with_bounce=True)
+    ind, dr = B >= 1, B == 0
+    log("photons: %d traced in %.1fs (%d direct, %d indirect)" % (len(P), time.time() - t, dr.sum(), ind.sum()))
+    Q, QN = scenes.photon_queries(xyz, idx, trace_fn(bvh), res=256)
+    ns = 8
+    nper = ns * ns
+    dirs = scenes.hemisphere_dirs(QN, ns, scenes.Stream(scenes.SEED_BASE + 6))
+    diffuse = np.random.default_rng(6).uniform(0.2, 0.9, (len(idx), 3)).astype(np.float32)
+    k_ind, r_ind, k_dir, r_dir = 100, 0.15, 100, 0.10
+    log("shading points: %d x %d samples = %d rays" % (len(Q), nper, len(Q) * nper))
+    gi = rq.PhotonMap(dev, P[ind], N[ind], W[ind])
+    gd = rq.PhotonMap(dev, P[dr], N[dr], W[dr])
+    d_q, d_dirs = torch.from_numpy(Q).cuda(), torch.from_numpy(dirs).cuda()
+    d_dif = torch.from_numpy(diffuse).cuda()
+    rgb = torch.empty((len(Q), 3), device="cuda")
+    ri = torch.empty((len(Q),), device="cuda")
+    out = {}
+    kdsc = rq.Scene(dev, xyz, idx, accel="kd")
+    for name, sc in (("bvh", bvh), ("kd", kdsc)):
+        fn = lambda sc=sc: sc.final_gather(gi, k_ind, r_ind, gd, k_dir, r_dir, d_dif, d_q, d_dirs, nper, rgb, ri)
+        wall, ev = _time_gpu(fn, args.steps, args.warmup)
+        out[name] = {"Mrays_s": round(len(Q) * nper / (ev / 1000.0) / 1e6, 2), "ms": round(ev, 3),
+                     "Mpoints_s": round(len(Q) / (ev / 1000.0) / 1e6, 4)}
+        out[name + "_rgb"] = rgb.cpu().numpy()
+        out[name + "_ri"] = ri.cpu().numpy()
+        log("final gather %s: %s" % (name, out[name]))
+    # CPU baseline and parity on a strided sample of shading points.
+    stride = max(1, len(Q) // 2048)
+    sel = np.arange(0, len(Q), stride)
+    sdirs = dirs.reshape(len(Q), nper, 3)[sel].reshape(-1, 3)
+    kd = oracle.KdTree(xyz, idx)
+    oi = oracle.PhotonMap(P[ind], N[ind], W[ind])
+    od = oracle.PhotonMap(P[dr], N[dr], W[dr])
+    t = time.perf_counter()
+    o_rgb, o_ri = oracle.final_gather(xyz, idx, kd, oi, k_ind, r_ind, od, k_dir, r_dir, diffuse, Q[sel], sdirs, nper,
+                                      nthreads=thr)
+    cpu_s = time.perf_counter() - t
+    parity = {"sample_points": int(len(sel))}
+    for name in ("kd", "bvh"):
+        g, gr = out.pop(name + "_rgb")[sel], out.pop(name + "_ri")[sel]
+        parity[name] = {"bit_equal": int(np.all(g.view(np.uint32) == o_rgb.view(np.uint32), axis=1).sum()),
+                        "within_1e-3": int((np.all(np.isclose(g, o_rgb, rtol=1e-3, atol=1e-6), axis=1) &
+                                            np.isclose(gr, o_ri, rtol=1e-3)).sum())}
+    log("parity", parity)
+    _line("final-gather Mrays/s (8x8 samples, indirect + direct estimates)", out["bvh"]["Mrays_s"], "Mrays/s",
+          out["bvh"]["ms"],
+          {"workload": "final gather at %d shading points x %d hemisphere samples on the Cornell box at 1/20 "
+                       "scale tessellated to %d triangles; %d indirect + %d direct photons, k = 100, r = 0.15 / 0.10"
+                       % (len(Q), nper, len(idx), ind.sum(), dr.sum()), "accel": "bvh4"},
+          paths=out, parity=parity,
+          cpu_baseline={"value": round(len(sel) * nper / cpu_s / 1e6, 5), "unit": "Mrays/s", "cores": thr,
+                        "kind": "port", "sample": "%d shading points x %d samples" % (len(sel), nper)})
+    for o in (gi, gd, kdsc, bvh):
+        o.destroy()
+    return 0
+
+
 def run_workload(args):
     import numpy as np
     import torch
@@ -587,6 +660,8 @@ def run_workload(args):
               results=results, cpu_cores=thr)
         return 0
 
+    if args.workload == "fg":
+        return _final_gather_workload(args, dev, thr, trace_fn)
     if args.workload == "c4":
         xyz, idx = scenes.config4_scene()
         sc = rq.Scene(dev, xyz, idx, accel="bvh")

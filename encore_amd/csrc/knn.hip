@@ -166,8 +166,10 @@ __global__ __launch_bounds__(64) void knn_kernel(KnnLaunch L) {
     for (uint32_t end = cnt; end > 0; --end) {
         const float topd = hd[0];
         const uint32_t tops = hs[0];
-        L.out_idx[base + end - 1] = L.perm[tops];
-        L.out_d2[base + end - 1] = topd;
+        if (L.out_idx) {
+            L.out_idx[base + end - 1] = L.perm[tops];
+            L.out_d2[base + end - 1] = topd;
+        }
         if (L.out_rgb) {
             const float4 pw = *reinterpret_cast<const float4*>(L.power + tops);
             const double w = L.caustic ? (rmax - (double)topd) / (double)topd : 1.0;
@@ -200,9 +202,11 @@ __global__ __launch_bounds__(64) void knn_kernel(KnnLaunch L) {
         hd[i * kKnnWave] = vd;
         hs[i * kKnnWave] = vs;
     }
-    for (uint32_t j = cnt; j < k; ++j) {
-        L.out_idx[base + j] = 0xFFFFFFFFu;
-        L.out_d2[base + j] = __int_as_float(0x7f800000);
+    if (L.out_idx) {
+        for (uint32_t j = cnt; j < k; ++j) {
+            L.out_idx[base + j] = 0xFFFFFFFFu;
+            L.out_d2[base + j] = __int_as_float(0x7f800000);
+        }
     }
     if (L.out_cnt) L.out_cnt[q] = cnt;
     if (L.out_rgb) {

@@ -214,11 +214,14 @@ rq_status rq_scene_create(rq_device* dev, const float* xyz, uint32_t nverts, con
         sc->predicate = dd.predicate;
         sc->ntris = ntris;
         std::vector<TriRec> tris;
+        std::vector<uint32_t> slot;
         if (dd.accel == RQ_ACCEL_BVH) {
             require(dd.bvh_width == 0 || dd.bvh_width == 2 || dd.bvh_width == 4, "bvh_width must be 0, 2 or 4");
             BvhHost h;
             build_bvh(xyz, nverts, tri_idx, ntris, dd.leaf_size, dd.bvh_width ? dd.bvh_width : 4, h);
             pack_triangles(xyz, tri_idx, h.tri_order.data(), ntris, dd.predicate, tris);
+            slot.assign(ntris, 0u);
+            for (uint32_t i = 0; i < ntris; ++i) slot[h.tri_order[i]] = i;
             sc->stats = h.stats;
             sc->width = h.width;
             if (h.width == 4) {
@@ -235,6 +238,12 @@ rq_status rq_scene_create(rq_device* dev, const float* xyz, uint32_t nverts, con
             build_kd(xyz, nverts, tri_idx, ntris, bmin, bmax, h);
             // One record per leaf reference, in leaf order.
             pack_triangles(xyz, tri_idx, h.leaf_prims.data(), (uint32_t)h.leaf_prims.size(), dd.predicate, tris);
+            // A primitive can sit in several leaves; any of its records will do.
+            slot.assign(ntris, 0xFFFFFFFFu);
+            for (uint32_t j = 0; j < (uint32_t)h.leaf_prims.size(); ++j)
+                if (slot[h.leaf_prims[j]] == 0xFFFFFFFFu) slot[h.leaf_prims[j]] = j;
+            for (uint32_t& v : slot)
+                if (v == 0xFFFFFFFFu) v = 0;  // in no leaf: never hit
             sc->stats = h.stats;
             for (int k = 0; k < 3; ++k) {
                 sc->root_pos[k] = h.root_pos[k];
@@ -246,7 +255,9 @@ rq_status rq_scene_create(rq_device* dev, const float* xyz, uint32_t nverts, con
         }
         sc->tris = DevBuf(tris.size() * sizeof(TriRec));
         sc->tris.upload(tris.data(), sc->tris.bytes);
-        sc->device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->leaf_prims.bytes;
+        sc->prim_slot = DevBuf(slot.size() * sizeof(uint32_t));
+        sc->prim_slot.upload(slot.data(), sc->prim_slot.bytes);
+        sc->device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->leaf_prims.bytes + sc->prim_slot.bytes;
         *out = sc.release();
     });
 }
@@ -436,9 +447,68 @@ rq_status rq_radiance_estimate(rq_photons* p, const float* d_qpos, const float* 
                                float max_dist, uint32_t caustic, float* d_rgb, uint32_t* d_idx, float* d_d2,
                                uint32_t* d_count, void* stream) {
     return guarded([&] {
-        require(nq == 0 || (d_rgb && d_idx && d_d2), "null output arrays");
+        require(nq == 0 || d_rgb, "null radiance output");
+        require((d_idx == nullptr) == (d_d2 == nullptr), "d_idx and d_d2 go together");
         knn_common(p, d_qpos, d_qnormal, nq, k, max_dist, d_idx, d_d2, d_count, d_rgb, caustic ? 1u : 0u,
                    (hipStream_t)stream);
+    });
+}
+
+rq_status rq_final_gather(rq_scene* scene, rq_photons* indirect, uint32_t k_ind, float r_ind, rq_photons* direct,
+                          uint32_t k_dir, float r_dir, const float* d_diffuse, const float* d_origin,
+                          uint32_t npoints, const float* d_dirs, uint32_t nper, float* d_rgb, float* d_ri,
+                          void* stream) {
+    return guarded([&] {
+        require(scene && indirect, "null scene or photon map");
+        require(nper >= 1, "at least one sample per point");
+        require(npoints == 0 || (d_diffuse && d_origin && d_dirs && d_rgb), "null argument");
+        require(indirect->dev == scene->dev && (!direct || direct->dev == scene->dev), "objects on different devices");
+        if (npoints == 0) return;
+        hipStream_t s = (hipStream_t)stream;
+        RQ_HIP(hipSetDevice(scene->dev->ordinal));
+        // Chunks of whole shading points, at most 4M sample rays each.
+        const uint64_t max_rays = std::max<uint64_t>(nper, 1u << 22);
+        const uint32_t chunk_pts = (uint32_t)std::max<uint64_t>(1, max_rays / nper);
+        const uint64_t cap = (uint64_t)std::min<uint64_t>(chunk_pts, npoints) * nper;
+        require(cap <= 0xFFFFFFF0ull, "too many samples per point");
+        const size_t per_ray = sizeof(rq_ray) + sizeof(rq_hit) + 4 * 3 * sizeof(float);
+        DevBuf work(cap * per_ray + 1024);
+        char* w = static_cast<char*>(work.ptr);
+        rq_ray* rays = reinterpret_cast<rq_ray*>(w);
+        rq_hit* hits = reinterpret_cast<rq_hit*>(w + cap * sizeof(rq_ray));
+        float* qpos = reinterpret_cast<float*>(w + cap * (sizeof(rq_ray) + sizeof(rq_hit)));
+        float* qnrm = qpos + 3 * cap;
+        float* rgb_ind = qnrm + 3 * cap;
+        float* rgb_dir = rgb_ind + 3 * cap;
+        for (uint32_t p0 = 0; p0 < npoints; p0 += chunk_pts) {
+            const uint32_t np = std::min(chunk_pts, npoints - p0);
+            FgLaunch L{};
+            L.origin = d_origin + 3 * (size_t)p0;
+            L.dirs = d_dirs + 3 * (size_t)p0 * nper;
+            L.npoints = np;
+            L.nper = nper;
+            L.nrays = np * nper;
+            L.rays = rays;
+            L.hits = hits;
+            L.tris = scene->tris.as<TriRec>();
+            L.prim_slot = scene->prim_slot.as<uint32_t>();
+            L.predicate = scene->predicate;
+            L.qpos = qpos;
+            L.qnrm = qnrm;
+            L.rgb_ind = rgb_ind;
+            L.rgb_dir = direct ? rgb_dir : nullptr;
+            L.diffuse = d_diffuse;
+            L.out_rgb = d_rgb + 3 * (size_t)p0;
+            L.out_ri = d_ri ? d_ri + p0 : nullptr;
+            launch_fg_rays(L, s);
+            trace(scene, rays, L.nrays, hits, nullptr, TraceKind::Closest, s);
+            launch_fg_attr(L, s);
+            knn_common(indirect, qpos, qnrm, L.nrays, k_ind, r_ind, nullptr, nullptr, nullptr, rgb_ind, 0, s);
+            if (direct) knn_common(direct, qpos, qnrm, L.nrays, k_dir, r_dir, nullptr, nullptr, nullptr, rgb_dir, 0, s);
+            launch_fg_reduce(L, s);
+        }
+        // The workspace is released when this call returns.
+        RQ_HIP(hipStreamSynchronize(s));
     });
 }
 

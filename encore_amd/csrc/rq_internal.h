@@ -199,8 +199,8 @@ struct KnnLaunch {
     uint32_t nq;
     uint32_t k;
     float r2;
-    uint32_t* out_idx;   // nq*k insertion indices (ascending distance), ~0 padding
-    float* out_d2;       // nq*k squared distances, +inf padding
+    uint32_t* out_idx;   // nq*k insertion indices (ascending distance), ~0 padding; null = radiance only
+    float* out_d2;       // nq*k squared distances, +inf padding (with out_idx)
     uint32_t* out_cnt;   // nq
     float* out_rgb;      // radiance estimate (nq*3) or null
     uint32_t caustic;
@@ -261,6 +261,28 @@ struct KernelConfig {
     int kd_fetch_threshold = 16;  // kd path (its refill claims rays with an atomic per round)
 };
 
+// Final gather (gather.hip).
+struct FgLaunch {
+    const float* origin;      // shading points, 3 floats each
+    const float* dirs;        // nper directions per point, 3 floats each
+    uint32_t npoints, nper, nrays;
+    rq_ray* rays;             // nrays
+    const rq_hit* hits;       // nrays
+    const TriRec* tris;
+    const uint32_t* prim_slot;
+    uint32_t predicate;
+    float* qpos;              // nrays x 3: hit point (1e30 for a miss)
+    float* qnrm;              // nrays x 3: hit normal
+    const float* rgb_ind;     // nrays x 3
+    const float* rgb_dir;     // nrays x 3 or null
+    const float* diffuse;     // 3 per prim
+    float* out_rgb;           // npoints x 3
+    float* out_ri;            // npoints or null
+};
+void launch_fg_rays(const FgLaunch& L, hipStream_t s);
+void launch_fg_attr(const FgLaunch& L, hipStream_t s);
+void launch_fg_reduce(const FgLaunch& L, hipStream_t s);
+
 // Threads per block: the BVH kernels run one wave per block (their LDS use
 // per wave is large, so single-wave blocks pack a CU best); the kd kernel
 // four.
@@ -314,6 +336,7 @@ struct rq_scene {
     rq_device* dev = nullptr;
     uint32_t accel = 0, predicate = 0, ntris = 0, width = 2;
     rq::DevBuf nodes, tris, leaf_prims;
+    rq::DevBuf prim_slot;  // prim id -> index of one of its triangle records
     float root_pos[3] = {0, 0, 0}, root_size[3] = {0, 0, 0};
     rq::BuildStats stats;
     uint64_t device_bytes = 0;

@@ -121,6 +121,10 @@ def load():
     L.rq_photons_balance_host.argtypes = [_vp, _u32, _u32, _vp, _vp]
     L.rq_knn.argtypes = [_vp, _vp, _vp, _u32, _u32, ctypes.c_float, _vp, _vp, _vp, _vp]
     L.rq_radiance_estimate.argtypes = [_vp, _vp, _vp, _u32, _u32, ctypes.c_float, _u32, _vp, _vp, _vp, _vp, _vp]
+    if hasattr(L, "rq_final_gather"):
+        L.rq_final_gather.argtypes = [_vp, _vp, _u32, ctypes.c_float, _vp, _u32, ctypes.c_float, _vp, _vp, _u32, _vp,
+                                      _u32, _vp, _vp, _vp]
+        L.rq_final_gather.restype = ctypes.c_int
     L.rq_debug_fail_alloc_after.argtypes = [ctypes.c_int64]
     L.rq_debug_fail_alloc_after.restype = None
     L.rq_host_tree_build.argtypes = [_vp, _u32, _vp, _u32, ctypes.POINTER(SceneDesc), ctypes.POINTER(_vp)]
@@ -224,6 +228,14 @@ class Scene:
         d["node_util"] = d["nodes"] / max(1, 64 * d["node_iters"])
         d["tri_util"] = d["tris"] / max(1, 64 * d["tri_iters"])
         return d
+
+    def final_gather(self, indirect, k_ind, r_ind, direct, k_dir, r_dir, diffuse, origin, dirs, nper, rgb, ri=None,
+                     stream=None):
+        """Final gather (synchronous): diffuse (ntris,3), origin (n,3),
+        dirs (n*nper,3) device tensors; rgb (n,3) and ri (n,) outputs."""
+        check(load().rq_final_gather(self.h, indirect.h, int(k_ind), float(r_ind), direct.h if direct else None,
+                                     int(k_dir), float(r_dir), _ptr(diffuse), _ptr(origin), origin.shape[0],
+                                     _ptr(dirs), int(nper), _ptr(rgb), _ptr(ri), _stream(stream)))
 
     def counts(self, rays, any_hit=False, n=None, stream=None):
         """(rays, nodes visited, triangles tested) of one query (synchronous)."""

@@ -308,6 +308,79 @@ def hit_points(xyz, idx, hits):
     return ok, p, nrm
 
 
+PI32 = np.float32(3.14159)  # ENCORE's PI (Common.h:36)
+
+
+def hemisphere_dirs(normals, ns, stream, jitter=0.5):
+    """Final-gather sample directions: Sampler(ns, ns, 1) with
+    SetJitterPercent(jitter) and GetSampleRaysForHemisphere (Sampler.cpp:23-47,
+    101-186, 217-231) for every normal, ns*ns unit vectors per normal in the
+    reference's order (phi division outer, theta division inner).  theta and
+    phi are cell centres of a (pi/2) x (2 pi) grid plus a jitter of up to
+    `jitter` of half a cell (two draws per sample, theta first); the point is
+    converted to Cartesian around +z, rotated onto the normal by
+    R^T Rx(acos n_z) R (TAffineMatrix::GetRotationMatrix) with R's rows the
+    rotation axis Cross(+z, n) and two vectors orthogonal to it, and
+    normalised.  The reference draws the jitter with rand(); here it comes
+    from `stream`.  Float32 throughout."""
+    f = np.float32
+    nrm = normalize_flush(np.asarray(normals, np.float32).reshape(-1, 3))
+    n = len(nrm)
+    m = ns * ns
+    theta_div = f(f(1) / f(ns)) * f(PI32 / f(2))
+    phi_div = f(f(1) / f(ns)) * f(f(2) * PI32)
+    ht, hp = f(theta_div / f(2)), f(phi_div / f(2))
+    row, col = np.meshgrid(np.arange(ns), np.arange(ns), indexing="ij")
+    theta = (theta_div * col.reshape(-1).astype(np.float32) + ht).astype(np.float32)
+    phi = (phi_div * row.reshape(-1).astype(np.float32) + hp).astype(np.float32)
+    theta = np.broadcast_to(theta, (n, m)).copy()
+    phi = np.broadcast_to(phi, (n, m)).copy()
+    if jitter > 0:
+        u = stream.draw(2 * n * m).reshape(n, m, 2).astype(np.float32)
+        theta = (theta + ((u[..., 0] * f(2) - f(1)) * ht) * f(jitter)).astype(np.float32)
+        phi = (phi + ((u[..., 1] * f(2) - f(1)) * hp) * f(jitter)).astype(np.float32)
+    st = np.sin(theta)
+    p = np.stack([st * np.cos(phi), st * np.sin(phi), np.cos(theta)], axis=-1).astype(np.float32)
+    p = normalize_flush(p)
+    out = p.copy()
+    z = np.array([0, 0, 1], np.float32)
+    same = np.all(nrm == z, axis=1)
+    flip = np.all(nrm == -z, axis=1)
+    out[flip] = (p[flip] + p[flip] * f(-2)).astype(np.float32)
+    rot = ~(same | flip)
+    ang = np.arccos(nrm[:, 2]).astype(np.float32)  # Dot(+z, n) = n_z
+    rot &= ang != 0
+    if rot.any():
+        r = normalize_flush(cross_flush(np.broadcast_to(z, (int(rot.sum()), 3)), nrm[rot]))
+        a = np.abs(r)
+        s_ = np.zeros_like(r)
+        xs = (a[:, 0] <= a[:, 1]) & (a[:, 0] <= a[:, 2])
+        ys = ~xs & (a[:, 1] <= a[:, 0]) & (a[:, 1] <= a[:, 2])
+        zs = ~xs & ~ys
+        s_[xs] = np.stack([np.zeros(xs.sum(), np.float32), -r[xs, 2], r[xs, 1]], axis=1)
+        s_[ys] = np.stack([-r[ys, 2], np.zeros(ys.sum(), np.float32), r[ys, 0]], axis=1)
+        s_[zs] = np.stack([-r[zs, 1], r[zs, 0], np.zeros(zs.sum(), np.float32)], axis=1)
+        s_ = normalize_flush(s_)
+        t_ = normalize_flush(cross_flush(r, s_))
+        R = np.stack([r, s_, t_], axis=1)  # rows r, s, t
+        c, sn = np.cos(ang[rot]).astype(np.float32), np.sin(ang[rot]).astype(np.float32)
+        X = np.zeros((len(c), 3, 3), np.float32)
+        X[:, 0, 0] = 1
+        X[:, 1, 1], X[:, 1, 2], X[:, 2, 1], X[:, 2, 2] = c, -sn, sn, c
+
+        def mul(A, B):  # float32, sums in k order
+            C = np.zeros_like(A)
+            for k in range(3):
+                C = (C + A[:, :, k:k + 1] * B[:, k:k + 1, :]).astype(np.float32)
+            return C
+
+        M = mul(mul(np.transpose(R, (0, 2, 1)), X), R)
+        q = p[rot]
+        out[rot] = (((M[:, None, :, 0] * q[..., 0:1] + M[:, None, :, 1] * q[..., 1:2]).astype(np.float32)
+                     + M[:, None, :, 2] * q[..., 2:3])).astype(np.float32)
+    return normalize_flush(out).reshape(n * m, 3)
+
+
 def shadow_rays(points, light_center, light_size, stream):
     """One ray per point toward a uniform point on a square light facing -y;
     origin pushed by 0.01 along the ray; tmax = |L - p| measured from the
@@ -472,16 +545,17 @@ def normals_of(xyz, idx, prims):
 
 
 def trace_photons(xyz, idx, trace, count, seed=SEED_BASE + 4, scale=1.0 / 20.0, light_frac=0.3, bounces=3,
-                  albedo=0.7, batch=262144):
+                  albedo=0.7, batch=262144, with_bounce=False):
     """Photons from a square light centred under the ceiling (side light_frac of
     the box, emitting downward with cosine weighting), stored at every
     diffuse hit; a photon survives each hit with probability `albedo` and
     bounces at most `bounces` times.  `trace(rays) -> hits (n,4)` is the
-    closest-hit query.  Returns positions, normals, powers (count x 3)."""
+    closest-hit query.  Returns positions, normals, powers (count x 3), and
+    with `with_bounce` the bounce index of each photon (0 = first hit)."""
     st = Stream(seed)
     half = 50.0 * scale
     side = 2 * half * light_frac
-    pos_out, nrm_out, pow_out = [], [], []
+    pos_out, nrm_out, pow_out, bnc_out = [], [], [], []
     stored = 0
     emitted = 0
     while stored < count:
@@ -498,7 +572,7 @@ def trace_photons(xyz, idx, trace, count, seed=SEED_BASE + 4, scale=1.0 / 20.0, 
         rays[:, 7] = np.inf
         power = np.full((n, 3), 1.0, np.float32)
         emitted += n
-        for _ in range(bounces + 1):
+        for bounce in range(bounces + 1):
             hits = trace(rays)
             ok, p, nrm = hit_points(xyz, idx, hits)
             prims = np.ascontiguousarray(hits[:, 3]).view(np.uint32)[ok]
@@ -506,6 +580,7 @@ def trace_photons(xyz, idx, trace, count, seed=SEED_BASE + 4, scale=1.0 / 20.0, 
             pos_out.append(p[good])
             nrm_out.append(nrm[good])
             pow_out.append(power[ok][good])
+            bnc_out.append(np.full(int(good.sum()), bounce, np.uint8))
             stored += int(good.sum())
             surv = st.draw(len(p)) < albedo
             keep = good & surv
@@ -524,6 +599,10 @@ def trace_photons(xyz, idx, trace, count, seed=SEED_BASE + 4, scale=1.0 / 20.0, 
     P = np.concatenate(pos_out)[:count]
     Nn = np.concatenate(nrm_out)[:count]
     W = np.concatenate(pow_out)[:count] / np.float32(emitted / 1e5)
+    if with_bounce:
+        # bounce 0 = first hit from the light (ENCORE's direct map), >= 1 the
+        # indirect map
+        return P.astype(np.float32), Nn.astype(np.float32), W.astype(np.float32), np.concatenate(bnc_out)[:count]
     return P.astype(np.float32), Nn.astype(np.float32), W.astype(np.float32)
 
 

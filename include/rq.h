@@ -281,10 +281,34 @@ rq_status rq_knn(rq_photons* photons, const float* d_qpos, const float* d_qnorma
  * (caustic = 0), or the cone-weighted sum (r - d)/d times k / (PI r^2) with r
  * and d the squared distances of the farthest and of each photon and
  * PI = 3.14159f (caustic = 1).  The gathered set is also written to d_idx,
- * d_d2, d_count as in rq_knn. */
+ * d_d2, d_count as in rq_knn; d_idx and d_d2 may both be NULL when only the
+ * radiance (and count) is wanted. */
 rq_status rq_radiance_estimate(rq_photons* photons, const float* d_qpos, const float* d_qnormal, uint32_t nq,
                                uint32_t k, float max_dist, uint32_t caustic, float* d_rgb, uint32_t* d_idx,
                                float* d_d2, uint32_t* d_count, void* stream);
+
+/* Final gather: PhotonMapperCPU::CalculateIndirectIllumination's
+ * final-gather branch (PhotonMapperCPU.cpp:1110-1167) for npoints shading
+ * points at once.  For point i (origin d_origin[3i..3i+2]) and each of its
+ * nper sample directions d_dirs[3(i nper + j)..] (unit vectors, as
+ * Sampler::GetSampleRaysForHemisphere returns them, Sampler.cpp:23-47, with
+ * the caller's jitter), a ray from the origin with no offset is traced to its
+ * closest hit; at each hit, with ENCORE's hit point and hit normal
+ * (TrianglePrim.cpp:236-239), the radiance estimate of `indirect` (k_ind,
+ * r_ind) and, if `direct` is not NULL, of `direct` (k_dir, r_dir) are each
+ * multiplied by the hit triangle's diffuse colour d_diffuse[3 prim..] and
+ * summed in sample order.  d_rgb[3i..] = that sum times 3.14159f / nper;
+ * d_ri[i] (may be NULL) = hits / sum(1 / hit distance), or 0 without hits.
+ * The caller applies the shading point's own diffuse colour and the 0.35
+ * scale (PhotonMapperCPU.cpp:1178).  With the kd scene the result is the
+ * reference's bit for bit (up to k-th-distance photon ties); with a BVH scene
+ * up to closest-hit ties and flush phantoms.  Synchronous; uses a temporary
+ * device workspace of up to 4M sample rays x 96 bytes.  No counterpart as a
+ * single call in ENCORE (it loops one shading point at a time). */
+rq_status rq_final_gather(rq_scene* scene, rq_photons* indirect, uint32_t k_ind, float r_ind, rq_photons* direct,
+                          uint32_t k_dir, float r_dir, const float* d_diffuse, const float* d_origin,
+                          uint32_t npoints, const float* d_dirs, uint32_t nper, float* d_rgb, float* d_ri,
+                          void* stream);
 
 /* Traversal work of one query over n rays, counted by an instrumented copy
  * of the same kernel (synchronous): counts[0] = rays, counts[1] = nodes

@@ -246,3 +246,85 @@ class PhotonMap:
         lib().orc_pm_radiance(self.h, qpos, qnrm, qpos.shape[0], k, float(max_dist), int(caustic), out,
                               nthreads or threads())
         return out
+
+
+def hit_attributes(xyz, idx, hits):
+    """ENCORE's hit point v0 + (v1-v0)*alpha + (v2-v0)*beta and hit normal
+    Normalize(Cross(e1, e2)) (TrianglePrim.cpp:236-239), float32 in the
+    reference's order, for hit rows (prim in column 3 as uint32 bits).
+    Returns (hit mask, points (n,3), normals (n,3)); miss rows are zero."""
+    prim = np.ascontiguousarray(hits[:, 3]).view(np.uint32)
+    ok = prim != NO_HIT
+    n = len(hits)
+    P = np.zeros((n, 3), np.float32)
+    N = np.zeros((n, 3), np.float32)
+    tri = np.asarray(idx)[prim[ok]]
+    v0 = xyz[tri[:, 0]].astype(np.float32)
+    e1 = (xyz[tri[:, 1]] - v0).astype(np.float32)
+    e2 = (xyz[tri[:, 2]] - v0).astype(np.float32)
+    a = hits[ok, 1:2].astype(np.float32)
+    b = hits[ok, 2:3].astype(np.float32)
+    P[ok] = ((v0 + e1 * a).astype(np.float32) + e2 * b).astype(np.float32)
+
+    def flush(x):
+        return np.where((x > -0.001) & (x < 0.001), np.float32(0), x).astype(np.float32)
+
+    cx = flush(e1[:, 1] * e2[:, 2] - e1[:, 2] * e2[:, 1])
+    cy = flush(np.float32(-1) * (e1[:, 0] * e2[:, 2] - e1[:, 2] * e2[:, 0]))
+    cz = flush(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0])
+    m = np.sqrt((cx * cx + cy * cy) + cz * cz).astype(np.float32)
+    with np.errstate(divide="ignore", invalid="ignore"):  # degenerate triangles give NaN, as in the reference
+        N[ok] = np.stack([flush(cx / m), flush(cy / m), flush(cz / m)], axis=1)
+    return ok, P, N
+
+
+def final_gather(xyz, idx, kd, indirect, k_ind, r_ind, direct, k_dir, r_dir, diffuse, origin, dirs, nper,
+                 nthreads=None):
+    """PhotonMapperCPU::CalculateIndirectIllumination's final-gather branch
+    (PhotonMapperCPU.cpp:1110-1167) for every shading point, composed from
+    the restated kd-tree and photon maps: nper rays per point from its origin
+    (no offset) along `dirs`; at each closest hit the radiance estimates of
+    `indirect` (k_ind, r_ind) and `direct` (k_dir, r_dir; may be None), each
+    times the hit triangle's diffuse colour, summed in sample order; times
+    3.14159f / nper.  Returns (rgb (n,3), Ri (n,)): Ri = hits / sum(1/distance)."""
+    origin = _c32(origin).reshape(-1, 3)
+    dirs = _c32(dirs).reshape(-1, 3)
+    npts = origin.shape[0]
+    rays = np.zeros((npts * nper, 8), np.float32)
+    rays[:, 0:3] = np.repeat(origin, nper, axis=0)
+    rays[:, 4:7] = dirs
+    rays[:, 7] = np.inf
+    hits = kd.intersect(rays, nthreads=nthreads)
+    ok, P, N = hit_attributes(xyz, idx, hits)
+    Q = P.copy()
+    Q[~ok] = np.float32(1e30)
+    ind = indirect.radiance(Q, N, k_ind, r_ind, nthreads=nthreads)
+    dr = direct.radiance(Q, N, k_dir, r_dir, nthreads=nthreads) if direct is not None else None
+    prim = np.ascontiguousarray(hits[:, 3]).view(np.uint32)
+    dif = np.asarray(diffuse, np.float32)[np.where(ok, prim, 0)]
+    okm = ok.reshape(npts, nper)
+    acc = np.zeros((npts, 3), np.float32)
+    ri = np.zeros(npts, np.float32)
+    nh = np.zeros(npts, np.float32)
+    ind = ind.reshape(npts, nper, 3)
+    dif = dif.reshape(npts, nper, 3)
+    P = P.reshape(npts, nper, 3)
+    if dr is not None:
+        dr = dr.reshape(npts, nper, 3)
+    for j in range(nper):
+        h = okm[:, j]
+        irr = (ind[:, j] * dif[:, j]).astype(np.float32)
+        if dr is not None:
+            irr = (irr + (dr[:, j] * dif[:, j]).astype(np.float32)).astype(np.float32)
+        acc[h] = (acc[h] + irr[h]).astype(np.float32)
+        v = (origin - P[:, j]).astype(np.float32)
+        mag = np.sqrt((v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2]).astype(np.float32)
+        with np.errstate(divide="ignore"):
+            inv = (np.float32(1) / mag).astype(np.float32)
+        ri[h] = (ri[h] + inv[h]).astype(np.float32)
+        nh[h] += 1
+    scale = np.float32(np.float32(3.14159) / np.float32(nper))
+    rgb = (acc * scale).astype(np.float32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        R = np.where(ri != 0, (nh / ri).astype(np.float32), np.float32(0)).astype(np.float32)
+    return rgb, R

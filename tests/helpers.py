@@ -69,7 +69,8 @@ def classify_closest(xyz, idx, rays, got, ref, brute=None, bit_exact=True):
         with np.errstate(invalid="ignore"):
             exact = miss_both | (same & close_rel(gt, rt) & close_rel(gu, ru, abs_tol=1e-3) &
                                  close_rel(gv, rv, abs_tol=1e-3))
-    out = dict(n=len(rays), exact=int(exact.sum()), tie=0, ref_defect=0, edge_band=0, bad=0, bad_idx=[])
+    out = dict(n=len(rays), exact=int(exact.sum()), tie=0, ref_defect=0, phantom_outside_box=0, edge_band=0, bad=0,
+               bad_idx=[])
     for i in np.nonzero(~exact)[0]:
         if not bit_exact and (_near_edge(gu[i], gv[i], gp[i]) or _near_edge(ru[i], rv[i], rp[i])):
             # A ray through a shared edge or vertex: ENCORE's predicate is not
@@ -86,6 +87,14 @@ def classify_closest(xyz, idx, rays, got, ref, brute=None, bit_exact=True):
             if genuine and abs(gt[i] - rt[i]) <= TIE_REL * abs(rt[i]):
                 out["tie"] += 1
                 continue
+        if rp[i] != oracle.NO_HIT and (gp[i] == oracle.NO_HIT or gt[i] > rt[i]):
+            # The reference's (closer) hit lies outside its own triangle's
+            # bounding box: a flush phantom of ENCORE's predicate that no BVH
+            # reaches, while `got` is an exact predicate hit (or none).
+            if outside_box(xyz, idx, rays[i], rt[i], rp[i]) > 0 and (
+                    gp[i] == oracle.NO_HIT or _exact_hit(xyz, idx, rays[i], gt[i], gp[i])):
+                out["phantom_outside_box"] += 1
+                continue
         if brute is not None:
             bt, _, _, bp = split_hits(brute[i:i + 1])
             g_is_brute = (gp[i] == bp[0]) or (gp[i] != oracle.NO_HIT and bp[0] != oracle.NO_HIT and
@@ -96,6 +105,20 @@ def classify_closest(xyz, idx, rays, got, ref, brute=None, bit_exact=True):
         out["bad"] += 1
         out["bad_idx"].append(int(i))
     return out
+
+
+def outside_box(xyz, idx, ray, t, prim):
+    """How far (world units) the point o + t d lies outside triangle prim's
+    bounding box, less a rounding allowance of 1e-4 of its magnitude."""
+    tri = xyz[idx[prim]].astype(np.float64)
+    P = ray[0:3].astype(np.float64) + float(t) * ray[4:7].astype(np.float64)
+    out = (np.maximum(tri.min(0) - P, 0) + np.maximum(P - tri.max(0), 0)).max()
+    return out - 1e-4 * max(1.0, np.abs(P).max())
+
+
+def _exact_hit(xyz, idx, ray, t, prim):
+    chk = prim_t(xyz, idx, ray[None, :], np.array([prim], np.uint32))[0]
+    return chk[3] > 0 and np.float32(chk[0]).view(np.uint32) == np.float32(t).view(np.uint32)
 
 
 def fp64_adjudicate(xyz, idx, rays, got, bad_idx):
