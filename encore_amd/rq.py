@@ -121,6 +121,11 @@ def load():
     L.rq_photons_balance_host.argtypes = [_vp, _u32, _u32, _vp, _vp]
     L.rq_knn.argtypes = [_vp, _vp, _vp, _u32, _u32, ctypes.c_float, _vp, _vp, _vp, _vp]
     L.rq_radiance_estimate.argtypes = [_vp, _vp, _vp, _u32, _u32, ctypes.c_float, _u32, _vp, _vp, _vp, _vp, _vp]
+    if hasattr(L, "rq_scene_save"):
+        L.rq_scene_save.argtypes = [_vp, ctypes.c_char_p]
+        L.rq_scene_save.restype = ctypes.c_int
+        L.rq_scene_load.argtypes = [_vp, ctypes.c_char_p, ctypes.POINTER(_vp)]
+        L.rq_scene_load.restype = ctypes.c_int
     if hasattr(L, "rq_final_gather"):
         L.rq_final_gather.argtypes = [_vp, _vp, _u32, ctypes.c_float, _vp, _u32, ctypes.c_float, _vp, _vp, _u32, _vp,
                                       _u32, _vp, _vp, _vp]
@@ -179,7 +184,10 @@ class Device:
             self.h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: the library may already be gone
+            pass
 
 
 class Scene:
@@ -205,7 +213,10 @@ class Scene:
             self.h = None
 
     def __del__(self):
-        self.destroy()
+        try:
+            self.destroy()
+        except Exception:  # interpreter shutdown: the library may already be gone
+            pass
 
     def stats(self):
         info = SceneInfo()
@@ -228,6 +239,23 @@ class Scene:
         d["node_util"] = d["nodes"] / max(1, 64 * d["node_iters"])
         d["tri_util"] = d["tris"] / max(1, 64 * d["tri_iters"])
         return d
+
+    def save(self, path):
+        """Write the scene to a packed scene file."""
+        check(load().rq_scene_save(self.h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, device, path):
+        """A scene read from a packed scene file (no build)."""
+        h = _vp()
+        check(load().rq_scene_load(device.h, os.fsencode(path), ctypes.byref(h)))
+        sc = cls.__new__(cls)
+        sc.h = h
+        sc.device = device
+        st = sc.stats()
+        sc.accel = "kd" if st["accel"] == ACCEL["kd"] else "bvh"
+        sc.predicate = [k for k, v in PRED.items() if v == st["predicate"]][0]
+        return sc
 
     def final_gather(self, indirect, k_ind, r_ind, direct, k_dir, r_dir, diffuse, origin, dirs, nper, rgb, ri=None,
                      stream=None):
@@ -295,7 +323,10 @@ class Batch:
             self.h = None
 
     def __del__(self):
-        self.destroy()
+        try:
+            self.destroy()
+        except Exception:  # interpreter shutdown: the library may already be gone
+            pass
 
 
 class Server:
@@ -350,7 +381,10 @@ class Server:
             self._live.clear()
 
     def __del__(self):
-        self.destroy()
+        try:
+            self.destroy()
+        except Exception:  # interpreter shutdown: the library may already be gone
+            pass
 
 
 class PhotonMap:
@@ -374,7 +408,10 @@ class PhotonMap:
             self.h = None
 
     def __del__(self):
-        self.destroy()
+        try:
+            self.destroy()
+        except Exception:  # interpreter shutdown: the library may already be gone
+            pass
 
     def order(self):
         perm = np.zeros(max(1, self.n), np.uint32)

@@ -134,6 +134,18 @@ rq_status rq_scene_create(rq_device* dev, const float* xyz, uint32_t nverts,
 void rq_scene_destroy(rq_scene* scene);
 rq_status rq_scene_stats(const rq_scene* scene, rq_scene_info* info);
 
+/* Packed scene files.  rq_scene_save writes a resident scene (its node,
+ * triangle, leaf-reference and prim-map arrays, with a versioned header and
+ * an FNV-1a checksum) to `path`; rq_scene_load reads one back onto `dev`
+ * without a build.  The loaded scene answers every query bit for bit as the
+ * saved one.  A file from another library version, with another record
+ * layout, truncated, with trailing bytes or a bad checksum is refused
+ * (RQ_ERR_INVALID_ARG or RQ_ERR_UNSUPPORTED).  No counterpart in ENCORE,
+ * which rebuilds its tree from the model list on every start
+ * (Kdtree.cpp:84-200). */
+rq_status rq_scene_save(const rq_scene* scene, const char* path);
+rq_status rq_scene_load(rq_device* dev, const char* path, rq_scene** out);
+
 /* Closest hit for n rays: d_rays[n] -> d_hits[n].  Replaces
  * AccelerationStructure::intersect (AccelerationStructure.h:63,
  * Kdtree.cpp:682-784) for a batch. */
