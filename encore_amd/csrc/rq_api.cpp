@@ -130,6 +130,10 @@ static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, ui
         launch_bvh(L, sc->predicate, kind, sc->width, grid, dev->kc, s);
 }
 
+void trace_closest(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, hipStream_t s) {
+    trace(sc, rays, n, hits, nullptr, TraceKind::Closest, s);
+}
+
 // Model-list bounds for the kd root: given, or the vertex bounds.
 static void vertex_bounds(const float* xyz, uint32_t nverts, const rq_scene_desc& dd, float bmin[3], float bmax[3]) {
     if (dd.has_bounds) {

@@ -283,6 +283,74 @@ void launch_fg_rays(const FgLaunch& L, hipStream_t s);
 void launch_fg_attr(const FgLaunch& L, hipStream_t s);
 void launch_fg_reduce(const FgLaunch& L, hipStream_t s);
 
+// Photon emission (photon_trace.hip).
+// A material as the tracer uses it: pd = mean diffuse reflectance,
+// pt = transmittance, ptotal = 0 + pd + pt (PhotonMapperCPU.cpp:615-618).
+struct PhMaterial {
+    float pd, pt, ptotal, transparency, index, pad[3];
+};
+static_assert(sizeof(PhMaterial) == 32, "material record is 32 bytes");
+struct PhLight {
+    float center[3], normal[3], length, width, power[3], pad[3];
+};
+// Draws per path: slot 0 light, 1-2 emission point, 3-4 emission direction,
+// then per bounce b: 5+3b the interaction draw, 6+3b and 7+3b the diffuse
+// direction.  Event slots per path: 0 the first stored photon, 1 its shadow
+// photon, b+1 the photon stored at bounce b >= 1.
+constexpr uint32_t kPhMaxBounces = 16;
+constexpr uint32_t kPhPathBits = 22;  // live-ray tag: local path in bits 0..21
+constexpr uint32_t kPhMaxBatch = 1u << kPhPathBits;
+constexpr uint32_t kPhScanBlock = 256;
+constexpr uint32_t kPhTableSize = 10000;  // rand() % 10000 outcomes
+struct PhOut {
+    float* pos;
+    float* nrm;
+    float* pow;
+    float* dir;
+    uint8_t* shadow;
+    uint32_t* path;
+};
+struct PhLaunch {
+    const TriRec* tris;
+    const uint32_t* prim_slot;
+    uint32_t predicate;
+    const uint32_t* prim_mat;
+    const PhMaterial* mats;
+    const PhLight* lights;
+    uint32_t nlights;
+    const float* phi_cs;  // kPhTableSize (cos, sin) pairs
+    uint64_t seed, base;  // draws key; global index of the batch's first path
+    uint32_t npaths, max_bounces, nslots;
+    rq_ray* path_ray;     // npaths: the photon's current ray
+    float* path_dir;      // npaths x 3: emission direction
+    uint32_t* path_light; // npaths
+    float* ev_pos;        // npaths x nslots x 3
+    float* ev_nrm;
+    uint8_t* ev_tag;      // npaths x nslots: map + 1 (0 none), bit 2 shadow
+    const rq_ray* in_rays;
+    const uint32_t* in_tags;
+    const rq_hit* hits;
+    uint32_t nin;
+    rq_ray* out_rays;
+    uint32_t* out_tags;
+    uint32_t* out_count;
+    unsigned long long* cnt;  // npaths: packed per-map event counts (21 bits each)
+    uint32_t* boff;           // nblocks x 3: exclusive block offsets
+    uint32_t nblocks;
+    uint32_t* ctl;            // [0..1] stop path + 1 (global, caustic), [2..4] written, [5..7] batch totals
+    uint32_t rem[3], acc[3];
+    float scale[3];
+    PhOut out[3];
+};
+void launch_ph_emit(const PhLaunch& L, hipStream_t s);
+void launch_ph_shade(const PhLaunch& L, hipStream_t s);
+void launch_ph_compact(const PhLaunch& L, hipStream_t s);
+// (cos, sin) of 2 * PI * (k / 10000) for every rand() % 10000 outcome k, in
+// float as Common.cpp:150-156 evaluates them (PI = 3.14159f).
+const std::vector<float>& ph_phi_table();
+// Closest hit of n rays on the scene (the traversal behind rq_intersect).
+void trace_closest(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, hipStream_t s);
+
 // Threads per block: the BVH kernels run one wave per block (their LDS use
 // per wave is large, so single-wave blocks pack a CU best); the kd kernel
 // four.
@@ -340,6 +408,9 @@ struct rq_scene {
     float root_pos[3] = {0, 0, 0}, root_size[3] = {0, 0, 0};
     rq::BuildStats stats;
     uint64_t device_bytes = 0;
+    rq::DevBuf materials;  // PhMaterial per material (rq_scene_set_materials)
+    rq::DevBuf prim_mat;   // prim id -> material index
+    uint32_t nmaterials = 0;
 };
 
 // Per-lane spill region (dwords) that covers the deepest traversal stack of

@@ -74,6 +74,32 @@ _vp = ctypes.c_void_p
 _u32 = ctypes.c_uint32
 
 
+class Material(ctypes.Structure):
+    _fields_ = [("diffuse", ctypes.c_float * 3), ("transmittance", ctypes.c_float),
+                ("transparency", ctypes.c_float), ("refraction_index", ctypes.c_float)]
+
+
+class RectLight(ctypes.Structure):
+    _fields_ = [("center", ctypes.c_float * 3), ("normal", ctypes.c_float * 3), ("length", ctypes.c_float),
+                ("width", ctypes.c_float), ("power", ctypes.c_float * 3)]
+
+
+class PhotonParams(ctypes.Structure):
+    _fields_ = [("max_bounces", _u32), ("capacity", _u32 * 3), ("scale", ctypes.c_float * 3),
+                ("seed", ctypes.c_uint64), ("max_paths", ctypes.c_uint64), ("batch_paths", _u32), ("pad", _u32)]
+
+
+class PhotonBuffers(ctypes.Structure):
+    _fields_ = [("pos", _vp), ("normal", _vp), ("power", _vp), ("dir", _vp), ("shadow", _vp), ("path", _vp)]
+
+
+class PhotonResult(ctypes.Structure):
+    _fields_ = [("count", _u32 * 3), ("paths", ctypes.c_uint64)]
+
+
+MAPS = ("global", "direct", "caustic")
+
+
 def load():
     global _LIB
     if _LIB is not None:
@@ -130,6 +156,12 @@ def load():
         L.rq_final_gather.argtypes = [_vp, _vp, _u32, ctypes.c_float, _vp, _u32, ctypes.c_float, _vp, _vp, _u32, _vp,
                                       _u32, _vp, _vp, _vp]
         L.rq_final_gather.restype = ctypes.c_int
+    if hasattr(L, "rq_emit_photons"):
+        L.rq_scene_set_materials.argtypes = [_vp, _vp, _u32, _vp]
+        L.rq_scene_set_materials.restype = ctypes.c_int
+        L.rq_emit_photons.argtypes = [_vp, _vp, _u32, ctypes.POINTER(PhotonParams), ctypes.POINTER(PhotonBuffers),
+                                      ctypes.POINTER(PhotonResult), _vp]
+        L.rq_emit_photons.restype = ctypes.c_int
     L.rq_debug_fail_alloc_after.argtypes = [ctypes.c_int64]
     L.rq_debug_fail_alloc_after.restype = None
     L.rq_host_tree_build.argtypes = [_vp, _u32, _vp, _u32, ctypes.POINTER(SceneDesc), ctypes.POINTER(_vp)]
@@ -264,6 +296,61 @@ class Scene:
         check(load().rq_final_gather(self.h, indirect.h, int(k_ind), float(r_ind), direct.h if direct else None,
                                      int(k_dir), float(r_dir), _ptr(diffuse), _ptr(origin), origin.shape[0],
                                      _ptr(dirs), int(nper), _ptr(rgb), _ptr(ri), _stream(stream)))
+
+    def set_materials(self, materials, prim_material):
+        """materials: sequence of dicts (diffuse, transmittance=0,
+        transparency=0, refraction_index=0); prim_material: (ntris,) ints."""
+        arr = (Material * len(materials))()
+        for i, m in enumerate(materials):
+            arr[i].diffuse[:] = [float(x) for x in m["diffuse"]]
+            arr[i].transmittance = float(m.get("transmittance", 0.0))
+            arr[i].transparency = float(m.get("transparency", 0.0))
+            arr[i].refraction_index = float(m.get("refraction_index", 0.0))
+        pm = np.ascontiguousarray(prim_material, np.uint32)
+        check(load().rq_scene_set_materials(self.h, ctypes.cast(arr, _vp), len(materials), pm.ctypes.data))
+
+    def emit_photons(self, lights, capacity, scale=(0.1, 0.05, 0.00005), max_bounces=3, seed=0, max_paths=0,
+                     batch_paths=0, extras=True, stream=None):
+        """Photon emission into three maps (global, direct, caustic).  lights:
+        sequence of dicts (center, normal, length, width, power).  Returns
+        (maps, paths): maps[name] = dict of torch tensors (pos, normal, power,
+        and with extras dir, shadow, path) cut to the stored count."""
+        import torch
+        la = (RectLight * len(lights))()
+        for i, l in enumerate(lights):
+            la[i].center[:] = [float(x) for x in l["center"]]
+            la[i].normal[:] = [float(x) for x in l["normal"]]
+            la[i].length = float(l["length"])
+            la[i].width = float(l["width"])
+            la[i].power[:] = [float(x) for x in l.get("power", (1.0, 1.0, 1.0))]
+        P = PhotonParams()
+        P.max_bounces = int(max_bounces)
+        P.capacity[:] = [int(c) for c in capacity]
+        P.scale[:] = [float(x) for x in scale]
+        P.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        P.max_paths = int(max_paths)
+        P.batch_paths = int(batch_paths)
+        bufs = (PhotonBuffers * 3)()
+        keep = []
+        for m in range(3):
+            c = max(1, int(capacity[m]))
+            t = dict(pos=torch.empty((c, 3), device="cuda"), normal=torch.empty((c, 3), device="cuda"),
+                     power=torch.empty((c, 3), device="cuda"))
+            if extras:
+                t.update(dir=torch.empty((c, 3), device="cuda"), shadow=torch.empty((c,), dtype=torch.uint8,
+                                                                                     device="cuda"),
+                         path=torch.empty((c,), dtype=torch.int32, device="cuda"))
+            for k, v in t.items():
+                setattr(bufs[m], k, v.data_ptr())
+            keep.append(t)
+        res = PhotonResult()
+        check(load().rq_emit_photons(self.h, ctypes.cast(la, _vp), len(lights), ctypes.byref(P), bufs,
+                                     ctypes.byref(res), _stream(stream)))
+        maps = {}
+        for m, name in enumerate(MAPS):
+            n = int(res.count[m])
+            maps[name] = {k: v[:n] for k, v in keep[m].items()}
+        return maps, int(res.paths)
 
     def counts(self, rays, any_hit=False, n=None, stream=None):
         """(rays, nodes visited, triangles tested) of one query (synchronous)."""

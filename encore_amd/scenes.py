@@ -535,6 +535,71 @@ def config4_scene(scale=1.0 / 20.0, tess=32, seed=SEED_BASE + 1):
     return tessellate(xyz, idx, tess)
 
 
+# Materials of environment/scenes/diffuseCornell.txt:5-30 in cornell()'s
+# triangle order: floor, ceiling, back (0.7 grey), left (red), right (blue),
+# then the two boxes (0.7, 0.7, 0.5).
+CORNELL_MATERIALS = [
+    dict(diffuse=(0.7, 0.7, 0.7)),
+    dict(diffuse=(0.6, 0.36, 0.36)),
+    dict(diffuse=(0.36, 0.36, 0.6)),
+    dict(diffuse=(0.7, 0.7, 0.5)),
+]
+# fgCompCornell.txt:28-33's outer walls: no diffuse reflection, refracting
+# (transparency 0.8, index 0.5, transmittance 0.3).
+GLASS = dict(diffuse=(0.0, 0.0, 0.0), transmittance=0.3, transparency=0.8, refraction_index=0.5)
+
+
+def cornell_prim_materials(tess=1):
+    """Material index of every triangle of cornell() (tessellated tess x tess
+    by tessellate(), which keeps children next to their parent)."""
+    ids = np.array([0, 0, 0, 0, 0, 0, 1, 1, 2, 2] + [3] * 24, np.uint32)
+    return np.repeat(ids, tess * tess)
+
+
+def cornell_light(scale=1.0 / 20.0, power=(1.0, 1.0, 1.0)):
+    """ENCORE's RECT_LIGHT (RectangleLight.cpp:14-80): a 30 x 30 square at
+    y = 49.99999 facing down, centre the mean of its corners, at world scale
+    `scale`."""
+    y = np.float32(49.99999)
+    cy = np.float32(np.float32(np.float32(y + y) + y) + y) / np.float32(4.0)
+    s = np.float32(scale)
+    return dict(center=(0.0, float(np.float32(cy * s)), 0.0), normal=(0.0, -1.0, 0.0),
+                length=float(np.float32(30.0) * s), width=float(np.float32(30.0) * s), power=power)
+
+
+def glass_shell(scale=1.0 / 20.0, tess=1):
+    """fgCompCornell's outer walls: ceiling, floor, back, right and left at
+    +-55 spanning 150, normals facing the inside (the winding of the cornell_*
+    PLY models).  Returns xyz, idx."""
+    h, e = 55.0, 75.0
+    walls = [
+        [[-e, h, -e], [e, h, -e], [e, h, e], [-e, h, e]],      # ceiling (normal -y)
+        [[-e, -h, -e], [-e, -h, e], [e, -h, e], [e, -h, -e]],  # floor (+y)
+        [[-e, -e, -h], [e, -e, -h], [e, e, -h], [-e, e, -h]],  # back (+z)
+        [[h, -e, -e], [h, -e, e], [h, e, e], [h, e, -e]],      # right (-x)
+        [[-h, -e, -e], [-h, e, -e], [-h, e, e], [-h, -e, e]],  # left (+x)
+    ]
+    xyz, idx = _merge([(np.array(w), _QUAD) for w in walls])
+    xyz = (xyz.astype(np.float64) * scale).astype(np.float32)
+    return tessellate(xyz, idx, tess) if tess > 1 else (xyz, idx)
+
+
+def photon_scene(tess=32, glass=False, scale=1.0 / 20.0):
+    """The config-4 Cornell box with materials (and, with `glass`, the
+    refracting outer shell of fgCompCornell).  Returns xyz, idx, materials,
+    prim_material, lights."""
+    xyz, idx = config4_scene(scale=scale, tess=tess)
+    mats = list(CORNELL_MATERIALS)
+    pm = cornell_prim_materials(tess)
+    if glass:
+        gx, gi = glass_shell(scale, tess=max(1, tess // 4))
+        idx = np.concatenate([idx, gi + len(xyz)]).astype(np.uint32)
+        xyz = np.concatenate([xyz, gx]).astype(np.float32)
+        mats.append(GLASS)
+        pm = np.concatenate([pm, np.full(len(gi), len(mats) - 1, np.uint32)])
+    return xyz, idx, mats, pm, [cornell_light(scale)]
+
+
 def normals_of(xyz, idx, prims):
     """ENCORE's hit normal Normalize(Cross(e1, e2)) (TrianglePrim.cpp:236-238)."""
     tri = idx[prims]

@@ -322,6 +322,89 @@ rq_status rq_final_gather(rq_scene* scene, rq_photons* indirect, uint32_t k_ind,
                           uint32_t npoints, const float* d_dirs, uint32_t nper, float* d_rgb, float* d_ri,
                           void* stream);
 
+/* ---------------------------------------------------------------------
+ * Photon emission: PhotonMapperCPU::EmitPhotons and TracePhoton
+ * (PhotonMapperCPU.cpp:503-775) on the device.
+ * --------------------------------------------------------------------- */
+
+/* A surface material, as the photon tracer reads it (Material.h:46-57):
+ * diffuse reflectance (its mean (r+g+b)/3 is the reflection probability),
+ * transmittance (the refraction probability), transparency (a surface
+ * refracts only when it is at least 0.6) and refraction index. */
+typedef struct rq_material {
+    float diffuse[3];
+    float transmittance;
+    float transparency;
+    float refraction_index;
+} rq_material;
+
+/* Attach materials to a scene: prim_material[ntris] (host) gives each
+ * triangle's index into materials[nmaterials].  Replaces earlier ones.
+ * Materials are not stored in packed scene files. */
+rq_status rq_scene_set_materials(rq_scene* scene, const rq_material* materials, uint32_t nmaterials,
+                                 const uint32_t* prim_material);
+
+/* Rectangle light (RectangleLight.cpp:14-80, 172-183): emission points
+ * uniform over a length x width rectangle centred at `center` facing
+ * `normal`, lifted 0.01 along the normal; cosine-weighted directions. */
+typedef struct rq_rect_light {
+    float center[3];
+    float normal[3];
+    float length, width;
+    float power[3];
+} rq_rect_light;
+
+/* Photon maps, in the order of PhotonMapperCPU's members. */
+enum { RQ_MAP_GLOBAL = 0, RQ_MAP_DIRECT = 1, RQ_MAP_CAUSTIC = 2 };
+
+typedef struct rq_photon_params {
+    uint32_t max_bounces;   /* MaxBounces (pmConfig: 3); at most 16 */
+    uint32_t capacity[3];   /* GlobalPhotons, DirectPhotons, CausticPhotons */
+    float scale[3];         /* ScalePhotons factors: PhotonScale, 0.05, 0.00005 */
+    uint64_t seed;          /* keys the counter-based draws that replace rand() */
+    uint64_t max_paths;     /* emission bound (0: max(2^24, 64 x capacities)) */
+    uint32_t batch_paths;   /* paths per device batch (0: up to 2^20) */
+    uint32_t pad;
+} rq_photon_params;
+
+/* Device output arrays of one map, each sized for its capacity.  pos,
+ * normal and power (3 floats per photon) are required when the capacity is
+ * non-zero; dir (incoming direction, 3 floats), shadow (1 byte: 1 for a
+ * shadow photon) and path (the emission index of the photon's path) may be
+ * NULL. */
+typedef struct rq_photon_buffers {
+    float* pos;
+    float* normal;
+    float* power;
+    float* dir;
+    uint8_t* shadow;
+    uint32_t* path;
+} rq_photon_buffers;
+
+typedef struct rq_photon_result {
+    uint32_t count[3];  /* photons stored per map */
+    uint64_t paths;     /* photon paths emitted */
+} rq_photon_result;
+
+/* Emit photon paths from the lights until the global map is full and then
+ * until the caustic map is full (EmitPhotons' two loops), tracing each path
+ * through the scene (TracePhoton): at every hit a draw E in [0, 1) against
+ * the hit material chooses diffuse reflection (E < mean diffuse: store, new
+ * cosine-weighted direction), refraction (E < mean diffuse +
+ * transmittance: TransmitPhoton) or absorption (store, stop).  A stored
+ * photon goes to the caustic map after a refraction, else to the global map
+ * after the first bounce, else to the direct map together with a shadow
+ * photon (negative power) at the next surface entered behind it
+ * (EmitShadowPhoton).  Each map accepts photons in emission order until it
+ * is full; powers are multiplied by the map's scale.  ENCORE draws from
+ * rand(); here every draw is a function of (seed, path, draw slot), so the
+ * photons are reproducible and the same for any batch size.  The scene needs
+ * materials (rq_scene_set_materials).  Synchronous; `stream` orders the
+ * work. */
+rq_status rq_emit_photons(rq_scene* scene, const rq_rect_light* lights, uint32_t nlights,
+                          const rq_photon_params* params, const rq_photon_buffers out[3],
+                          rq_photon_result* result, void* stream);
+
 /* Traversal work of one query over n rays, counted by an instrumented copy
  * of the same kernel (synchronous): counts[0] = rays, counts[1] = nodes
  * visited (64-byte BVH2, 128-byte BVH4 or 8-byte kd nodes), counts[2] =

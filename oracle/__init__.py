@@ -51,6 +51,13 @@ def lib():
             "orc_pm_order": (None, [_vp, _u32p, _u8p]),
             "orc_pm_knn": (None, [_vp, _f32p, _f32p, _u32, _int, ctypes.c_float, _u32p, _f32p, _u32p, _int]),
             "orc_pm_radiance": (None, [_vp, _f32p, _f32p, _u32, _int, ctypes.c_float, _int, _f32p, _int]),
+            "orc_photon_dir": (None, [_f32p, ctypes.c_float, _u32p, _u32p, _u32, _f32p]),
+            "orc_light_point": (None, [_f32p, _u32p, _u32p, _u32, _f32p]),
+            "orc_ph_rand": (None, [ctypes.c_uint64, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS"), _u32p,
+                                   _u32, _u32p]),
+            "orc_transmit_ray": (_int, [_f32p, _f32p, _f32p, _int, _f32p, _f32p]),
+            "orc_emit_photons": (None, [_vp, _f32p, _u32p, _f32p, _u32, _u32, _u32p, _f32p, ctypes.c_uint64,
+                                        ctypes.c_uint64, _vp, _u32p, ctypes.POINTER(ctypes.c_uint64)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -328,3 +335,100 @@ def final_gather(xyz, idx, kd, indirect, k_ind, r_ind, direct, k_dir, r_dir, dif
     with np.errstate(divide="ignore", invalid="ignore"):
         R = np.where(ri != 0, (nh / ri).astype(np.float32), np.float32(0)).astype(np.float32)
     return rgb, R
+
+
+MAPS = ("global", "direct", "caustic")
+
+
+def material_table(materials):
+    """rq_material rows (diffuse rgb, transmittance, transparency,
+    refraction index) from dicts."""
+    t = np.zeros((len(materials), 6), np.float32)
+    for i, m in enumerate(materials):
+        t[i, 0:3] = m["diffuse"]
+        t[i, 3] = m.get("transmittance", 0.0)
+        t[i, 4] = m.get("transparency", 0.0)
+        t[i, 5] = m.get("refraction_index", 0.0)
+    return t
+
+
+def light_table(lights):
+    """rq_rect_light rows (center, normal, length, width, power) from dicts."""
+    t = np.zeros((len(lights), 11), np.float32)
+    for i, l in enumerate(lights):
+        t[i, 0:3] = l["center"]
+        t[i, 3:6] = l["normal"]
+        t[i, 6] = l["length"]
+        t[i, 7] = l["width"]
+        t[i, 8:11] = l.get("power", (1.0, 1.0, 1.0))
+    return t
+
+
+def emit_photons(kd, materials, prim_material, lights, capacity, scale=(0.1, 0.05, 0.00005), max_bounces=3, seed=0,
+                 max_paths=0):
+    """ENCORE's EmitPhotons / TracePhoton (PhotonMapperCPU.cpp:503-835) on the
+    oracle kd-tree, one path at a time, with the counter-based draws.
+    Returns (maps, paths); maps[name] = dict(pos, normal, power, dir,
+    shadow, path) of the stored photons in insertion order."""
+    cap = np.asarray(capacity, np.uint32)
+    sc = np.asarray(scale, np.float32)
+    if not max_paths:
+        max_paths = max(1 << 24, 64 * int(cap.astype(np.uint64).sum()))
+    arrays = []
+    ptrs = (ctypes.c_void_p * 18)()
+    for m in range(3):
+        c = max(1, int(cap[m]))
+        a = dict(pos=np.zeros((c, 3), np.float32), normal=np.zeros((c, 3), np.float32),
+                 power=np.zeros((c, 3), np.float32), dir=np.zeros((c, 3), np.float32),
+                 shadow=np.zeros(c, np.uint8), path=np.zeros(c, np.uint32))
+        for j, k in enumerate(("pos", "normal", "power", "dir", "shadow", "path")):
+            ptrs[6 * m + j] = a[k].ctypes.data
+        arrays.append(a)
+    counts = np.zeros(3, np.uint32)
+    paths = ctypes.c_uint64(0)
+    mt = material_table(materials)
+    lt = light_table(lights)
+    pm = _cu32(prim_material).reshape(-1)
+    lib().orc_emit_photons(kd.h, mt, pm, lt, len(lights), int(max_bounces), cap, sc, int(seed) & (2 ** 64 - 1),
+                           int(max_paths), ptrs, counts, ctypes.byref(paths))
+    maps = {}
+    for m, name in enumerate(MAPS):
+        n = int(counts[m])
+        maps[name] = {k: v[:n] for k, v in arrays[m].items()}
+    return maps, int(paths.value)
+
+
+def photon_dir(normal, e1_lo, r1, r2):
+    """randDirectionN / randDirectionSuperN about `normal` for draws r1, r2."""
+    r1 = _cu32(r1).reshape(-1)
+    r2 = _cu32(r2).reshape(-1)
+    out = np.zeros((len(r1), 3), np.float32)
+    lib().orc_photon_dir(_c32(normal).reshape(3), float(e1_lo), r1, r2, len(r1), out)
+    return out
+
+
+def light_point(light, r1, r2):
+    """RectangleLight::EmitPhoton's origin for draws r1, r2."""
+    r1 = _cu32(r1).reshape(-1)
+    r2 = _cu32(r2).reshape(-1)
+    out = np.zeros((len(r1), 3), np.float32)
+    lib().orc_light_point(light_table([light])[0], r1, r2, len(r1), out)
+    return out
+
+
+def ph_rand(seed, path, slot):
+    """The counter-based draw that replaces rand() for (path, slot) pairs."""
+    path = np.ascontiguousarray(path, np.uint64).reshape(-1)
+    slot = _cu32(slot).reshape(-1)
+    out = np.zeros(len(path), np.uint32)
+    lib().orc_ph_rand(int(seed) & (2 ** 64 - 1), path, slot, len(path), out)
+    return out
+
+
+def transmit_ray(d, P, N, entering, material):
+    """TransmitRay for one hit: (origin, direction) or None when the surface
+    does not refract."""
+    out = np.zeros(6, np.float32)
+    ok = lib().orc_transmit_ray(_c32(d).reshape(3), _c32(P).reshape(3), _c32(N).reshape(3), int(entering),
+                                material_table([material])[0], out)
+    return (out[:3], out[3:]) if ok else None

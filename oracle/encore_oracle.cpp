@@ -909,3 +909,333 @@ void orc_pm_radiance(void* h, const float* qpos, const float* qnrm, uint32_t nq,
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Photon emission (PhotonMapperCPU.cpp:503-835), one path at a time as the
+// reference runs it: EmitPhotons' two loops, TracePhoton's bounce loop with
+// its Full() checks, ReflectPhotonDiffusely, EmitShadowPhoton, TransmitPhoton
+// and TransmitRay.  rand() is replaced by a counter-based draw of (seed, path,
+// slot) reduced modulo 10000 as randNumber reduces rand() (Common.cpp:17-21):
+//   slot 0 light (rand() % lights), 1-2 emission point, 3-4 emission
+//   direction, 5+3b interaction draw at bounce b, 6+3b and 7+3b the diffuse
+//   direction.
+// The direction samplers (randDirectionN / randDirectionSuperN,
+// Common.cpp:98-182) keep their formulas with cos(acos(x)) = x,
+// sin(acos(x)) = sqrt(1 - x^2), and the cosine and sine of 2*PI*e2 from
+// double-precision libm rounded to float.  GetRotationMatrix
+// (TAffineMatrix.h:160-200) keeps its unnormalised axis.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct OLight {  // layout of rq_rect_light
+    float center[3], normal[3], length, width, power[3];
+};
+struct OMat {  // layout of rq_material
+    float diffuse[3], transmittance, transparency, index;
+};
+
+inline F3 f3(float x, float y, float z) {
+    F3 r;
+    r.c[0] = x;
+    r.c[1] = y;
+    r.c[2] = z;
+    return r;
+}
+inline F3 add3(const F3& a, const F3& b) { return f3(a.c[0] + b.c[0], a.c[1] + b.c[1], a.c[2] + b.c[2]); }
+inline F3 mul3(const F3& a, float s) { return f3(a.c[0] * s, a.c[1] * s, a.c[2] * s); }
+inline F3 norm3(const F3& a) {
+    float r[3];
+    orc_normalize(a.c, r);
+    return f3(r[0], r[1], r[2]);
+}
+inline bool eq3(const F3& a, const F3& b) { return a.c[0] == b.c[0] && a.c[1] == b.c[1] && a.c[2] == b.c[2]; }
+inline F3 neg3(const F3& a) { return f3(-a.c[0], -a.c[1], -a.c[2]); }
+
+uint32_t ph_rand(uint64_t seed, uint64_t path, uint32_t slot) {
+    uint64_t z = seed + (path * 64ull + slot + 1ull) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (uint32_t)(z >> 33);
+}
+
+float rand_number(float lo, float hi, uint32_t r) {  // randNumber (Common.cpp:17-21)
+    return (hi - lo) * ((float)(int)(r % 10000u) / (float)10000) + lo;
+}
+
+// TAffineMatrix::GetRotationMatrix(axis, acos(c)) (TAffineMatrix.h:160-200)
+// applied to v: M = R^T X R with rows r = axis, s, t.
+F3 rotate_about(const F3& r, float c, const F3& v) {
+    const float ax = std::fabs(r.c[0]), ay = std::fabs(r.c[1]), az = std::fabs(r.c[2]);
+    F3 s;
+    if (ax <= ay && ax <= az)
+        s = f3(0.0f, -r.c[2], r.c[1]);
+    else if (ay <= ax && ay <= az)
+        s = f3(-r.c[2], 0.0f, r.c[0]);
+    else
+        s = f3(-r.c[1], r.c[0], 0.0f);
+    s = norm3(s);
+    const F3 t = norm3(cross_flush(r, s));
+    const float sa = std::sqrt((1.0f - c) * (1.0f + c));
+    const float w0 = dot3(r, v), w1 = dot3(s, v), w2 = dot3(t, v);
+    const float u1 = c * w1 - sa * w2, u2 = sa * w1 + c * w2;
+    F3 o;
+    for (int k = 0; k < 3; ++k) o.c[k] = (r.c[k] * w0 + s.c[k] * u1) + t.c[k] * u2;
+    return o;
+}
+
+// randDirectionN (e1 from 0.01) / randDirectionSuperN (e1 from 0.001).
+F3 rand_dir_about(F3 n, float e1_lo, uint32_t r1, uint32_t r2) {
+    const float e1 = rand_number(e1_lo, 1.0f, r1);
+    const float e2 = rand_number(0.0f, 1.0f, r2);
+    const float phi = (2 * 3.14159f) * e2;
+    const float cp = (float)std::cos((double)phi), sp = (float)std::sin((double)phi);
+    const float ct = std::sqrt(e1), st = std::sqrt(1.0f - e1);
+    const F3 d = f3(st * cp, st * sp, ct);
+    n = norm3(n);
+    const F3 z = f3(0.0f, 0.0f, 1.0f);
+    if (eq3(z, n)) return d;
+    if (eq3(z, neg3(n))) return neg3(d);
+    return norm3(rotate_about(cross_flush(z, n), dot3(n, z), d));
+}
+
+// RectangleLight::EmitPhoton's origin (RectangleLight.cpp:172-183) through
+// Sampler::GetSamplePointsForRect (Sampler.cpp:49-95, 154-218) with one cell
+// and full jitter.
+F3 light_point(const OLight& l, uint32_t r1, uint32_t r2) {
+    const float half = 1.0f / 2.0f;
+    const float jx = rand_number(-half, half, r1) * 1.0f;
+    const float jz = rand_number(-half, half, r2) * 1.0f;
+    F3 p = f3(half + jx, 0.0f, half + jz);
+    p = add3(f3(-0.5f, 0.0f, -0.5f), p);
+    p.c[0] *= l.length / 1.0f;
+    p.c[2] *= l.width / 1.0f;
+    const F3 n = norm3(f3(l.normal[0], l.normal[1], l.normal[2]));
+    const F3 up = f3(0.0f, 1.0f, 0.0f);
+    if (eq3(up, n)) {
+    } else if (eq3(up, neg3(n))) {
+        p = add3(p, mul3(p, -2.0f));
+    } else {
+        const float c = dot3(up, n);
+        if (c != 1.0f) p = rotate_about(norm3(cross_flush(up, n)), c, p);  // theta = acos(c) != 0
+    }
+    p = add3(f3(l.center[0], l.center[1], l.center[2]), p);
+    return add3(mul3(f3(l.normal[0], l.normal[1], l.normal[2]), 0.01f), p);
+}
+
+struct OHit {
+    bool has;
+    uint32_t prim;
+    F3 P, N;
+    bool entering;
+};
+
+OHit intersect(const KdTree& T, const F3& o, const F3& d) {
+    HitRec h;
+    kd_intersect_one(T, o, d, &h);
+    OHit r;
+    r.has = h.has;
+    r.prim = h.prim;
+    if (!h.has) return r;
+    const TriObj& t = T.scene.tris[h.prim];
+    const F3 v0 = t.v[0]->p, e1 = sub(t.v[1]->p, v0), e2 = sub(t.v[2]->p, v0);
+    for (int k = 0; k < 3; ++k) r.P.c[k] = (v0.c[k] + e1.c[k] * h.a) + e2.c[k] * h.b;  // TrianglePrim.cpp:239
+    r.N = norm3(cross_flush(e1, e2));                                                  // :236-238
+    r.entering = dot3(d, r.N) < 0;                                                      // :242
+    return r;
+}
+
+struct OMap {
+    uint32_t cap, n = 0;
+    float *pos, *nrm, *pow, *dir;
+    uint8_t* shadow;
+    uint32_t* path;
+    bool full() const { return n == cap; }  // PhotonMap::Full (PhotonMap.h:54)
+    void add(const F3& P, const F3& N, const float* power, bool neg, const F3& incoming, bool sh, uint64_t gp) {
+        if (n >= cap) return;  // PhotonMap::AddPhoton (PhotonMap.cpp:46-53)
+        for (int k = 0; k < 3; ++k) {
+            pos[3 * (size_t)n + k] = P.c[k];
+            nrm[3 * (size_t)n + k] = N.c[k];
+            pow[3 * (size_t)n + k] = neg ? -power[k] : power[k];
+            dir[3 * (size_t)n + k] = incoming.c[k];
+        }
+        shadow[n] = sh ? 1 : 0;
+        path[n] = (uint32_t)gp;
+        ++n;
+    }
+};
+
+struct OEmit {
+    const KdTree* T;
+    const OMat* mats;
+    const uint32_t* prim_mat;
+    const OLight* lights;
+    uint32_t nlights, max_bounces;
+    uint64_t seed;
+    OMap* maps;  // global, direct, caustic
+};
+
+// TransmitRay (PhotonMapperCPU.cpp:800-835).
+bool transmit_ray(const F3& d, const OHit& h, const OMat& m, F3& oo, F3& od) {
+    if (m.transparency < 0.6) return false;  // caller keeps its ray
+    float c1, c2;
+    const float air = 0.99f;
+    if (h.entering) {
+        c1 = air;
+        c2 = m.index;
+    } else {
+        c2 = air;
+        c1 = m.index;
+    }
+    const float cr = (c1 == 0) ? (float)(c2 / 0.0001) : (c2 / c1);
+    const float nd = dot3(h.N, d);
+    const float th2 = std::sqrt(1 - (cr * cr) * (1 - (nd * nd)));
+    const F3 T = norm3(add3(mul3(d, cr), mul3(h.N, cr * nd - th2)));
+    od = T;
+    oo = add3(h.P, mul3(T, 0.01f));
+    return true;
+}
+
+void trace_path(const OEmit& E, uint64_t gp) {
+    const OLight& l = E.lights[ph_rand(E.seed, gp, 0) % E.nlights];
+    const F3 origin = light_point(l, ph_rand(E.seed, gp, 1), ph_rand(E.seed, gp, 2));
+    const F3 n = f3(l.normal[0], l.normal[1], l.normal[2]);
+    const F3 dir0 = rand_dir_about(n, 0.01f, ph_rand(E.seed, gp, 3), ph_rand(E.seed, gp, 4));
+    // TracePhoton (PhotonMapperCPU.cpp:603-655)
+    F3 ro = add3(origin, mul3(dir0, 0.01f)), rd = dir0;
+    bool specular = false;  // LastReflection: NONE
+    for (uint32_t b = 0; b <= E.max_bounces; ++b) {
+        const OHit h = intersect(*E.T, ro, rd);
+        if (!h.has) break;
+        const OMat& m = E.mats[E.prim_mat[h.prim]];
+        const float pS = 0;
+        const float pD = (m.diffuse[0] + m.diffuse[1] + m.diffuse[2]) / 3.0f;
+        const float pT = m.transmittance;
+        const float pTotal = pS + pD + pT;
+        if (!(pTotal <= 1.0f)) continue;  // nothing happens at this hit
+        const float Ed = rand_number(0.0f, 1.0f, ph_rand(E.seed, gp, 5 + 3 * b));
+        if (Ed < pS) {
+        } else if (Ed < pS + pD || !(Ed < pTotal)) {
+            // ReflectPhotonDiffusely (PhotonMapperCPU.cpp:700-745)
+            if (specular) {
+                if (!E.maps[2].full()) E.maps[2].add(h.P, h.N, l.power, false, dir0, false, gp);
+            } else if (b > 0 && !E.maps[0].full()) {
+                E.maps[0].add(h.P, h.N, l.power, false, dir0, false, gp);
+            } else if (b == 0 && !E.maps[1].full()) {
+                E.maps[1].add(h.P, h.N, l.power, false, dir0, false, gp);
+                // EmitShadowPhoton (PhotonMapperCPU.cpp:775-797)
+                F3 so = add3(h.P, mul3(rd, 2.0f));
+                OHit sh = intersect(*E.T, so, rd);
+                while (sh.has && !sh.entering) {
+                    so = add3(sh.P, mul3(rd, 0.01f));
+                    sh = intersect(*E.T, so, rd);
+                }
+                if (sh.has) E.maps[1].add(sh.P, sh.N, l.power, true, rd, true, gp);
+            }
+            specular = false;
+            const F3 nd = rand_dir_about(h.N, 0.001f, ph_rand(E.seed, gp, 6 + 3 * b), ph_rand(E.seed, gp, 7 + 3 * b));
+            ro = add3(h.P, mul3(nd, 0.001f));
+            rd = nd;
+            if (!(Ed < pS + pD)) break;  // absorbed
+        } else {
+            // TransmitPhoton (PhotonMapperCPU.cpp:747-773)
+            F3 to, td;
+            if (!transmit_ray(rd, h, m, to, td)) continue;
+            const OHit h2 = intersect(*E.T, to, td);
+            if (h2.has && !h2.entering) {
+                const OMat& m2 = E.mats[E.prim_mat[h2.prim]];
+                if (!transmit_ray(td, h2, m2, ro, rd)) {
+                    ro = to;
+                    rd = td;
+                }
+            }
+            specular = true;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Pieces of the emission for direct checks: n draws of the direction sampler
+// about `nrm` (e1 from e1_lo) and of the light's emission point, for draw
+// values r1[i], r2[i]; and the counter-based draw itself.
+void orc_photon_dir(const float* nrm, float e1_lo, const uint32_t* r1, const uint32_t* r2, uint32_t n, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const F3 d = rand_dir_about(f3(nrm[0], nrm[1], nrm[2]), e1_lo, r1[i], r2[i]);
+        for (int k = 0; k < 3; ++k) out[3 * (size_t)i + k] = d.c[k];
+    }
+}
+
+void orc_light_point(const float* light, const uint32_t* r1, const uint32_t* r2, uint32_t n, float* out) {
+    OLight l;
+    std::memcpy(&l, light, sizeof(OLight));
+    for (uint32_t i = 0; i < n; ++i) {
+        const F3 p = light_point(l, r1[i], r2[i]);
+        for (int k = 0; k < 3; ++k) out[3 * (size_t)i + k] = p.c[k];
+    }
+}
+
+void orc_ph_rand(uint64_t seed, const uint64_t* path, const uint32_t* slot, uint32_t n, uint32_t* out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = ph_rand(seed, path[i], slot[i]);
+}
+
+// TransmitRay for one hit: d, P, N, entering, material row; returns 0 when the
+// surface does not refract, else writes origin and direction.
+int orc_transmit_ray(const float* d, const float* P, const float* N, int entering, const float* mat, float* out6) {
+    OHit h;
+    h.has = true;
+    h.prim = 0;
+    h.P = f3(P[0], P[1], P[2]);
+    h.N = f3(N[0], N[1], N[2]);
+    h.entering = entering != 0;
+    OMat m;
+    std::memcpy(&m, mat, sizeof(OMat));
+    F3 o, t;
+    if (!transmit_ray(f3(d[0], d[1], d[2]), h, m, o, t)) return 0;
+    for (int k = 0; k < 3; ++k) {
+        out6[k] = o.c[k];
+        out6[3 + k] = t.c[k];
+    }
+    return 1;
+}
+
+// maps: 3 x {pos, nrm, pow, dir (float*), shadow (uint8*), path (uint32*)}
+// with capacity[3]; counts[3] and paths out.  EmitPhotons (PhotonMapperCPU.cpp:
+// 537-585): paths until the global map is full, then until the caustic map is
+// full (bounded by max_paths), then ScalePhotons on each map.
+void orc_emit_photons(void* kd, const void* mats, const uint32_t* prim_mat, const void* lights, uint32_t nlights,
+                      uint32_t max_bounces, const uint32_t* capacity, const float* scale, uint64_t seed,
+                      uint64_t max_paths, void** bufs, uint32_t* counts, uint64_t* paths_out) {
+    OMap maps[3];
+    for (int m = 0; m < 3; ++m) {
+        maps[m].cap = capacity[m];
+        maps[m].pos = static_cast<float*>(bufs[6 * m]);
+        maps[m].nrm = static_cast<float*>(bufs[6 * m + 1]);
+        maps[m].pow = static_cast<float*>(bufs[6 * m + 2]);
+        maps[m].dir = static_cast<float*>(bufs[6 * m + 3]);
+        maps[m].shadow = static_cast<uint8_t*>(bufs[6 * m + 4]);
+        maps[m].path = static_cast<uint32_t*>(bufs[6 * m + 5]);
+    }
+    OEmit E;
+    E.T = static_cast<const KdTree*>(kd);
+    E.mats = static_cast<const OMat*>(mats);
+    E.prim_mat = prim_mat;
+    E.lights = static_cast<const OLight*>(lights);
+    E.nlights = nlights;
+    E.max_bounces = max_bounces;
+    E.seed = seed;
+    E.maps = maps;
+    uint64_t gp = 0;
+    while (!maps[0].full() && gp < max_paths) trace_path(E, gp++);
+    while (!maps[2].full() && gp < max_paths) trace_path(E, gp++);
+    for (int m = 0; m < 3; ++m) {
+        for (uint32_t i = 0; i < maps[m].n; ++i)
+            for (int k = 0; k < 3; ++k) maps[m].pow[3 * (size_t)i + k] *= scale[m];  // ScalePhotons
+        counts[m] = maps[m].n;
+    }
+    *paths_out = gp;
+}
+
+}  // extern "C"

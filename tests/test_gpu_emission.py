@@ -1,0 +1,157 @@
+"""GPU photon emission (rq_emit_photons) against the oracle's sequential
+restatement of EmitPhotons / TracePhoton.  On ENCORE's kd-tree every ray
+hits what the oracle's kd-tree hits, so all three maps come out bit for bit
+the same, photon for photon; on a BVH the few rays that tie or meet the
+reference's flush phantoms change their own paths only."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from encore_amd import rq, scenes
+from helpers import log
+from test_oracle_emission import _closed_box, closed_box_expectation
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("pos", "normal", "power", "dir", "shadow", "path")
+
+
+def _gpu(sc, lights, cap, **kw):
+    maps, paths = sc.emit_photons(lights, cap, **kw)
+    torch.cuda.synchronize()
+    return {m: {k: v.cpu().numpy() for k, v in maps[m].items()} for m in maps}, paths
+
+
+def _same(a, b):
+    for m in oracle.MAPS:
+        for f in FIELDS:
+            x, y = a[m][f], b[m][f]
+            if x.dtype == np.float32:
+                x, y = x.view(np.uint32), y.view(np.uint32)
+            assert np.array_equal(x, y.astype(x.dtype)), (m, f)
+
+
+@pytest.fixture(scope="module", params=[False, True], ids=["cornell", "glass"])
+def pscene(request):
+    xyz, idx, mats, pm, lights = scenes.photon_scene(tess=8, glass=request.param)
+    kd = oracle.KdTree(xyz, idx)
+    cap = (25000, 50000, 2000 if request.param else 0)  # pmConfig.txt: GlobalPhotons 25000, DirectPhotons 50000
+    o_maps, o_paths = oracle.emit_photons(kd, mats, pm, lights, cap, seed=7)
+    return dict(xyz=xyz, idx=idx, mats=mats, pm=pm, lights=lights, kd=kd, cap=cap, o=(o_maps, o_paths),
+                glass=request.param)
+
+
+def test_emission_kd_matches_oracle(device, pscene):
+    f = pscene
+    log("case: photon emission kd, glass=%s, capacities %s" % (f["glass"], f["cap"]))
+    sc = rq.Scene(device, f["xyz"], f["idx"], accel="kd")
+    sc.set_materials(f["mats"], f["pm"])
+    g, paths = _gpu(sc, f["lights"], f["cap"], seed=7)
+    o, o_paths = f["o"]
+    log("paths gpu %d oracle %d; counts %s" % (paths, o_paths, [len(g[m]["pos"]) for m in oracle.MAPS]))
+    assert paths == o_paths
+    _same(g, o)
+    sc.destroy()
+
+
+def test_emission_is_independent_of_batch_size(device, pscene):
+    f = pscene
+    log("case: photon emission batch sizes, glass=%s" % f["glass"])
+    sc = rq.Scene(device, f["xyz"], f["idx"], accel="kd")
+    sc.set_materials(f["mats"], f["pm"])
+    a, pa = _gpu(sc, f["lights"], f["cap"], seed=7, batch_paths=1024)
+    b, pb = _gpu(sc, f["lights"], f["cap"], seed=7, batch_paths=3000)
+    assert pa == pb == f["o"][1]
+    _same(a, b)
+    _same(a, f["o"][0])
+    sc.destroy()
+
+
+@pytest.mark.parametrize("width", [4, 2])
+def test_emission_bvh_close_to_oracle(device, pscene, width):
+    """Photons are compared by (path, position, normal): at least 99% of the
+    oracle's photons of each map appear identically."""
+    f = pscene
+    log("case: photon emission bvh%d, glass=%s" % (width, f["glass"]))
+    sc = rq.Scene(device, f["xyz"], f["idx"], accel="bvh", width=width)
+    sc.set_materials(f["mats"], f["pm"])
+    g, paths = _gpu(sc, f["lights"], f["cap"], seed=7)
+    o, o_paths = f["o"]
+    assert abs(paths - o_paths) <= 0.01 * o_paths
+    for m in oracle.MAPS:
+        assert len(g[m]["pos"]) == len(o[m]["pos"])
+        if not len(o[m]["pos"]):
+            continue
+
+        def keys(x):
+            return set(zip(x["path"].tolist(), map(bytes, x["pos"].view(np.uint32)),
+                           map(bytes, x["normal"].view(np.uint32))))
+        common = len(keys(g[m]) & keys(o[m]))
+        log("bvh%d %s: %d of %d photons identical" % (width, m, common, len(o[m]["pos"])))
+        assert common >= 0.99 * len(o[m]["pos"])
+    sc.destroy()
+
+
+@pytest.mark.parametrize("cap", [(4000, 700, 0), (1, 1, 0), (0, 500, 300), (3000, 6000, 1)])
+def test_emission_capacity_edges(device, cap):
+    """The direct map filling first, one-photon maps, no global map (only
+    the caustic loop runs), a one-photon caustic map: all as the oracle."""
+    xyz, idx, mats, pm, lights = scenes.photon_scene(tess=4, glass=True)
+    kd = oracle.KdTree(xyz, idx)
+    o, o_paths = oracle.emit_photons(kd, mats, pm, lights, cap, seed=21, max_bounces=5)
+    log("case: photon capacities %s" % (cap,))
+    sc = rq.Scene(device, xyz, idx, accel="kd")
+    sc.set_materials(mats, pm)
+    g, paths = _gpu(sc, lights, cap, seed=21, max_bounces=5, batch_paths=2048)
+    assert paths == o_paths
+    _same(g, o)
+    sc.destroy()
+
+
+def test_emission_closed_box(device):
+    xyz, idx = _closed_box()
+    light = dict(center=(0.0, 0.9, 0.0), normal=(0.0, -1.0, 0.0), length=0.5, width=0.5)
+    log("case: photon emission closed box")
+    for accel in ("kd", "bvh"):
+        sc = rq.Scene(device, xyz, idx, accel=accel)
+        sc.set_materials([dict(diffuse=(1, 1, 1))], np.zeros(12, np.uint32))
+        g, paths = _gpu(sc, [light], (1000, 5000, 0), seed=5)
+        assert 334 <= paths <= 345
+        closed_box_expectation(g, paths, 1000)
+        sc.destroy()
+
+
+def test_emission_bound_when_a_map_cannot_fill(device):
+    xyz, idx = _closed_box()
+    light = dict(center=(0.0, 0.9, 0.0), normal=(0.0, -1.0, 0.0), length=0.5, width=0.5)
+    kd = oracle.KdTree(xyz, idx)
+    o, o_paths = oracle.emit_photons(kd, [dict(diffuse=(1, 1, 1))], np.zeros(12, np.uint32), [light], (30, 5000, 10),
+                                     seed=5, max_paths=5000)
+    sc = rq.Scene(device, xyz, idx, accel="kd")
+    sc.set_materials([dict(diffuse=(1, 1, 1))], np.zeros(12, np.uint32))
+    g, paths = _gpu(sc, [light], (30, 5000, 10), seed=5, max_paths=5000, batch_paths=1024)
+    assert paths == o_paths == 5000
+    _same(g, o)
+    sc.destroy()
+
+
+def test_emission_rejects_bad_arguments(device):
+    xyz, idx = _closed_box()
+    light = dict(center=(0.0, 0.9, 0.0), normal=(0.0, -1.0, 0.0), length=0.5, width=0.5)
+    sc = rq.Scene(device, xyz, idx, accel="bvh")
+    with pytest.raises(rq.RQError) as e:  # no materials yet
+        sc.emit_photons([light], (10, 10, 0))
+    assert e.value.status == 1
+    with pytest.raises(rq.RQError):
+        sc.set_materials([dict(diffuse=(1, 1, 1))], np.ones(12, np.uint32))  # index out of range
+    sc.set_materials([dict(diffuse=(1, 1, 1))], np.zeros(12, np.uint32))
+    with pytest.raises(rq.RQError):
+        sc.emit_photons([light], (10, 10, 0), max_bounces=17)
+    with pytest.raises(rq.RQError):
+        sc.emit_photons([dict(light, normal=(0.0, 0.0, 0.0))], (10, 10, 0))
+    with pytest.raises(rq.RQError):
+        sc.emit_photons([], (10, 10, 0))
+    maps, paths = sc.emit_photons([light], (0, 10, 0))  # both loops end at once
+    assert paths == 0 and all(len(maps[m]["pos"]) == 0 for m in maps)
+    sc.destroy()

@@ -1,0 +1,306 @@
+"""Photon-emission oracle on the CPU: the direction samplers, the light's
+emission point and TransmitRay against closed forms and against the
+reference's matrix form evaluated independently in float64; the sequential
+map filling of EmitPhotons on scenes whose photon counts and positions have
+closed forms.
+
+The reference draws from rand(); the oracle and the engine draw from a
+counter-based stream reduced the same way (rand() % 10000), so the checks
+here are of the formulas and the control flow, and of the distributions the
+draws produce."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from encore_amd import scenes
+
+LIGHT = scenes.cornell_light()
+
+
+def _local_dirs(e1_lo, r1, r2):
+    """randDirectionN's +z-frame direction in float64 with the reference's
+    trigonometry: theta = acos(sqrt(e1)), phi = 2*PI*e2."""
+    k1 = (np.asarray(r1) % 10000).astype(np.float64)
+    k2 = (np.asarray(r2) % 10000).astype(np.float64)
+    e1 = (1.0 - np.float32(e1_lo)) * (k1 / 10000.0) + np.float32(e1_lo)
+    e2 = k2 / 10000.0
+    th = np.arccos(np.sqrt(e1))
+    ph = 2 * np.float32(3.14159) * e2
+    return np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], axis=1)
+
+
+def _draws(n, seed=1):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, 2 ** 31, n).astype(np.uint32), rng.integers(0, 2 ** 31, n).astype(np.uint32)
+
+
+def test_draws_are_uniform_and_keyed():
+    paths = np.repeat(np.arange(40000, dtype=np.uint64), 5)
+    slots = np.tile(np.arange(5, dtype=np.uint32), 40000)
+    r = oracle.ph_rand(123, paths, slots)
+    k = r % 10000
+    hist = np.bincount(k // 100, minlength=100)
+    chi2 = ((hist - len(k) / 100) ** 2 / (len(k) / 100)).sum()
+    assert chi2 < 99 + 5 * math.sqrt(2 * 99)  # 99 degrees of freedom
+    assert r.max() < 2 ** 31  # a 31-bit value, as rand() returns
+    assert np.array_equal(r, oracle.ph_rand(123, paths, slots))
+    assert not np.array_equal(r, oracle.ph_rand(124, paths, slots))
+    a = (r.reshape(-1, 5)[:, 0] % 10000) / 1e4
+    b = (r.reshape(-1, 5)[:, 1] % 10000) / 1e4
+    assert abs(np.corrcoef(a, b)[0, 1]) < 0.02
+
+
+@pytest.mark.parametrize("e1_lo", [0.01, 0.001])
+def test_direction_about_axes(e1_lo):
+    """n = +z returns the local direction, n = -z its negation; n = +x, +y,
+    -y are quarter and half turns about the axis z x n (Rodrigues)."""
+    r1, r2 = _draws(500)
+    loc = _local_dirs(e1_lo, r1, r2)
+    got = oracle.photon_dir([0, 0, 1], e1_lo, r1, r2)
+    assert np.allclose(got, loc, atol=1e-6)
+    got = oracle.photon_dir([0, 0, -1], e1_lo, r1, r2)
+    assert np.allclose(got, -loc, atol=1e-6)
+    x, y, z = loc[:, 0], loc[:, 1], loc[:, 2]
+    cases = {(1, 0, 0): np.stack([z, y, -x], 1), (0, 1, 0): np.stack([x, z, -y], 1),
+             (0, -1, 0): np.stack([x, -z, y], 1)}
+    for n, want in cases.items():
+        got = oracle.photon_dir(n, e1_lo, r1, r2)
+        # a rotated direction is normalised with the 1e-3 flush
+        assert np.allclose(got, _flush64(want), atol=1e-6), n
+        assert (got == 0).any() or np.abs(want).min() >= 1e-3
+
+
+def _flush64(v):
+    return np.where(np.abs(v) < 1e-3, 0.0, v)
+
+
+def _reference_matrix_dir(n, e1_lo, r1, r2):
+    """Common.cpp:160-181 evaluated in float64 from the source text: axis =
+    Cross(z, n) (float, flushed, not normalised), GetRotationMatrix's rows r,
+    s, t and R^T X(acos(n.z)) R, then Normalize."""
+    n = scenes.normalize_flush(np.asarray(n, np.float32))
+    r = scenes.cross_flush(np.array([0, 0, 1], np.float32), n).astype(np.float64)
+    a = np.abs(r)
+    if a[0] <= a[1] and a[0] <= a[2]:
+        s = np.array([0, -r[2], r[1]])
+    elif a[1] <= a[0] and a[1] <= a[2]:
+        s = np.array([-r[2], 0, r[0]])
+    else:
+        s = np.array([-r[1], r[0], 0])
+    s /= np.linalg.norm(s)
+    t = np.cross(r, s)
+    t /= np.linalg.norm(t)
+    R = np.stack([r, s, t])
+    ang = math.acos(float(n[2]))
+    X = np.array([[1, 0, 0], [0, math.cos(ang), -math.sin(ang)], [0, math.sin(ang), math.cos(ang)]])
+    M = R.T @ X @ R
+    d = _local_dirs(e1_lo, r1, r2) @ M.T
+    return _flush64(d / np.linalg.norm(d, axis=1, keepdims=True)), n
+
+
+def test_direction_tilted_normals_match_matrix_form():
+    """For tilted normals the rotation axis Cross(z, n) has length sin(theta)
+    < 1 and GetRotationMatrix does not normalise it, so the component along
+    the axis is scaled by sin^2(theta): directions stay in n's hemisphere but
+    are not cosine-distributed.  The restatement equals the reference's
+    matrix product evaluated independently."""
+    rng = np.random.default_rng(5)
+    r1, r2 = _draws(400, 9)
+    for _ in range(30):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        got = oracle.photon_dir(n, 0.001, r1, r2)
+        want, nn = _reference_matrix_dir(n, 0.001, r1, r2)
+        assert np.allclose(got, want, atol=3e-6), n
+        assert (got.astype(np.float64) @ nn.astype(np.float64) > 0).all()
+        # v = +z goes to n itself: the sampler's most probable direction
+        zc = oracle.photon_dir(n, 0.001, [9999], [0])
+        assert np.allclose(zc[0], nn, atol=0.05)
+
+
+@pytest.mark.parametrize("e1_lo,normal", [(0.001, (0, 1, 0)), (0.01, (0, -1, 0)), (0.001, (1, 0, 0))])
+def test_direction_distribution_is_the_references(e1_lo, normal):
+    """About an axis normal the polar cosine is sqrt(e1), e1 on randNumber's
+    10000-level grid over [e1_lo, 1): its mean and second moment, and the
+    azimuth's uniformity."""
+    n = 200000
+    r = oracle.ph_rand(77, np.arange(n, dtype=np.uint64), np.zeros(n, np.uint32))
+    r2 = oracle.ph_rand(77, np.arange(n, dtype=np.uint64), np.ones(n, np.uint32))
+    d = oracle.photon_dir(normal, e1_lo, r, r2).astype(np.float64)
+    c = d @ np.asarray(normal, np.float64)
+    grid = (1.0 - e1_lo) * (np.arange(10000) / 10000.0) + e1_lo
+    mean, m2 = np.sqrt(grid).mean(), grid.mean()
+    sd = np.sqrt(grid).std()
+    assert abs(c.mean() - mean) < 5 * sd / math.sqrt(n)
+    assert abs((c * c).mean() - m2) < 5 * grid.std() / math.sqrt(n)
+    assert c.min() >= math.sqrt(e1_lo) - 1e-6
+    # azimuth: the two tangent components average to zero
+    tang = d - c[:, None] * np.asarray(normal, np.float64)[None, :]
+    assert np.abs(tang.mean(axis=0)).max() < 5 * 0.5 / math.sqrt(n)
+
+
+def test_light_point_is_uniform_on_the_square():
+    n = 100000
+    r1 = oracle.ph_rand(3, np.arange(n, dtype=np.uint64), np.full(n, 1, np.uint32))
+    r2 = oracle.ph_rand(3, np.arange(n, dtype=np.uint64), np.full(n, 2, np.uint32))
+    p = oracle.light_point(LIGHT, r1, r2).astype(np.float64)
+    half = LIGHT["length"] / 2
+    assert np.all(np.abs(p[:, 0]) <= half) and np.all(np.abs(p[:, 2]) <= half)
+    # the flip for a downward light, then centre, then 0.01 along the normal
+    assert np.all(p[:, 1] == np.float32(np.float32(LIGHT["center"][1]) + np.float32(0.0)) + np.float32(-0.01))
+    sd = 2 * half / math.sqrt(12)
+    assert abs(p[:, 0].mean()) < 5 * sd / math.sqrt(n) and abs(p[:, 2].mean()) < 5 * sd / math.sqrt(n)
+    assert abs(p[:, 0].std() - sd) < 0.01 * sd
+    # draw 0 maps to the far corner after the flip
+    q = oracle.light_point(LIGHT, [0], [0])[0]
+    assert q[0] == np.float32(half) and q[2] == np.float32(half)
+
+
+GLASS = scenes.GLASS
+
+
+def test_transmit_ray_closed_forms():
+    """TransmitRay (PhotonMapperCPU.cpp:800-835): at normal incidence the ray
+    goes straight on; at incidence angle i on a surface of index n, entering
+    from air (0.99), the direction is proportional to (cr sin i, -2 cr cos i
+    - cos t) with cr = n / 0.99 and sin t = cr sin i (the source's normal term
+    has cr*Dot(N, d) where Snell's law has -cr*Dot(N, d)); total internal
+    reflection gives NaN; transparency below 0.6 does not refract."""
+    o, t = oracle.transmit_ray([0, -1, 0], [0.5, 0, 0.25], [0, 1, 0], True, GLASS)
+    assert np.array_equal(t, np.float32([0, -1, 0]))
+    assert np.allclose(o, [0.5, -0.01, 0.25], atol=1e-7)
+    for deg in (10.0, 30.0, 55.0, 80.0):
+        i = math.radians(deg)
+        d = np.array([math.sin(i), -math.cos(i), 0.0], np.float32)
+        _, t = oracle.transmit_ray(d, [0, 0, 0], [0, 1, 0], True, GLASS)
+        cr = 0.5 / 0.99
+        ct = math.sqrt(1 - cr * cr * math.sin(i) ** 2)
+        w = np.array([cr * math.sin(i), -2 * cr * math.cos(i) - ct, 0.0])
+        assert np.allclose(t, w / np.linalg.norm(w), atol=2e-6), deg
+    # leaving the glass (index 0.5 -> air 0.99): cr = 1.98 > 1, grazing -> NaN
+    d = np.array([math.sin(math.radians(70)), math.cos(math.radians(70)), 0.0], np.float32)
+    _, t = oracle.transmit_ray(d, [0, 0, 0], [0, 1, 0], False, GLASS)
+    assert np.isnan(t).all()
+    assert oracle.transmit_ray([0, -1, 0], [0, 0, 0], [0, 1, 0], True, dict(GLASS, transparency=0.59)) is None
+
+
+def _floor(y=0.0, half=1000.0):
+    xyz = np.array([[-half, y, -half], [-half, y, half], [half, y, half], [half, y, -half]], np.float32)
+    return xyz, np.array([[0, 1, 2], [2, 3, 0]], np.uint32)
+
+
+def test_emission_black_floor_closed_form():
+    """A black floor under a downward light: every photon is absorbed at its
+    first hit and stored in the direct map at the intersection of its
+    emission ray with the floor; nothing lies below the floor, so there are
+    no shadow photons; the global map never fills and emission stops at the
+    path bound (where the reference would loop for ever)."""
+    xyz, idx = _floor(half=16.0)  # directions keep cos >= 0.1: every photon lands within 10
+    kd = oracle.KdTree(xyz, idx)
+    light = dict(center=(0.0, 1.0, 0.0), normal=(0.0, -1.0, 0.0), length=0.5, width=0.25, power=(2.0, 3.0, 4.0))
+    maps, paths = oracle.emit_photons(kd, [dict(diffuse=(0, 0, 0))], np.zeros(2, np.uint32), [light],
+                                      (10, 300, 0), seed=11, max_paths=200)
+    assert paths == 200
+    d = maps["direct"]
+    assert len(d["pos"]) == 200 and len(maps["global"]["pos"]) == 0
+    assert not d["shadow"].any()
+    assert np.array_equal(d["path"], np.arange(200))
+    assert np.all(d["power"] == np.float32([2, 3, 4]) * np.float32(0.05))
+    g = np.arange(200, dtype=np.uint64)
+    o = oracle.light_point(light, oracle.ph_rand(11, g, np.full(200, 1, np.uint32)),
+                           oracle.ph_rand(11, g, np.full(200, 2, np.uint32))).astype(np.float64)
+    dirs = np.stack([oracle.photon_dir(light["normal"], 0.01, oracle.ph_rand(11, [p], [3]),
+                                       oracle.ph_rand(11, [p], [4]))[0] for p in range(200)]).astype(np.float64)
+    assert np.array_equal(d["dir"], dirs.astype(np.float32))
+    start = o + 0.01 * dirs
+    t = -start[:, 1] / dirs[:, 1]
+    want = start + t[:, None] * dirs
+    assert np.allclose(d["pos"], want, atol=1e-5)  # float32 hit point v0 + e1 a + e2 b on a 32-wide quad
+    assert np.all(d["normal"] == np.float32([0, 1, 0]))
+
+
+def _closed_box(half=1.0):
+    """A cube with every face's normal pointing inside."""
+    h = half
+    walls = [
+        [[-h, -h, -h], [-h, -h, h], [h, -h, h], [h, -h, -h]],  # floor +y
+        [[-h, h, -h], [h, h, -h], [h, h, h], [-h, h, h]],      # ceiling -y
+        [[-h, -h, -h], [h, -h, -h], [h, h, -h], [-h, h, -h]],  # back +z
+        [[-h, -h, h], [-h, h, h], [h, h, h], [h, -h, h]],      # front -z
+        [[-h, -h, -h], [-h, h, -h], [-h, h, h], [-h, -h, h]],  # left +x
+        [[h, -h, -h], [h, -h, h], [h, h, h], [h, h, -h]],      # right -x
+    ]
+    xyz = np.concatenate([np.array(w, np.float32) for w in walls])
+    idx = np.concatenate([np.array([[0, 1, 2], [2, 3, 0]], np.uint32) + 4 * i for i in range(6)])
+    return xyz, idx
+
+
+def test_closed_box_normals_face_inside():
+    xyz, idx = _closed_box()
+    n = scenes.normals_of(xyz, idx, np.arange(len(idx)))
+    c = xyz[idx].mean(axis=1)
+    assert np.all((n * c).sum(axis=1) < 0)
+
+
+def closed_box_expectation(maps, paths, cap):
+    """Checks of a closed white box (mean reflectance exactly 1): every photon
+    reflects at every hit, so each path stores one direct photon and then one
+    global photon per bounce (3), except that a reflection from within 0.001
+    of an edge starts its next ray 0.001 beyond the adjacent face, outside the
+    box, and the path ends there.  Shadow rays start outside and find nothing.
+    The global map fills on the path that brings it to `cap`, which ends the
+    emission; that path's further photons are dropped."""
+    g, d = maps["global"], maps["direct"]
+    assert len(g["pos"]) == cap and len(d["pos"]) == paths
+    assert np.array_equal(d["path"], np.arange(paths)) and not d["shadow"].any()
+    per = np.bincount(g["path"], minlength=paths)
+    assert per[-1] >= 1 and per[:-1].sum() < cap  # the last path is the one that fills the map
+    assert np.all(np.diff(g["path"].astype(np.int64)) >= 0)  # insertion in emission order
+    short = np.nonzero(per[:-1] < 3)[0]
+    assert len(short) <= 0.02 * paths
+    for p in short:  # the path's last stored photon lies within 0.001 of a second face
+        sel = g["path"] == p
+        last = g["pos"][sel][-1] if sel.any() else d["pos"][p]
+        assert (np.abs(last) >= 0.9989).sum() >= 2, (p, last)
+    assert np.all(np.abs(g["pos"]) <= 1.0 + 1e-6)
+    assert np.all(np.isclose(np.abs(g["pos"]).max(axis=1), 1.0, atol=1e-5))
+
+
+def test_emission_white_closed_box_counts():
+    xyz, idx = _closed_box()
+    kd = oracle.KdTree(xyz, idx)
+    light = dict(center=(0.0, 0.9, 0.0), normal=(0.0, -1.0, 0.0), length=0.5, width=0.5)
+    maps, paths = oracle.emit_photons(kd, [dict(diffuse=(1, 1, 1))], np.zeros(12, np.uint32), [light],
+                                      (1000, 5000, 0), seed=5)
+    assert 334 <= paths <= 345
+    closed_box_expectation(maps, paths, 1000)
+
+
+def test_emission_caustic_map_that_cannot_fill_is_bounded():
+    """With room in the caustic map and nothing that refracts, EmitPhotons'
+    second loop never ends in the reference; here it stops at max_paths."""
+    xyz, idx = _closed_box()
+    kd = oracle.KdTree(xyz, idx)
+    light = dict(center=(0.0, 0.9, 0.0), normal=(0.0, -1.0, 0.0), length=0.5, width=0.5)
+    maps, paths = oracle.emit_photons(kd, [dict(diffuse=(1, 1, 1))], np.zeros(12, np.uint32), [light],
+                                      (30, 5000, 10), seed=5, max_paths=500)
+    assert paths == 500 and len(maps["caustic"]["pos"]) == 0
+    assert len(maps["global"]["pos"]) == 30 and len(maps["direct"]["pos"]) == 500
+
+
+def test_emission_glass_scene_fills_all_maps():
+    """fgCompCornell's refracting outer shell: photons that leave the box
+    refract into the shell and back, and their next stored photon goes to
+    the caustic map.  Every map fills; caustic photons lie on or beyond the
+    inner box's open front or on the shell."""
+    xyz, idx, mats, pm, lights = scenes.photon_scene(tess=4, glass=True)
+    kd = oracle.KdTree(xyz, idx)
+    maps, paths = oracle.emit_photons(kd, mats, pm, lights, (3000, 6000, 300), seed=9)
+    assert [len(maps[m]["pos"]) for m in oracle.MAPS] == [3000, 6000, 300]
+    c = maps["caustic"]
+    assert np.all(c["power"] == np.float32(0.00005)) and not c["shadow"].any()
+    s = maps["direct"]["shadow"].astype(bool)
+    assert s.any() and np.all(maps["direct"]["power"][s] == np.float32(-1.0) * np.float32(0.05))
+    assert paths > 3000
