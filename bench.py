@@ -48,9 +48,13 @@ def parse():
     ap.add_argument("--no-counts", action="store_true")
     ap.add_argument("--no-exact", action="store_true", help="skip the bit-exact kd-path measurement")
     ap.add_argument("--dump", default="", help="write the adjudicated parity differences to this .npz")
-    ap.add_argument("--workload", choices=["c3", "c1", "c2", "c4", "c5", "fg"], default="c3",
+    ap.add_argument("--workload", choices=["c3", "c1", "c2", "c4", "c5", "fg", "pt"], default="c3",
                     help="c3 (default): 8M incoherent rays at 1.31M triangles; c1: Cornell box eye rays; c2: "
-                         "1080p eye + shadow rays at 1.31M triangles; c4: photon gather; c5: serving")
+                         "1080p eye + shadow rays at 1.31M triangles; c4: photon gather; c5: serving; fg: final "
+                         "gather; pt: photon emission")
+    ap.add_argument("--pt-sample", type=int, default=100000,
+                    help="pt: global-map capacity of the CPU baseline and parity sample")
+    ap.add_argument("--seed", type=int, default=7, help="pt: seed of the emission draws")
     ap.add_argument("--requests", type=int, default=160, help="c5: requests per arrival-rate point")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run one device-to-device copy of this many bytes first (counter calibration)")
@@ -552,6 +556,70 @@ def _final_gather_workload(args, dev, thr, trace_fn):
     return 0
 
 
+def _photon_emission_workload(args, dev, thr):
+    """Photon emission (PhotonMapperCPU::EmitPhotons / TracePhoton,
+    PhotonMapperCPU.cpp:503-775) into the global (1,000,000), direct
+    (2,000,000) and caustic (0) maps, 3 bounces, from ENCORE's rectangle light
+    in the Cornell box at 1/20 scale tessellated to 34,816 triangles with the
+    scene file's materials.  One step = one complete emission."""
+    import numpy as np
+
+    import oracle
+    from encore_amd import rq, scenes
+
+    xyz, idx, mats, pm, lights = scenes.photon_scene(tess=32)
+    cap = (1000000, 2000000, 0)
+    out, stored = {}, {}
+    for accel in ("bvh", "kd"):
+        sc = rq.Scene(dev, xyz, idx, accel=accel)
+        sc.set_materials(mats, pm)
+        res = {}
+
+        def fn(sc=sc, res=res):
+            maps, paths = sc.emit_photons(lights, cap, seed=args.seed, extras=False)
+            res["paths"] = paths
+            res["counts"] = [int(maps[m]["pos"].shape[0]) for m in rq.MAPS]
+
+        steps = args.steps if accel == "bvh" else 1
+        wall, ev = _time_gpu(fn, steps, args.warmup if accel == "bvh" else 1)
+        n = sum(res["counts"])
+        out[accel] = {"Mphotons_s": round(n / wall / 1e6, 3), "Mpaths_s": round(res["paths"] / wall / 1e6, 3),
+                      "ms": round(wall * 1e3, 3), "paths": res["paths"], "counts": res["counts"]}
+        stored[accel] = sc
+        log("emission %s: %s" % (accel, out[accel]))
+    # CPU baseline (the oracle's sequential emission, one thread as the
+    # reference's loop) and bit-exact parity of the kd scene on a smaller map.
+    kd = oracle.KdTree(xyz, idx)
+    scap = (args.pt_sample, 2 * args.pt_sample, 0)
+    t = time.perf_counter()
+    o, o_paths = oracle.emit_photons(kd, mats, pm, lights, scap, seed=args.seed)
+    cpu_s = time.perf_counter() - t
+    n_cpu = sum(len(o[m]["pos"]) for m in oracle.MAPS)
+    parity = {"sample_global_capacity": args.pt_sample}
+    for accel in ("kd", "bvh"):
+        g, paths = stored[accel].emit_photons(lights, scap, seed=args.seed)
+        same = 0
+        for m in oracle.MAPS:
+            gm = {k: v.cpu().numpy() for k, v in g[m].items()}
+            if len(gm["pos"]) == len(o[m]["pos"]):
+                same += int((np.all(gm["pos"].view(np.uint32) == o[m]["pos"].view(np.uint32), axis=1) &
+                             np.all(gm["normal"].view(np.uint32) == o[m]["normal"].view(np.uint32), axis=1) &
+                             (gm["path"].astype(np.uint32) == o[m]["path"])).sum())
+        parity[accel] = {"paths_equal": paths == o_paths, "photons_equal": same, "photons": n_cpu}
+    log("parity", parity)
+    _line("photon-emission Mphotons/s (3 bounces, global 1M + direct maps)", out["bvh"]["Mphotons_s"], "Mphotons/s",
+          out["bvh"]["ms"],
+          {"workload": "photon emission from the 1.5 x 1.5 ceiling light of the Cornell box at 1/20 scale "
+                       "tessellated to %d triangles until the global map holds %d photons (direct map %d), "
+                       "3 bounces" % (len(idx), cap[0], cap[1]), "accel": "bvh4"},
+          paths=out, parity=parity,
+          cpu_baseline={"value": round(n_cpu / cpu_s / 1e6, 5), "unit": "Mphotons/s", "cores": 1, "kind": "port",
+                        "sample": "emission until the global map holds %d photons (%d paths)" % (scap[0], o_paths)})
+    for sc in stored.values():
+        sc.destroy()
+    return 0
+
+
 def run_workload(args):
     import numpy as np
     import torch
@@ -662,6 +730,8 @@ def run_workload(args):
 
     if args.workload == "fg":
         return _final_gather_workload(args, dev, thr, trace_fn)
+    if args.workload == "pt":
+        return _photon_emission_workload(args, dev, thr)
     if args.workload == "c4":
         xyz, idx = scenes.config4_scene()
         sc = rq.Scene(dev, xyz, idx, accel="bvh")

@@ -286,11 +286,14 @@ __device__ void shade_one(const PhLaunch& L, uint32_t i, Emit& e) {
     const bool entering = dot3(d, N) < 0.0f;  // TrianglePrim.cpp:242
     if (kind == kKindShadow) {
         // EmitShadowPhoton: pass through surfaces being left, store at the
-        // first one entered.
-        if (!entering)
-            e.push(make_ray(add(P, mul(d, 0.01f)), d), tag);
-        else
+        // first one entered.  The bounce field counts the surfaces passed; the
+        // chain gives up at kPhShadowPasses, where a phantom hit behind the
+        // ray (Cross()'s flush) would keep the reference's loop going for ever.
+        if (!entering) {
+            if (b + 1 < kPhShadowPasses) e.push(make_ray(add(P, mul(d, 0.01f)), d), make_tag(lp, kKindShadow, b + 1, 0));
+        } else {
             store_event(L, lp, 1, kTagDirect | kTagShadow, P, N);
+        }
         return;
     }
     const PhMaterial m = L.mats[L.prim_mat[h.prim]];

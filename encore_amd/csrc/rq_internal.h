@@ -298,6 +298,7 @@ struct PhLight {
 // direction.  Event slots per path: 0 the first stored photon, 1 its shadow
 // photon, b+1 the photon stored at bounce b >= 1.
 constexpr uint32_t kPhMaxBounces = 16;
+constexpr uint32_t kPhShadowPasses = 32;  // surfaces a shadow-photon ray may leave before it gives up
 constexpr uint32_t kPhPathBits = 22;  // live-ray tag: local path in bits 0..21
 constexpr uint32_t kPhMaxBatch = 1u << kPhPathBits;
 constexpr uint32_t kPhScanBlock = 256;

@@ -1065,6 +1065,8 @@ struct OMap {
     }
 };
 
+constexpr uint32_t kShadowPasses = 32;
+
 struct OEmit {
     const KdTree* T;
     const OMat* mats;
@@ -1124,9 +1126,18 @@ void trace_path(const OEmit& E, uint64_t gp) {
             } else if (b == 0 && !E.maps[1].full()) {
                 E.maps[1].add(h.P, h.N, l.power, false, dir0, false, gp);
                 // EmitShadowPhoton (PhotonMapperCPU.cpp:775-797)
+                // The chain gives up after kShadowPasses surfaces left: a
+                // phantom hit whose hit point lies behind the ray (Cross()'s
+                // flush) returns the same point for ever, and the reference's
+                // loop never ends there.
                 F3 so = add3(h.P, mul3(rd, 2.0f));
                 OHit sh = intersect(*E.T, so, rd);
+                uint32_t passes = 0;
                 while (sh.has && !sh.entering) {
+                    if (++passes == kShadowPasses) {
+                        sh.has = false;
+                        break;
+                    }
                     so = add3(sh.P, mul3(rd, 0.01f));
                     sh = intersect(*E.T, so, rd);
                 }

@@ -155,3 +155,18 @@ def test_emission_rejects_bad_arguments(device):
     maps, paths = sc.emit_photons([light], (0, 10, 0))  # both loops end at once
     assert paths == 0 and all(len(maps[m]["pos"]) == 0 for m in maps)
     sc.destroy()
+
+
+def test_emission_shadow_chain_bound_matches_oracle(device):
+    """The scene and seed where shadow-photon rays loop on phantom hits: the
+    bounded chain gives the oracle's maps bit for bit."""
+    xyz, idx, mats, pm, lights = scenes.photon_scene(tess=32)
+    kd = oracle.KdTree(xyz, idx)
+    o, o_paths = oracle.emit_photons(kd, mats, pm, lights, (20000, 40000, 0), seed=7)
+    log("case: photon emission shadow-chain bound (tess 32)")
+    sc = rq.Scene(device, xyz, idx, accel="kd")
+    sc.set_materials(mats, pm)
+    g, paths = _gpu(sc, lights, (20000, 40000, 0), seed=7)
+    assert paths == o_paths
+    _same(g, o)
+    sc.destroy()

@@ -304,3 +304,14 @@ def test_emission_glass_scene_fills_all_maps():
     s = maps["direct"]["shadow"].astype(bool)
     assert s.any() and np.all(maps["direct"]["power"][s] == np.float32(-1.0) * np.float32(0.05))
     assert paths > 3000
+
+
+def test_emission_shadow_chain_is_bounded():
+    """On the 34,816-triangle box a few shadow-photon rays meet a phantom hit
+    of the small box whose hit point lies behind the ray, so EmitShadowPhoton
+    would return to it for ever (the first at path 12437 with seed 7).  The
+    chain gives up after 32 surfaces; the emission completes."""
+    xyz, idx, mats, pm, lights = scenes.photon_scene(tess=32)
+    kd = oracle.KdTree(xyz, idx)
+    maps, paths = oracle.emit_photons(kd, mats, pm, lights, (20000, 40000, 0), seed=7)
+    assert paths > 12437 and len(maps["global"]["pos"]) == 20000
