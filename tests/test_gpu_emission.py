@@ -9,7 +9,7 @@ import torch
 
 import oracle
 from encore_amd import rq, scenes
-from helpers import log
+from helpers import log, outside_box
 from test_oracle_emission import _closed_box, closed_box_expectation
 
 pytestmark = pytest.mark.gpu
@@ -68,10 +68,54 @@ def test_emission_is_independent_of_batch_size(device, pscene):
     sc.destroy()
 
 
+def _keys(x, shadow):
+    sel = x["shadow"].astype(bool) == shadow
+    return set(zip(x["path"][sel].astype(np.int64).tolist(), map(bytes, x["pos"][sel].view(np.uint32)),
+                   map(bytes, x["normal"][sel].view(np.uint32))))
+
+
+def _shadow_ray_ties(f, maps, paths):
+    """For each path, whether its shadow-photon chain (from the direct
+    photon's hit point, 2 along the photon's direction, through surfaces
+    being left) meets, at some step, two or more triangles at its closest
+    distance (a tie: a box standing on the floor has its bottom face in the
+    floor's plane) or a reference phantom outside its triangle's box."""
+    d = maps["direct"]
+    first = {}
+    for i in np.nonzero(d["shadow"] == 0)[0]:
+        first.setdefault(int(d["path"][i]), i)
+    rows = [first[p] for p in paths]
+    rays = np.zeros((len(rows), 8), np.float32)
+    rays[:, 0:3] = d["pos"][rows] + d["dir"][rows] * np.float32(2.0)
+    rays[:, 4:7] = d["dir"][rows]
+    rays[:, 7] = np.inf
+    tie = np.zeros(len(rows), bool)
+    live = np.arange(len(rows))
+    for _ in range(32):
+        if not len(live):
+            break
+        h = f["kd"].intersect(rays[live])
+        hit = h[:, 3].view(np.uint32) != oracle.NO_HIT
+        live, h = live[hit], h[hit]
+        tie[live] |= oracle.count_hits_below(f["xyz"], f["idx"], rays[live], h[:, 0], 0.0) >= 2
+        # a reference phantom outside its triangle's box, which no BVH reaches
+        prim = h[:, 3].view(np.uint32)
+        tie[live] |= np.array([outside_box(f["xyz"], f["idx"], rays[r], h[k, 0], prim[k]) > 0
+                               for k, r in enumerate(live)], bool)
+        _, P, N = oracle.hit_attributes(f["xyz"], f["idx"], h)
+        leaving = (rays[live, 4:7] * N).sum(axis=1) >= 0
+        live, P = live[leaving], P[leaving]
+        rays[live, 0:3] = P + rays[live, 4:7] * np.float32(0.01)
+    return tie
+
+
 @pytest.mark.parametrize("width", [4, 2])
 def test_emission_bvh_close_to_oracle(device, pscene, width):
-    """Photons are compared by (path, position, normal): at least 99% of the
-    oracle's photons of each map appear identically."""
+    """Photons are compared by (path, position, normal).  At least 99% of
+    the oracle's photons appear identically; every shadow photon found by
+    only one of the two comes from a shadow ray whose closest hit is a tie
+    between coplanar triangles (the BVH may return the surface being left,
+    so the ray goes on and finds nothing)."""
     f = pscene
     log("case: photon emission bvh%d, glass=%s" % (width, f["glass"]))
     sc = rq.Scene(device, f["xyz"], f["idx"], accel="bvh", width=width)
@@ -80,16 +124,18 @@ def test_emission_bvh_close_to_oracle(device, pscene, width):
     o, o_paths = f["o"]
     assert abs(paths - o_paths) <= 0.01 * o_paths
     for m in oracle.MAPS:
-        assert len(g[m]["pos"]) == len(o[m]["pos"])
         if not len(o[m]["pos"]):
+            assert not len(g[m]["pos"])
             continue
-
-        def keys(x):
-            return set(zip(x["path"].tolist(), map(bytes, x["pos"].view(np.uint32)),
-                           map(bytes, x["normal"].view(np.uint32))))
-        common = len(keys(g[m]) & keys(o[m]))
-        log("bvh%d %s: %d of %d photons identical" % (width, m, common, len(o[m]["pos"])))
-        assert common >= 0.99 * len(o[m]["pos"])
+        common = len(_keys(g[m], False) & _keys(o[m], False))
+        n = int((o[m]["shadow"] == 0).sum())
+        log("bvh%d %s: %d of %d photons identical" % (width, m, common, n))
+        assert common >= 0.99 * n
+    gs, os_ = _keys(g["direct"], True), _keys(o["direct"], True)
+    only = sorted({k[0] for k in gs ^ os_})
+    log("bvh%d shadow photons: %d gpu, %d oracle, %d paths differ" % (width, len(gs), len(os_), len(only)))
+    if only:
+        assert _shadow_ray_ties(f, o, only).all()
     sc.destroy()
 
 
