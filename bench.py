@@ -596,16 +596,20 @@ def _photon_emission_workload(args, dev, thr):
     cpu_s = time.perf_counter() - t
     n_cpu = sum(len(o[m]["pos"]) for m in oracle.MAPS)
     parity = {"sample_global_capacity": args.pt_sample}
+    def keyed(x):  # photons by (path, position bits, normal bits, shadow)
+        return set(zip(x["path"].astype(np.int64).tolist(), map(bytes, x["pos"].view(np.uint32)),
+                       map(bytes, x["normal"].view(np.uint32)), x["shadow"].tolist()))
+
     for accel in ("kd", "bvh"):
         g, paths = stored[accel].emit_photons(lights, scap, seed=args.seed)
-        same = 0
+        same, exact = 0, True
         for m in oracle.MAPS:
             gm = {k: v.cpu().numpy() for k, v in g[m].items()}
-            if len(gm["pos"]) == len(o[m]["pos"]):
-                same += int((np.all(gm["pos"].view(np.uint32) == o[m]["pos"].view(np.uint32), axis=1) &
-                             np.all(gm["normal"].view(np.uint32) == o[m]["normal"].view(np.uint32), axis=1) &
-                             (gm["path"].astype(np.uint32) == o[m]["path"])).sum())
-        parity[accel] = {"paths_equal": paths == o_paths, "photons_equal": same, "photons": n_cpu}
+            same += len(keyed(gm) & keyed(o[m]))
+            exact = exact and len(gm["pos"]) == len(o[m]["pos"]) and all(
+                np.array_equal(gm[k].view(np.uint8), o[m][k].astype(gm[k].dtype).view(np.uint8)) for k in gm)
+        parity[accel] = {"paths": paths, "oracle_paths": o_paths, "photons_identical": same, "photons": n_cpu,
+                         "maps_bit_equal": bool(exact)}
     log("parity", parity)
     _line("photon-emission Mphotons/s (3 bounces, global 1M + direct maps)", out["bvh"]["Mphotons_s"], "Mphotons/s",
           out["bvh"]["ms"],

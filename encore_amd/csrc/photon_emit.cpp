@@ -76,9 +76,8 @@ void set_materials(rq_scene* sc, const rq_material* mats, uint32_t nmats, const 
     sc->nmaterials = nmats;
 }
 
-// Workspace of one batch size.
+// Workspace of one batch size, carved from the device's emission buffer.
 struct PhWork {
-    DevBuf buf;
     rq_ray *path_ray = nullptr, *rays[2] = {nullptr, nullptr};
     float *path_dir = nullptr, *ev_pos = nullptr, *ev_nrm = nullptr, *phi = nullptr;
     uint32_t *path_light = nullptr, *tags[2] = {nullptr, nullptr}, *boff = nullptr, *ctl = nullptr;
@@ -87,7 +86,7 @@ struct PhWork {
     rq_hit* hits = nullptr;
     PhLight* lights = nullptr;
 
-    PhWork(uint32_t bmax, uint32_t nslots, uint32_t nlights) {
+    PhWork(uint32_t bmax, uint32_t nslots, uint32_t nlights, DevBuf& buf) {
         const size_t B = bmax, Q = 2 * (size_t)bmax, S = B * nslots;
         const size_t nblocks = (B + kPhScanBlock - 1) / kPhScanBlock;
         size_t off = 0;
@@ -102,7 +101,10 @@ struct PhWork {
                      o_t0 = take(Q * 4), o_t1 = take(Q * 4), o_boff = take(nblocks * 12), o_ctl = take(64),
                      o_evt = take(S), o_cnt = take(B * 8), o_hits = take(Q * sizeof(rq_hit)),
                      o_lights = take((size_t)nlights * sizeof(PhLight));
-        buf = DevBuf(off);
+        if (buf.bytes < off) {
+            buf = DevBuf();  // release the old buffer before taking the larger one
+            buf = DevBuf(off);
+        }
         char* b = static_cast<char*>(buf.ptr);
         path_ray = reinterpret_cast<rq_ray*>(b + o_pray);
         rays[0] = reinterpret_cast<rq_ray*>(b + o_r0);
@@ -170,7 +172,14 @@ void emit(rq_scene* sc, const rq_rect_light* lights, uint32_t nlights, const rq_
     const uint32_t bmax = std::max<uint32_t>(1024, std::min<uint32_t>(P.batch_paths ? P.batch_paths : 1u << 20,
                                                                       kPhMaxBatch));
     RQ_HIP(hipSetDevice(sc->dev->ordinal));
-    PhWork w(bmax, nslots, nlights);
+    std::lock_guard<std::mutex> hold(sc->dev->emit_mu);
+    // Every call leaves the workspace idle: it ends synchronised with its
+    // stream, also when it fails part way.
+    struct Drain {
+        hipStream_t s;
+        ~Drain() { (void)hipStreamSynchronize(s); }
+    } drain{s};
+    PhWork w(bmax, nslots, nlights, sc->dev->emit_work);
     const std::vector<float>& phi = ph_phi_table();
     RQ_HIP(hipMemcpyAsync(w.phi, phi.data(), phi.size() * 4, hipMemcpyHostToDevice, s));
     RQ_HIP(hipMemcpyAsync(w.lights, hl.data(), hl.size() * sizeof(PhLight), hipMemcpyHostToDevice, s));

@@ -391,6 +391,10 @@ struct rq_device {
     // stacks after it.  Launches on one stream are ordered, so they can share.
     std::map<hipStream_t, rq::DevBuf> scratch;
     rq::KernelConfig kc;
+    // Photon-emission workspace, kept between calls (grown on demand);
+    // emissions on one device take turns on it.
+    std::mutex emit_mu;
+    rq::DevBuf emit_work;
 };
 
 struct rq_photons {
