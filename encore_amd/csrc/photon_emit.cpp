@@ -211,7 +211,17 @@ void emit(rq_scene* sc, const rq_rect_light* lights, uint32_t nlights, const rq_
                          out ? out[m].power : nullptr,  out ? out[m].dir : nullptr,
                          out ? out[m].shadow : nullptr, out ? out[m].path : nullptr};
     }
-    uint32_t* counter = w.ctl + 8;  // append counter of the ray queues
+    // Live-ray counts of the two queues.  Iterations are queued in groups
+    // without waiting: the traversal and the shading read the count on the
+    // device, and an iteration over an empty queue does nothing.
+    uint32_t* count[2] = {w.ctl + 8, w.ctl + 9};
+    const uint32_t qmax = 2 * bmax;
+    const int shade_blocks = 4 * sc->dev->num_cus;
+    constexpr int kGroup = 4;
+    // Every live ray advances a path by a bounce, a refraction step or a
+    // shadow-photon pass, so a batch drains within this many iterations.
+    const uint32_t max_iters = 2 * (P.max_bounces + 1) + kPhShadowPasses + 2 * kGroup;
+    L.stride = bmax;
     // Photons per path seen so far, to size the next batch.
     double seen_paths = 0, seen[3] = {0, 0, 0};
     uint64_t paths = 0;
@@ -231,26 +241,28 @@ void emit(rq_scene* sc, const rq_rect_light* lights, uint32_t nlights, const rq_
         L.nblocks = (uint32_t)((np + kPhScanBlock - 1) / kPhScanBlock);
         // Emission, then the wavefront until no ray is left.
         int cur = 0;
-        launch_zero_u32(counter, 1, s);
+        launch_zero_u32(count[cur], 1, s);
         L.out_rays = w.rays[cur];
         L.out_tags = w.tags[cur];
-        L.out_count = counter;
+        L.out_count = count[cur];
         launch_ph_emit(L, s);
-        uint32_t nlive = read_u32(counter, s);
-        uint32_t steps = 0;
-        while (nlive) {
-            if (++steps > 100000) throw Error(RQ_ERR_DEVICE, "photon tracing did not converge");
-            trace_closest(sc, w.rays[cur], nlive, w.hits, s);
-            launch_zero_u32(counter, 1, s);
-            L.in_rays = w.rays[cur];
-            L.in_tags = w.tags[cur];
-            L.hits = w.hits;
-            L.nin = nlive;
-            L.out_rays = w.rays[1 - cur];
-            L.out_tags = w.tags[1 - cur];
-            launch_ph_shade(L, s);
-            cur = 1 - cur;
-            nlive = read_u32(counter, s);
+        for (uint32_t iters = 0;;) {
+            for (int g = 0; g < kGroup; ++g, ++iters) {
+                trace_closest(sc, w.rays[cur], qmax, w.hits, s, count[cur]);
+                launch_zero_u32(count[1 - cur], 1, s);
+                L.in_rays = w.rays[cur];
+                L.in_tags = w.tags[cur];
+                L.hits = w.hits;
+                L.nin_dev = count[cur];
+                L.nin_max = qmax;
+                L.out_rays = w.rays[1 - cur];
+                L.out_tags = w.tags[1 - cur];
+                L.out_count = count[1 - cur];
+                launch_ph_shade(L, shade_blocks, s);
+                cur = 1 - cur;
+            }
+            if (read_u32(count[cur], s) == 0) break;
+            if (iters >= max_iters) throw Error(RQ_ERR_DEVICE, "photon tracing did not drain");
         }
         // Fill the maps in emission order.
         for (int m = 0; m < 3; ++m) {

@@ -23,6 +23,8 @@
 // axis left unnormalised as the reference leaves it.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "rq_internal.h"
 
 namespace rq {
@@ -202,20 +204,33 @@ struct Emit {
     }
 };
 
-// Appends every lane's new rays with one atomic per wave.  All 64 lanes of
-// the wave must call it.
-__device__ __forceinline__ void wave_append(const PhLaunch& L, const Emit& e) {
-    const uint32_t lane = __lane_id();
+constexpr uint32_t kPhWaves = kPhBlock / 64;
+
+// Appends every thread's new rays to the output queue with one atomic per
+// block (one counter shared by a million rays serialises at the L2 when
+// every wave takes its own).  Every thread of the block must call it; s_w
+// holds kPhWaves + 1 words of LDS.
+__device__ __forceinline__ void block_append(const PhLaunch& L, const Emit& e, uint32_t* s_w) {
+    const uint32_t lane = __lane_id(), wid = threadIdx.x / 64;
     uint32_t incl = e.n;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
         const uint32_t v = __shfl_up(incl, off, 64);
         if (lane >= (uint32_t)off) incl += v;
     }
-    const uint32_t total = __shfl(incl, 63, 64);
-    uint32_t base = 0;
-    if (lane == 63 && total) base = atomicAdd(L.out_count, total);
-    base = __shfl(base, 63, 64) + (incl - e.n);
+    if (lane == 63) s_w[wid] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t w = 0; w < kPhWaves; ++w) {
+            const uint32_t t = s_w[w];
+            s_w[w] = run;
+            run += t;
+        }
+        s_w[kPhWaves] = run ? atomicAdd(L.out_count, run) : 0;
+    }
+    __syncthreads();
+    const uint32_t base = s_w[kPhWaves] + s_w[wid] + (incl - e.n);
     if (e.n > 0) {
         L.out_rays[base] = e.r0;
         L.out_tags[base] = e.t0;
@@ -224,10 +239,11 @@ __device__ __forceinline__ void wave_append(const PhLaunch& L, const Emit& e) {
         L.out_rays[base + 1] = e.r1;
         L.out_tags[base + 1] = e.t1;
     }
+    __syncthreads();  // s_w is reused by the next call
 }
 
 __device__ __forceinline__ void store_event(const PhLaunch& L, uint32_t lp, uint32_t slot, uint8_t tag, V3 P, V3 N) {
-    const size_t e = (size_t)lp * L.nslots + slot;
+    const size_t e = (size_t)slot * L.stride + lp;
     L.ev_pos[3 * e] = P.x;
     L.ev_pos[3 * e + 1] = P.y;
     L.ev_pos[3 * e + 2] = P.z;
@@ -237,7 +253,8 @@ __device__ __forceinline__ void store_event(const PhLaunch& L, uint32_t lp, uint
     L.ev_tag[e] = tag;
 }
 
-__global__ void ph_emit(PhLaunch L) {
+__global__ __launch_bounds__(kPhBlock) void ph_emit(PhLaunch L) {
+    __shared__ uint32_t s_w[kPhWaves + 1];
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     Emit e;
     if (p < L.npaths) {
@@ -254,10 +271,10 @@ __global__ void ph_emit(PhLaunch L) {
         L.path_dir[3 * (size_t)p + 1] = d.y;
         L.path_dir[3 * (size_t)p + 2] = d.z;
         L.path_light[p] = li;
-        for (uint32_t s = 0; s < L.nslots; ++s) L.ev_tag[(size_t)p * L.nslots + s] = 0;
+        for (uint32_t s = 0; s < L.nslots; ++s) L.ev_tag[(size_t)s * L.stride + p] = 0;
         if (ray_finite(r)) e.push(r, make_tag(p, kKindPath, 0, 0));
     }
-    wave_append(L, e);
+    block_append(L, e, s_w);
 }
 
 // The photon continues with `r` at bounce nb (TracePhoton's loop condition
@@ -346,11 +363,17 @@ __device__ void shade_one(const PhLaunch& L, uint32_t i, Emit& e) {
     e.push(tr, make_tag(lp, kKindTransmit, b, 1));
 }
 
-__global__ void ph_shade(PhLaunch L) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    Emit e;
-    if (i < L.nin) shade_one(L, i, e);
-    wave_append(L, e);
+// Grid-stride over the live rays, whose count is read on the device, so the
+// host can queue iterations without waiting for each one.
+__global__ __launch_bounds__(kPhBlock) void ph_shade(PhLaunch L) {
+    __shared__ uint32_t s_w[kPhWaves + 1];
+    const uint32_t nin = *L.nin_dev;
+    for (uint32_t base = blockIdx.x * kPhBlock; base < nin; base += gridDim.x * kPhBlock) {
+        Emit e;
+        const uint32_t i = base + threadIdx.x;
+        if (i < nin) shade_one(L, i, e);
+        block_append(L, e, s_w);
+    }
 }
 
 // ----- compaction in emission order ----------------------------------------
@@ -381,7 +404,7 @@ __global__ void ph_count(PhLaunch L) {
     unsigned long long c = 0;
     if (p < L.npaths) {
         for (uint32_t s = 0; s < L.nslots; ++s) {
-            const uint32_t t = L.ev_tag[(size_t)p * L.nslots + s] & 3u;
+            const uint32_t t = L.ev_tag[(size_t)s * L.stride + p] & 3u;
             if (t) c += 1ull << (21 * (t - 1));
         }
         L.cnt[p] = c;
@@ -455,7 +478,7 @@ __global__ void ph_scatter(PhLaunch L) {
         const uint32_t li = L.path_light[p];
         const PhLight& l = L.lights[li];
         for (uint32_t s = 0; s < L.nslots; ++s) {
-            const size_t e = (size_t)p * L.nslots + s;
+            const size_t e = (size_t)s * L.stride + p;
             const uint8_t t = L.ev_tag[e];
             if (!(t & 3u)) continue;
             const uint32_t m = (t & 3u) - 1;
@@ -485,13 +508,14 @@ __global__ void ph_scatter(PhLaunch L) {
 }  // namespace dev
 
 void launch_ph_emit(const PhLaunch& L, hipStream_t s) {
-    hipLaunchKernelGGL(dev::ph_emit, dim3((L.npaths + 255) / 256), dim3(256), 0, s, L);
+    hipLaunchKernelGGL(dev::ph_emit, dim3((L.npaths + kPhBlock - 1) / kPhBlock), dim3(kPhBlock), 0, s, L);
     RQ_HIP(hipGetLastError());
 }
 
-void launch_ph_shade(const PhLaunch& L, hipStream_t s) {
-    if (L.nin == 0) return;
-    hipLaunchKernelGGL(dev::ph_shade, dim3((L.nin + 255) / 256), dim3(256), 0, s, L);
+void launch_ph_shade(const PhLaunch& L, int max_blocks, hipStream_t s) {
+    const uint32_t need = (L.nin_max + kPhBlock - 1) / kPhBlock;
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(need, (uint32_t)max_blocks));
+    hipLaunchKernelGGL(dev::ph_shade, dim3(grid), dim3(kPhBlock), 0, s, L);
     RQ_HIP(hipGetLastError());
 }
 

@@ -95,8 +95,9 @@ static int grid_for(rq_device* dev, uint32_t accel, uint32_t pred, TraceKind kin
     return (int)grid;
 }
 
+// n_dev: the ray count is read on the device from there (n bounds it).
 static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, uint32_t* occ, TraceKind kind,
-                  hipStream_t s, unsigned long long* counts = nullptr) {
+                  hipStream_t s, unsigned long long* counts = nullptr, const uint32_t* n_dev = nullptr) {
     if (n == 0) return;
     rq_device* dev = sc->dev;
     RQ_HIP(hipSetDevice(dev->ordinal));
@@ -112,6 +113,7 @@ static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, ui
     L.hits = hits;
     L.occluded = occ;
     L.n = n;
+    L.n_dev = n_dev;
     L.counter = scratch;
     L.spill = scratch + 64;
     L.spill_depth = words;
@@ -130,8 +132,8 @@ static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, ui
         launch_bvh(L, sc->predicate, kind, sc->width, grid, dev->kc, s);
 }
 
-void trace_closest(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, hipStream_t s) {
-    trace(sc, rays, n, hits, nullptr, TraceKind::Closest, s);
+void trace_closest(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, hipStream_t s, const uint32_t* n_dev) {
+    trace(sc, rays, n, hits, nullptr, TraceKind::Closest, s, nullptr, n_dev);
 }
 
 // Model-list bounds for the kd root: given, or the vertex bounds.

@@ -241,6 +241,7 @@ struct TraceLaunch {
     rq_hit* hits;            // closest
     uint32_t* occluded;      // any hit
     uint32_t n;
+    const uint32_t* n_dev;   // if set, the ray count is read here at kernel start (n is then a bound)
     uint32_t* counter;       // work counter, zeroed before launch
     uint32_t* spill;         // per-lane spill stacks
     uint32_t spill_depth;    // entries per lane
@@ -298,7 +299,8 @@ struct PhLight {
 // direction.  Event slots per path: 0 the first stored photon, 1 its shadow
 // photon, b+1 the photon stored at bounce b >= 1.
 constexpr uint32_t kPhMaxBounces = 16;
-constexpr uint32_t kPhShadowPasses = 32;  // surfaces a shadow-photon ray may leave before it gives up
+constexpr uint32_t kPhShadowPasses = 8;  // surfaces a shadow-photon ray may leave before it gives up
+constexpr uint32_t kPhBlock = 1024;      // threads per block of the emission and shading kernels
 constexpr uint32_t kPhPathBits = 22;  // live-ray tag: local path in bits 0..21
 constexpr uint32_t kPhMaxBatch = 1u << kPhPathBits;
 constexpr uint32_t kPhScanBlock = 256;
@@ -325,13 +327,16 @@ struct PhLaunch {
     rq_ray* path_ray;     // npaths: the photon's current ray
     float* path_dir;      // npaths x 3: emission direction
     uint32_t* path_light; // npaths
-    float* ev_pos;        // npaths x nslots x 3
+    // Events, slot-major: event (slot, path) at slot * stride + path.
+    uint32_t stride;
+    float* ev_pos;        // 3 floats per event
     float* ev_nrm;
-    uint8_t* ev_tag;      // npaths x nslots: map + 1 (0 none), bit 2 shadow
+    uint8_t* ev_tag;      // map + 1 (0 none), bit 2 shadow
     const rq_ray* in_rays;
     const uint32_t* in_tags;
     const rq_hit* hits;
-    uint32_t nin;
+    const uint32_t* nin_dev;  // live rays in the input queue (read on the device)
+    uint32_t nin_max;
     rq_ray* out_rays;
     uint32_t* out_tags;
     uint32_t* out_count;
@@ -344,13 +349,15 @@ struct PhLaunch {
     PhOut out[3];
 };
 void launch_ph_emit(const PhLaunch& L, hipStream_t s);
-void launch_ph_shade(const PhLaunch& L, hipStream_t s);
+void launch_ph_shade(const PhLaunch& L, int max_blocks, hipStream_t s);
 void launch_ph_compact(const PhLaunch& L, hipStream_t s);
 // (cos, sin) of 2 * PI * (k / 10000) for every rand() % 10000 outcome k, in
 // float as Common.cpp:150-156 evaluates them (PI = 3.14159f).
 const std::vector<float>& ph_phi_table();
-// Closest hit of n rays on the scene (the traversal behind rq_intersect).
-void trace_closest(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, hipStream_t s);
+// Closest hit of n rays on the scene (the traversal behind rq_intersect);
+// with n_dev the count is read on the device and n only bounds it.
+void trace_closest(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, hipStream_t s,
+                   const uint32_t* n_dev = nullptr);
 
 // Threads per block: the BVH kernels run one wave per block (their LDS use
 // per wave is large, so single-wave blocks pack a CU best); the kd kernel

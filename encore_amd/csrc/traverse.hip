@@ -395,6 +395,7 @@ __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
     const void* nodes = L.nodes;
     const TriRec* tris = L.tris;
     void* out = ANY ? static_cast<void*>(L.occluded) : static_cast<void*>(L.hits);
+    const uint32_t nrays = L.n_dev ? *L.n_dev : L.n;
     const float kInf = __int_as_float(0x7f800000);
 
     RayState r;
@@ -437,7 +438,7 @@ __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
                 if (idle == 0) break;
                 if (ch_used == ch_n) {
                     if (cl_base >= cl_end) {
-                        const uint32_t left = L.n - min(L.n, cl_end);
+                        const uint32_t left = nrays - min(nrays, cl_end);
                         const uint32_t want =
                             min(2048u, max((uint32_t)kWave, (left / (4u * gridDim.x)) & ~(uint32_t)(kWave - 1)));
                         // 64-bit counter: it runs past n by up to a claim per
@@ -447,12 +448,12 @@ __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
                             b = atomicAdd(reinterpret_cast<unsigned long long*>(L.counter), (unsigned long long)want);
                         b = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(b >> 32)) << 32) |
                             __builtin_amdgcn_readfirstlane((uint32_t)b);
-                        if (b >= L.n) {
+                        if (b >= nrays) {
                             exhausted = true;
                             break;
                         }
                         cl_base = (uint32_t)b;
-                        cl_end = (uint32_t)min((unsigned long long)L.n, b + want);
+                        cl_end = (uint32_t)min((unsigned long long)nrays, b + want);
                     }
                     const uint32_t base = cl_base;
                     ch_base = base;
@@ -736,6 +737,7 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
     const TriRec* tris = L.tris;
     const float kInf = __int_as_float(0x7f800000);
     const uint32_t thresh = L.fetch_threshold;
+    const uint32_t nrays = L.n_dev ? *L.n_dev : L.n;
 
     RayState r;
     uint32_t ray = kNone;
@@ -745,7 +747,7 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
     uint32_t n_nodes = 0, n_tris = 0;  // COUNT only
 
     for (;;) {
-        ray = fetch_work(ray, L.counter, L.n);
+        ray = fetch_work(ray, L.counter, nrays);
         if (ray != kNone && ray != kDone && !active) {
             float rtmin;
             init_ray(r, L.rays, ray, rtmin, rtmax);

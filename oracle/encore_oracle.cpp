@@ -1065,7 +1065,7 @@ struct OMap {
     }
 };
 
-constexpr uint32_t kShadowPasses = 32;
+constexpr uint32_t kShadowPasses = 8;
 
 struct OEmit {
     const KdTree* T;

@@ -310,7 +310,7 @@ def test_emission_shadow_chain_is_bounded():
     """On the 34,816-triangle box a few shadow-photon rays meet a phantom hit
     of the small box whose hit point lies behind the ray, so EmitShadowPhoton
     would return to it for ever (the first at path 12437 with seed 7).  The
-    chain gives up after 32 surfaces; the emission completes."""
+    chain gives up after 8 surfaces; the emission completes."""
     xyz, idx, mats, pm, lights = scenes.photon_scene(tess=32)
     kd = oracle.KdTree(xyz, idx)
     maps, paths = oracle.emit_photons(kd, mats, pm, lights, (20000, 40000, 0), seed=7)
