@@ -246,22 +246,30 @@ void emit(rq_scene* sc, const rq_rect_light* lights, uint32_t nlights, const rq_
         L.out_tags = w.tags[cur];
         L.out_count = count[cur];
         launch_ph_emit(L, s);
+        // Bound on the live rays, for launch sizes: a path has at most two
+        // rays alive (its own and a shadow-photon ray), and an iteration at
+        // most doubles the live count.
+        const uint64_t per_batch = std::min<uint64_t>(qmax, 2 * np);
+        uint64_t bound = np;
         for (uint32_t iters = 0;;) {
             for (int g = 0; g < kGroup; ++g, ++iters) {
-                trace_closest(sc, w.rays[cur], qmax, w.hits, s, count[cur]);
+                const uint32_t nb = (uint32_t)std::min<uint64_t>(bound, per_batch);
+                trace_closest(sc, w.rays[cur], nb, w.hits, s, count[cur]);
                 launch_zero_u32(count[1 - cur], 1, s);
                 L.in_rays = w.rays[cur];
                 L.in_tags = w.tags[cur];
                 L.hits = w.hits;
                 L.nin_dev = count[cur];
-                L.nin_max = qmax;
+                L.nin_max = nb;
                 L.out_rays = w.rays[1 - cur];
                 L.out_tags = w.tags[1 - cur];
                 L.out_count = count[1 - cur];
                 launch_ph_shade(L, shade_blocks, s);
                 cur = 1 - cur;
+                bound = std::min<uint64_t>(2 * bound, per_batch);
             }
-            if (read_u32(count[cur], s) == 0) break;
+            bound = read_u32(count[cur], s);
+            if (bound == 0) break;
             if (iters >= max_iters) throw Error(RQ_ERR_DEVICE, "photon tracing did not drain");
         }
         // Fill the maps in emission order.
