@@ -136,6 +136,94 @@ void trace_closest(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, h
     trace(sc, rays, n, hits, nullptr, TraceKind::Closest, s, nullptr, n_dev);
 }
 
+static void vertex_bounds(const float* xyz, uint32_t nverts, const rq_scene_desc& dd, float bmin[3], float bmax[3]);
+
+// ENCORE's kd-tree of the triangles: nodes uploaded, the triangle records
+// (one per leaf reference, in leaf order) and the prim -> record map returned.
+static void kd_scene_arrays(rq_scene* sc, const float* xyz, uint32_t nverts, const uint32_t* tri_idx,
+                            uint32_t ntris, const rq_scene_desc& dd, std::vector<TriRec>& tris,
+                            std::vector<uint32_t>& slot) {
+    float bmin[3], bmax[3];
+    vertex_bounds(xyz, nverts, dd, bmin, bmax);
+    KdHost h;
+    build_kd(xyz, nverts, tri_idx, ntris, bmin, bmax, h);
+    pack_triangles(xyz, tri_idx, h.leaf_prims.data(), (uint32_t)h.leaf_prims.size(), dd.predicate, tris);
+    // A primitive can sit in several leaves; any of its records will do.
+    slot.assign(ntris, 0xFFFFFFFFu);
+    for (uint32_t j = 0; j < (uint32_t)h.leaf_prims.size(); ++j)
+        if (slot[h.leaf_prims[j]] == 0xFFFFFFFFu) slot[h.leaf_prims[j]] = j;
+    for (uint32_t& v : slot)
+        if (v == 0xFFFFFFFFu) v = 0;  // in no leaf: never hit
+    sc->stats = h.stats;
+    for (int k = 0; k < 3; ++k) {
+        sc->root_pos[k] = h.root_pos[k];
+        sc->root_size[k] = h.root_size[k];
+    }
+    sc->nodes = DevBuf(h.nodes.size() * sizeof(KdNodeG));
+    sc->nodes.upload(h.nodes.data(), sc->nodes.bytes);
+}
+
+// A kd scene rebuilt from moved vertices (same triangles and options).
+static void rebuild_kd(rq_scene* sc, const float* xyz) {
+    std::vector<TriRec> tris;
+    std::vector<uint32_t> slot;
+    rq_scene_desc dd = sc->desc;
+    dd.has_bounds = 0;  // the vertices may have left the bounds given at creation
+    kd_scene_arrays(sc, xyz, sc->nverts, sc->idx_host.data(), sc->ntris, dd, tris, slot);
+    sc->tris = DevBuf(tris.size() * sizeof(TriRec));
+    sc->tris.upload(tris.data(), sc->tris.bytes);
+    sc->prim_slot.upload(slot.data(), slot.size() * 4);
+    sc->device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->leaf_prims.bytes + sc->prim_slot.bytes;
+}
+
+// Refit support of a BVH scene: each record's vertex indices (leaf order)
+// and the inner nodes grouped by depth, deepest level first.
+static void prepare_refit(rq_scene* sc, const std::vector<uint32_t>& slot) {
+    const uint32_t n = sc->ntris;
+    std::vector<uint32_t> vidx(3 * (size_t)n);
+    for (uint32_t t = 0; t < n; ++t)
+        for (int j = 0; j < 3; ++j) vidx[3 * (size_t)slot[t] + j] = sc->idx_host[3 * (size_t)t + j];
+    sc->rec_vidx = DevBuf(vidx.size() * 4);
+    sc->rec_vidx.upload(vidx.data(), sc->rec_vidx.bytes);
+    // Depth of every node, from the node array itself.
+    const uint32_t nn = sc->stats.nnodes;
+    std::vector<uint32_t> refs(sc->width == 4 ? 4 * (size_t)nn : 2 * (size_t)nn);
+    if (sc->width == 4) {
+        std::vector<BvhNode4> nodes(nn);
+        RQ_HIP(hipMemcpy(nodes.data(), sc->nodes.ptr, nn * sizeof(BvhNode4), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < nn; ++i)
+            for (int c = 0; c < 4; ++c) refs[4 * (size_t)i + c] = nodes[i].ref[c];
+    } else {
+        std::vector<BvhNode> nodes(nn);
+        RQ_HIP(hipMemcpy(nodes.data(), sc->nodes.ptr, nn * sizeof(BvhNode), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < nn; ++i)
+            for (int c = 0; c < 2; ++c) refs[2 * (size_t)i + c] = nodes[i].ref[c];
+    }
+    const uint32_t w = sc->width == 4 ? 4 : 2;
+    std::vector<uint32_t> depth(nn, 0), queue{0};
+    uint32_t maxd = 0;
+    for (size_t q = 0; q < queue.size(); ++q) {
+        const uint32_t i = queue[q];
+        for (uint32_t c = 0; c < w; ++c) {
+            const uint32_t r = refs[(size_t)w * i + c];
+            if (r == kRefEmpty || (r & kRefLeaf)) continue;
+            depth[r] = depth[i] + 1;
+            maxd = std::max(maxd, depth[r]);
+            queue.push_back(r);
+        }
+    }
+    std::vector<uint32_t> order;
+    order.reserve(nn);
+    sc->refit_levels.assign(1, 0);
+    for (int64_t d = maxd; d >= 0; --d) {
+        for (uint32_t i = 0; i < nn; ++i)
+            if (depth[i] == (uint32_t)d) order.push_back(i);
+        sc->refit_levels.push_back((uint32_t)order.size());
+    }
+    sc->refit_order = DevBuf(order.size() * 4);
+    sc->refit_order.upload(order.data(), sc->refit_order.bytes);
+}
+
 // Model-list bounds for the kd root: given, or the vertex bounds.
 static void vertex_bounds(const float* xyz, uint32_t nverts, const rq_scene_desc& dd, float bmin[3], float bmax[3]) {
     if (dd.has_bounds) {
@@ -238,33 +326,53 @@ rq_status rq_scene_create(rq_device* dev, const float* xyz, uint32_t nverts, con
                 sc->nodes.upload(h.nodes.data(), sc->nodes.bytes);
             }
         } else {
-            float bmin[3], bmax[3];
-            vertex_bounds(xyz, nverts, dd, bmin, bmax);
-            KdHost h;
-            build_kd(xyz, nverts, tri_idx, ntris, bmin, bmax, h);
-            // One record per leaf reference, in leaf order.
-            pack_triangles(xyz, tri_idx, h.leaf_prims.data(), (uint32_t)h.leaf_prims.size(), dd.predicate, tris);
-            // A primitive can sit in several leaves; any of its records will do.
-            slot.assign(ntris, 0xFFFFFFFFu);
-            for (uint32_t j = 0; j < (uint32_t)h.leaf_prims.size(); ++j)
-                if (slot[h.leaf_prims[j]] == 0xFFFFFFFFu) slot[h.leaf_prims[j]] = j;
-            for (uint32_t& v : slot)
-                if (v == 0xFFFFFFFFu) v = 0;  // in no leaf: never hit
-            sc->stats = h.stats;
-            for (int k = 0; k < 3; ++k) {
-                sc->root_pos[k] = h.root_pos[k];
-                sc->root_size[k] = h.root_size[k];
-            }
-            sc->nodes = DevBuf(h.nodes.size() * sizeof(KdNodeG));
-            sc->nodes.upload(h.nodes.data(), sc->nodes.bytes);
-
+            kd_scene_arrays(sc.get(), xyz, nverts, tri_idx, ntris, dd, tris, slot);
         }
         sc->tris = DevBuf(tris.size() * sizeof(TriRec));
         sc->tris.upload(tris.data(), sc->tris.bytes);
         sc->prim_slot = DevBuf(slot.size() * sizeof(uint32_t));
         sc->prim_slot.upload(slot.data(), sc->prim_slot.bytes);
-        sc->device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->leaf_prims.bytes + sc->prim_slot.bytes;
+        sc->nverts = nverts;
+        sc->idx_host.assign(tri_idx, tri_idx + 3 * (size_t)ntris);
+        sc->desc = dd;
+        if (dd.accel == RQ_ACCEL_BVH) prepare_refit(sc.get(), slot);
+        sc->device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->leaf_prims.bytes + sc->prim_slot.bytes +
+                           sc->rec_vidx.bytes + sc->refit_order.bytes;
         *out = sc.release();
+    });
+}
+
+rq_status rq_scene_refit(rq_scene* sc, const float* d_xyz, uint32_t nverts, void* stream) {
+    return guarded([&] {
+        require(sc && d_xyz, "null argument");
+        require(!sc->idx_host.empty(), "scene has no triangle list (loaded from a file): rebuild it to update");
+        require(nverts == sc->nverts, "vertex count differs from the scene's");
+        hipStream_t s = (hipStream_t)stream;
+        RQ_HIP(hipSetDevice(sc->dev->ordinal));
+        if (sc->accel == RQ_ACCEL_KD) {
+            // Kdtree::update builds the tree again (Kdtree.cpp:84-200): so does
+            // this, on the host, from the moved vertices.
+            std::vector<float> xyz(3 * (size_t)nverts);
+            RQ_HIP(hipMemcpyAsync(xyz.data(), d_xyz, xyz.size() * 4, hipMemcpyDeviceToHost, s));
+            RQ_HIP(hipStreamSynchronize(s));
+            rebuild_kd(sc, xyz.data());
+            return;
+        }
+        RefitLaunch L{};
+        L.xyz = d_xyz;
+        L.rec_vidx = sc->rec_vidx.as<uint32_t>();
+        L.tris = sc->tris.as<TriRec>();
+        L.nrecs = sc->ntris;
+        L.predicate = sc->predicate;
+        L.nodes = sc->nodes.ptr;
+        L.width = sc->width;
+        L.order = sc->refit_order.as<uint32_t>();
+        launch_refit_tris(L, s);
+        for (size_t l = 0; l + 1 < sc->refit_levels.size(); ++l) {
+            L.begin = sc->refit_levels[l];
+            L.end = sc->refit_levels[l + 1];
+            launch_refit_level(L, s);
+        }
     });
 }
 

@@ -284,6 +284,23 @@ void launch_fg_rays(const FgLaunch& L, hipStream_t s);
 void launch_fg_attr(const FgLaunch& L, hipStream_t s);
 void launch_fg_reduce(const FgLaunch& L, hipStream_t s);
 
+// BVH update for moved vertices (refit.hip): triangle records rewritten from
+// the new vertices, then every child box recomputed level by level, deepest
+// first, as the min/max of its triangles' vertices or of its child node's
+// boxes.
+struct RefitLaunch {
+    const float* xyz;          // new vertex positions, 3 per vertex
+    const uint32_t* rec_vidx;  // 3 vertex indices per triangle record
+    TriRec* tris;
+    uint32_t nrecs, predicate;
+    void* nodes;               // BvhNode (width 2) or BvhNode4
+    uint32_t width;
+    const uint32_t* order;     // node ids of one level: order[begin..end)
+    uint32_t begin, end;
+};
+void launch_refit_tris(const RefitLaunch& L, hipStream_t s);
+void launch_refit_level(const RefitLaunch& L, hipStream_t s);
+
 // Photon emission (photon_trace.hip).
 // A material as the tracer uses it: pd = mean diffuse reflectance,
 // pt = transmittance, ptotal = 0 + pd + pt (PhotonMapperCPU.cpp:615-618).
@@ -420,6 +437,15 @@ struct rq_scene {
     float root_pos[3] = {0, 0, 0}, root_size[3] = {0, 0, 0};
     rq::BuildStats stats;
     uint64_t device_bytes = 0;
+    // Update support: the triangle list and build options of rq_scene_create,
+    // and for a BVH the vertex indices of every record (leaf order) and its
+    // nodes grouped by depth, deepest level first.  Absent in scenes loaded
+    // from files.
+    uint32_t nverts = 0;
+    std::vector<uint32_t> idx_host;
+    rq_scene_desc desc{};
+    rq::DevBuf rec_vidx, refit_order;
+    std::vector<uint32_t> refit_levels;  // offsets into refit_order, one level per pair
     rq::DevBuf materials;  // PhMaterial per material (rq_scene_set_materials)
     rq::DevBuf prim_mat;   // prim id -> material index
     uint32_t nmaterials = 0;

@@ -156,6 +156,9 @@ def load():
         L.rq_final_gather.argtypes = [_vp, _vp, _u32, ctypes.c_float, _vp, _u32, ctypes.c_float, _vp, _vp, _u32, _vp,
                                       _u32, _vp, _vp, _vp]
         L.rq_final_gather.restype = ctypes.c_int
+    if hasattr(L, "rq_scene_refit"):
+        L.rq_scene_refit.argtypes = [_vp, _vp, _u32, _vp]
+        L.rq_scene_refit.restype = ctypes.c_int
     if hasattr(L, "rq_emit_photons"):
         L.rq_scene_set_materials.argtypes = [_vp, _vp, _u32, _vp]
         L.rq_scene_set_materials.restype = ctypes.c_int
@@ -296,6 +299,15 @@ class Scene:
         check(load().rq_final_gather(self.h, indirect.h, int(k_ind), float(r_ind), direct.h if direct else None,
                                      int(k_dir), float(r_dir), _ptr(diffuse), _ptr(origin), origin.shape[0],
                                      _ptr(dirs), int(nper), _ptr(rgb), _ptr(ri), _stream(stream)))
+
+    def refit(self, xyz, stream=None):
+        """Update for moved vertices: xyz is a (nverts, 3) float32 device
+        tensor (or host array, copied up first)."""
+        import torch
+        t = xyz if isinstance(xyz, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(xyz, np.float32)).cuda()
+        t = t.contiguous()
+        check(load().rq_scene_refit(self.h, t.data_ptr(), t.shape[0], _stream(stream)))
+        return t
 
     def set_materials(self, materials, prim_material):
         """materials: sequence of dicts (diffuse, transmittance=0,

@@ -15,8 +15,9 @@
  *    prefix are host memory and are copied before the call returns.
  *  - `stream` is a `hipStream_t` passed as `void*`; NULL is the default
  *    stream.  Query calls are asynchronous with respect to the host.
- *  - Scenes and photon maps are immutable after creation, so any number of
- *    threads may issue queries on one object concurrently on distinct streams.
+ *  - Scenes and photon maps change only through rq_scene_refit and
+ *    rq_scene_set_materials; between such calls any number of threads may
+ *    issue queries on one object concurrently on distinct streams.
  */
 #ifndef ENCORE_RQ_H
 #define ENCORE_RQ_H
@@ -133,6 +134,17 @@ rq_status rq_scene_create(rq_device* dev, const float* xyz, uint32_t nverts,
                           const rq_scene_desc* desc, rq_scene** out);
 void rq_scene_destroy(rq_scene* scene);
 rq_status rq_scene_stats(const rq_scene* scene, rq_scene_info* info);
+
+/* Update for moved vertices (AccelerationStructure::update,
+ * AccelerationStructure.h:33-67): d_xyz (device, 3*nverts floats, the same
+ * vertex count and triangles as at rq_scene_create) replaces the positions.
+ * A BVH keeps its tree and refits every box on the device, exactly, from the
+ * new vertices (Bvh::update, BVH.cpp:52-121, only grows its old boxes); a kd
+ * scene is built again on the host, as Kdtree::update does, over the new
+ * vertex bounds (bounds given at creation are not reused: the vertices may
+ * have left them).  Queries on the scene must not run while it updates.
+ * Scenes loaded from files cannot be updated (RQ_ERR_INVALID_ARG). */
+rq_status rq_scene_refit(rq_scene* scene, const float* d_xyz, uint32_t nverts, void* stream);
 
 /* Packed scene files.  rq_scene_save writes a resident scene (its node,
  * triangle, leaf-reference and prim-map arrays, with a versioned header and
