@@ -48,10 +48,10 @@ def parse():
     ap.add_argument("--no-counts", action="store_true")
     ap.add_argument("--no-exact", action="store_true", help="skip the bit-exact kd-path measurement")
     ap.add_argument("--dump", default="", help="write the adjudicated parity differences to this .npz")
-    ap.add_argument("--workload", choices=["c3", "c1", "c2", "c4", "c5", "fg", "pt"], default="c3",
+    ap.add_argument("--workload", choices=["c3", "c1", "c2", "c4", "c5", "fg", "pt", "refit"], default="c3",
                     help="c3 (default): 8M incoherent rays at 1.31M triangles; c1: Cornell box eye rays; c2: "
                          "1080p eye + shadow rays at 1.31M triangles; c4: photon gather; c5: serving; fg: final "
-                         "gather; pt: photon emission")
+                         "gather; pt: photon emission; refit: scene update")
     ap.add_argument("--pt-sample", type=int, default=100000,
                     help="pt: global-map capacity of the CPU baseline and parity sample")
     ap.add_argument("--seed", type=int, default=7, help="pt: seed of the emission draws")
@@ -624,6 +624,80 @@ def _photon_emission_workload(args, dev, thr):
     return 0
 
 
+def _refit_workload(args, dev, thr):
+    """Scene update (AccelerationStructure::update) at 1.31M triangles: the
+    displaced icosphere of configs 2/3 turned 0.05 rad about y with a
+    travelling 1% wobble.  One step = one rq_scene_refit of the BVH4 scene
+    (exact boxes from the new vertices).  Also: the config-3 rays' speed on
+    the refitted tree against a tree built for the moved vertices, and
+    parity on a ray sample against the oracle kd-tree built on them."""
+    import math
+
+    import numpy as np
+    import torch
+
+    import oracle
+    from encore_amd import rq, scenes
+
+    xyz, idx = scenes.config2_scene(level=args.level)
+    p = xyz.astype(np.float64)
+    a = 0.05
+    x = math.cos(a) * p[:, 0] + math.sin(a) * p[:, 2]
+    z = -math.sin(a) * p[:, 0] + math.cos(a) * p[:, 2]
+    w = 1.0 + 0.01 * np.sin(p[:, 1] / 12.0)
+    moved = np.stack([x * w, p[:, 1], z * w], axis=1).astype(np.float32)
+    d_rest, d_moved = torch.from_numpy(xyz).cuda(), torch.from_numpy(moved).cuda()
+    sc = rq.Scene(dev, xyz, idx, accel="bvh")
+    build_s = sc.stats()["build_seconds"]
+    flip = [0]
+
+    def step():
+        flip[0] ^= 1
+        sc.refit(d_moved if flip[0] else d_rest)
+
+    wall, ev = _time_gpu(step, args.steps * 2, args.warmup * 2)
+    sc.refit(d_moved)
+    torch.cuda.synchronize()
+    # Config-3 rays on the moved geometry: refitted tree against a fresh build.
+    prim = scenes.config2_primary()
+    fresh = rq.Scene(dev, moved, idx, accel="bvh")
+    ph = torch.empty((len(prim), 4), device="cuda")
+    fresh.intersect(torch.from_numpy(prim).cuda(), ph)
+    rays, _ = scenes.config3_rays(moved, idx, prim, ph.cpu().numpy(), count=args.rays, nbatches=1)
+    d_rays = torch.from_numpy(rays).cuda()
+    hits = torch.empty((len(rays), 4), device="cuda")
+    speed = {}
+    for name, s_ in (("refitted", sc), ("rebuilt", fresh)):
+        _, t = _time_gpu(lambda s_=s_: s_.intersect(d_rays, hits), args.steps, args.warmup)
+        speed[name] = round(len(rays) / (t / 1e3) / 1e6, 1)
+    sc.intersect(d_rays, hits)
+    got = hits.cpu().numpy()
+    stride = max(1, len(rays) // 65536)
+    samp = np.ascontiguousarray(rays[::stride])
+    kd = oracle.KdTree(moved, idx)
+    ref = kd.intersect(samp, nthreads=thr)
+    parity = _classify(got[::stride], ref, moved, idx, samp, thr)
+    log("refit parity", parity)
+    # ENCORE's Kdtree::update builds the tree again: the oracle's kd build of
+    # the moved vertices, one thread, is the CPU baseline.
+    t0 = time.perf_counter()
+    oracle.KdTree(moved, idx)
+    cpu_s = time.perf_counter() - t0
+    out = {"refit_ms": round(ev, 4), "host_build_s": round(build_s, 3),
+           "c3_rays_Mrays_s": speed, "rays": len(rays)}
+    log("refit", out)
+    _line("scene-update Mtris/s (BVH4 refit)", len(idx) / (ev / 1e3) / 1e6, "Mtris/s", ev,
+          {"workload": "refit of the BVH4 scene of the %d-triangle displaced icosphere to vertices turned 0.05 rad "
+                       "with a 1%% wobble, alternating with the rest pose" % len(idx), "accel": "bvh4"},
+          paths=out, parity=parity,
+          cpu_baseline={"value": round(len(idx) / cpu_s / 1e6, 4), "unit": "Mtris/s", "cores": 1, "kind": "port",
+                        "sample": "ENCORE's kd-tree build (Kdtree::update rebuilds) of the moved %d triangles"
+                                  % len(idx)})
+    sc.destroy()
+    fresh.destroy()
+    return 0
+
+
 def run_workload(args):
     import numpy as np
     import torch
@@ -736,6 +810,8 @@ def run_workload(args):
         return _final_gather_workload(args, dev, thr, trace_fn)
     if args.workload == "pt":
         return _photon_emission_workload(args, dev, thr)
+    if args.workload == "refit":
+        return _refit_workload(args, dev, thr)
     if args.workload == "c4":
         xyz, idx = scenes.config4_scene()
         sc = rq.Scene(dev, xyz, idx, accel="bvh")
