@@ -656,6 +656,7 @@ def _refit_workload(args, dev, thr):
         sc.refit(d_moved if flip[0] else d_rest)
 
     wall, ev = _time_gpu(step, args.steps * 2, args.warmup * 2)
+    log("refit: %.4f ms per update of %d triangles (host build %.3f s)" % (ev, len(idx), build_s))
     sc.refit(d_moved)
     torch.cuda.synchronize()
     # Config-3 rays on the moved geometry: refitted tree against a fresh build.
@@ -663,7 +664,7 @@ def _refit_workload(args, dev, thr):
     fresh = rq.Scene(dev, moved, idx, accel="bvh")
     ph = torch.empty((len(prim), 4), device="cuda")
     fresh.intersect(torch.from_numpy(prim).cuda(), ph)
-    rays, _ = scenes.config3_rays(moved, idx, prim, ph.cpu().numpy(), count=args.rays, nbatches=1)
+    rays, _ = scenes.config3_rays(moved, idx, prim, ph.cpu().numpy(), count=args.rays, nbatches=args.batches)
     d_rays = torch.from_numpy(rays).cuda()
     hits = torch.empty((len(rays), 4), device="cuda")
     speed = {}

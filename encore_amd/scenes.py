@@ -440,6 +440,8 @@ def bounce_rays(points, normals, incoming, count, stream, offset):
 def lognormal_batches(total, count, stream, median=65536, sigma=1.18, lo=4096, hi=1048576):
     """`count` batch sizes: exp(ln median + sigma z) clipped to [lo, hi], then
     rescaled (and re-clipped) until they sum to exactly `total`."""
+    if not count * lo <= total <= count * hi:
+        raise ValueError("%d batches of %d..%d rays cannot hold %d rays" % (count, lo, hi, total))
     z = stream.normal(count)
     s = np.clip(np.exp(np.log(median) + sigma * z), lo, hi)
     for _ in range(64):
