@@ -176,15 +176,19 @@ static void rebuild_kd(rq_scene* sc, const float* xyz) {
     sc->device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->leaf_prims.bytes + sc->prim_slot.bytes;
 }
 
-// Refit support of a BVH scene: each record's vertex indices (leaf order)
-// and the inner nodes grouped by depth, deepest level first.
-static void prepare_refit(rq_scene* sc, const std::vector<uint32_t>& slot) {
+// Refit support of a BVH scene, made on its first update: each record's
+// vertex indices (leaf order) and the inner nodes grouped by depth, deepest
+// level first.
+static void prepare_refit(rq_scene* sc) {
     const uint32_t n = sc->ntris;
-    std::vector<uint32_t> vidx(3 * (size_t)n);
-    for (uint32_t t = 0; t < n; ++t)
-        for (int j = 0; j < 3; ++j) vidx[3 * (size_t)slot[t] + j] = sc->idx_host[3 * (size_t)t + j];
-    sc->rec_vidx = DevBuf(vidx.size() * 4);
-    sc->rec_vidx.upload(vidx.data(), sc->rec_vidx.bytes);
+    if (!sc->rec_vidx.ptr) {
+        std::vector<uint32_t> slot(n), vidx(3 * (size_t)n);
+        RQ_HIP(hipMemcpy(slot.data(), sc->prim_slot.ptr, (size_t)n * 4, hipMemcpyDeviceToHost));
+        for (uint32_t t = 0; t < n; ++t)
+            for (int j = 0; j < 3; ++j) vidx[3 * (size_t)slot[t] + j] = sc->idx_host[3 * (size_t)t + j];
+        sc->rec_vidx = DevBuf(vidx.size() * 4);
+        sc->rec_vidx.upload(vidx.data(), sc->rec_vidx.bytes);
+    }
     // Depth of every node, from the node array itself.
     const uint32_t nn = sc->stats.nnodes;
     std::vector<uint32_t> refs(sc->width == 4 ? 4 * (size_t)nn : 2 * (size_t)nn);
@@ -309,7 +313,12 @@ rq_status rq_scene_create(rq_device* dev, const float* xyz, uint32_t nverts, con
         sc->ntris = ntris;
         std::vector<TriRec> tris;
         std::vector<uint32_t> slot;
-        if (dd.accel == RQ_ACCEL_BVH) {
+        require(dd.builder == RQ_BUILD_HOST_SAH || dd.builder == RQ_BUILD_DEVICE_LBVH, "unknown builder");
+        if (dd.accel == RQ_ACCEL_BVH && dd.builder == RQ_BUILD_DEVICE_LBVH) {
+            if (dd.bvh_width == 4) throw Error(RQ_ERR_UNSUPPORTED, "the device builder makes BVH2 scenes");
+            for (size_t i = 0; i < 3 * (size_t)ntris; ++i) require(tri_idx[i] < nverts, "triangle index out of range");
+            build_bvh_device(sc.get(), xyz, nverts, tri_idx, ntris, dd.leaf_size, nullptr);
+        } else if (dd.accel == RQ_ACCEL_BVH) {
             require(dd.bvh_width == 0 || dd.bvh_width == 2 || dd.bvh_width == 4, "bvh_width must be 0, 2 or 4");
             BvhHost h;
             build_bvh(xyz, nverts, tri_idx, ntris, dd.leaf_size, dd.bvh_width ? dd.bvh_width : 4, h);
@@ -328,14 +337,15 @@ rq_status rq_scene_create(rq_device* dev, const float* xyz, uint32_t nverts, con
         } else {
             kd_scene_arrays(sc.get(), xyz, nverts, tri_idx, ntris, dd, tris, slot);
         }
-        sc->tris = DevBuf(tris.size() * sizeof(TriRec));
-        sc->tris.upload(tris.data(), sc->tris.bytes);
-        sc->prim_slot = DevBuf(slot.size() * sizeof(uint32_t));
-        sc->prim_slot.upload(slot.data(), sc->prim_slot.bytes);
+        if (!tris.empty()) {  // host builds; the device builder has written its own arrays
+            sc->tris = DevBuf(tris.size() * sizeof(TriRec));
+            sc->tris.upload(tris.data(), sc->tris.bytes);
+            sc->prim_slot = DevBuf(slot.size() * sizeof(uint32_t));
+            sc->prim_slot.upload(slot.data(), sc->prim_slot.bytes);
+        }
         sc->nverts = nverts;
         sc->idx_host.assign(tri_idx, tri_idx + 3 * (size_t)ntris);
         sc->desc = dd;
-        if (dd.accel == RQ_ACCEL_BVH) prepare_refit(sc.get(), slot);
         sc->device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->leaf_prims.bytes + sc->prim_slot.bytes +
                            sc->rec_vidx.bytes + sc->refit_order.bytes;
         *out = sc.release();
@@ -358,6 +368,7 @@ rq_status rq_scene_refit(rq_scene* sc, const float* d_xyz, uint32_t nverts, void
             rebuild_kd(sc, xyz.data());
             return;
         }
+        if (sc->refit_levels.empty()) prepare_refit(sc);
         RefitLaunch L{};
         L.xyz = d_xyz;
         L.rec_vidx = sc->rec_vidx.as<uint32_t>();

@@ -99,7 +99,7 @@ constexpr uint32_t kRefSentinel = 0x7FFFFFFFu;
 constexpr uint32_t kLeafMaxCount = 16;
 constexpr uint32_t kLeafMaxFirst = (1u << 27) - 1;
 
-inline uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
+__host__ __device__ inline uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
     return kRefLeaf | ((count - 1) << 27) | first;
 }
 
@@ -299,6 +299,10 @@ struct RefitLaunch {
     uint32_t begin, end;
 };
 void launch_refit_tris(const RefitLaunch& L, hipStream_t s);
+// GPU build of a BVH2 scene (lbvh.hip): fills the scene's nodes, triangle
+// records, prim map, record vertex indices and statistics.
+void build_bvh_device(rq_scene* sc, const float* xyz, uint32_t nverts, const uint32_t* idx, uint32_t ntris,
+                      uint32_t leaf_size, hipStream_t s);
 void launch_refit_level(const RefitLaunch& L, hipStream_t s);
 
 // Photon emission (photon_trace.hip).

@@ -31,7 +31,7 @@ class RQError(RuntimeError):
 class SceneDesc(ctypes.Structure):
     _fields_ = [("accel", ctypes.c_uint32), ("predicate", ctypes.c_uint32), ("leaf_size", ctypes.c_uint32),
                 ("has_bounds", ctypes.c_uint32), ("bounds_min", ctypes.c_float * 3),
-                ("bounds_max", ctypes.c_float * 3), ("bvh_width", ctypes.c_uint32)]
+                ("bounds_max", ctypes.c_float * 3), ("bvh_width", ctypes.c_uint32), ("builder", ctypes.c_uint32)]
 
 
 class SceneInfo(ctypes.Structure):
@@ -228,12 +228,14 @@ class Device:
 class Scene:
     """An acceleration structure resident in HBM."""
 
-    def __init__(self, device, xyz, idx, accel="bvh", predicate="encore", leaf_size=0, bounds=None, width=0):
-        """width: BVH children per node, 2 or 4 (0 = 4)."""
+    def __init__(self, device, xyz, idx, accel="bvh", predicate="encore", leaf_size=0, bounds=None, width=0,
+                 builder="sah"):
+        """width: BVH children per node, 2 or 4 (0 = 4); builder: "sah" (host)
+        or "lbvh" (on the GPU, width 2)."""
         L = load()
         xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
         idx = np.ascontiguousarray(idx, np.uint32).reshape(-1)
-        d = _desc(accel, predicate, leaf_size, bounds, width)
+        d = _desc(accel, predicate, leaf_size, bounds, width, builder)
         h = _vp()
         check(L.rq_scene_create(device.h, xyz.ctypes.data, len(xyz) // 3, idx.ctypes.data, len(idx) // 3,
                                 ctypes.byref(d), ctypes.byref(h)))
@@ -547,12 +549,16 @@ BVH4_NODE_DTYPE = np.dtype([("lox", "<f4", 4), ("hix", "<f4", 4), ("loy", "<f4",
 REF_EMPTY = 0xFFFFFFFF
 
 
-def _desc(accel, predicate, leaf_size, bounds, width=0):
+BUILDER = {"sah": 0, "lbvh": 1}
+
+
+def _desc(accel, predicate, leaf_size, bounds, width=0, builder="sah"):
     d = SceneDesc()
     d.accel = ACCEL[accel]
     d.predicate = PRED[predicate]
     d.leaf_size = leaf_size
     d.bvh_width = width
+    d.builder = BUILDER[builder]
     if bounds is not None:
         d.has_bounds = 1
         for k in range(3):

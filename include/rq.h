@@ -96,7 +96,19 @@ typedef struct rq_scene_desc {
     float bounds_max[3];
     uint32_t bvh_width;    /* BVH only: children per node, 2 (64-byte nodes)
                               or 4 (128-byte nodes); 0 = 4 */
+    uint32_t builder;      /* BVH only: RQ_BUILD_HOST_SAH (0) or
+                              RQ_BUILD_DEVICE_LBVH (1, width 2) */
 } rq_scene_desc;
+
+enum {
+    /* Binned-SAH build on the host's cores: the fastest trees. */
+    RQ_BUILD_HOST_SAH = 0,
+    /* Linear BVH built on the GPU (Karras 2012 over 63-bit Morton codes of
+     * the triangle centroids, subtrees of up to leaf_size triangles made
+     * leaves): builds in milliseconds for scenes admitted on the fly; its
+     * trees trace more slowly than SAH trees. */
+    RQ_BUILD_DEVICE_LBVH = 1
+};
 
 /* Scene statistics, filled by rq_scene_stats. */
 typedef struct rq_scene_info {
