@@ -305,6 +305,9 @@ void build_bvh_device(rq_scene* sc, const float* xyz, uint32_t nverts, const uin
     const uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
     RQ_HIP(hipMemcpyAsync(cb.ptr, init, sizeof(init), hipMemcpyHostToDevice, s));
     launch_zero_u32(L.arrive, (uint32_t)ni, s);
+    // Slots of internal nodes that end up inside a leaf are never written by
+    // lb_emit: keep them zero rather than stale memory.
+    launch_zero_u32(nodes.as<uint32_t>(), (uint32_t)(ni * sizeof(BvhNode) / 4), s);
     launch_zero_u32(L.stats, 2, s);
     hipLaunchKernelGGL(dev::lb_boxes, dim3(grid_of(n)), dim3(256), 0, s, L);
     RQ_HIP(hipGetLastError());

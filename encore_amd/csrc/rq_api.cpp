@@ -203,14 +203,19 @@ static void prepare_refit(rq_scene* sc) {
         for (uint32_t i = 0; i < nn; ++i)
             for (int c = 0; c < 2; ++c) refs[2 * (size_t)i + c] = nodes[i].ref[c];
     }
+    // Only nodes reached from the root get a level: a GPU-built scene leaves
+    // the slots of subtrees it made into leaves unused.
     const uint32_t w = sc->width == 4 ? 4 : 2;
-    std::vector<uint32_t> depth(nn, 0), queue{0};
+    constexpr uint32_t kUnreached = 0xFFFFFFFFu;
+    std::vector<uint32_t> depth(nn, kUnreached), queue{0};
+    depth[0] = 0;
     uint32_t maxd = 0;
     for (size_t q = 0; q < queue.size(); ++q) {
         const uint32_t i = queue[q];
         for (uint32_t c = 0; c < w; ++c) {
             const uint32_t r = refs[(size_t)w * i + c];
             if (r == kRefEmpty || (r & kRefLeaf)) continue;
+            if (r >= nn || depth[r] != kUnreached) throw Error(RQ_ERR_DEVICE, "BVH node array is not a tree");
             depth[r] = depth[i] + 1;
             maxd = std::max(maxd, depth[r]);
             queue.push_back(r);
