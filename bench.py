@@ -671,6 +671,20 @@ def _refit_workload(args, dev, thr):
     for name, s_ in (("refitted", sc), ("rebuilt", fresh)):
         _, t = _time_gpu(lambda s_=s_: s_.intersect(d_rays, hits), args.steps, args.warmup)
         speed[name] = round(len(rays) / (t / 1e3) / 1e6, 1)
+    # The GPU build (linear BVH, BVH2 layout) of the moved vertices: wall time
+    # of rq_scene_create including the upload, and the same rays on its tree.
+    builds = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        lb = rq.Scene(dev, moved, idx, accel="bvh", builder="lbvh")
+        builds.append(time.perf_counter() - t0)
+        if len(builds) < 3:
+            lb.destroy()
+    _, t = _time_gpu(lambda: lb.intersect(d_rays, hits), args.steps, args.warmup)
+    speed["lbvh"] = round(len(rays) / (t / 1e3) / 1e6, 1)
+    lb.intersect(d_rays, hits)
+    got_lb = hits.cpu().numpy()
+    log("lbvh build %.1f ms, %.1f Mrays/s" % (min(builds) * 1e3, speed["lbvh"]))
     sc.intersect(d_rays, hits)
     got = hits.cpu().numpy()
     stride = max(1, len(rays) // 65536)
@@ -678,14 +692,16 @@ def _refit_workload(args, dev, thr):
     kd = oracle.KdTree(moved, idx)
     ref = kd.intersect(samp, nthreads=thr)
     parity = _classify(got[::stride], ref, moved, idx, samp, thr)
-    log("refit parity", parity)
+    parity_lb = _classify(got_lb[::stride], ref, moved, idx, samp, thr)
+    log("refit parity", parity, "lbvh parity", parity_lb)
     # ENCORE's Kdtree::update builds the tree again: the oracle's kd build of
     # the moved vertices, one thread, is the CPU baseline.
     t0 = time.perf_counter()
     oracle.KdTree(moved, idx)
     cpu_s = time.perf_counter() - t0
-    out = {"refit_ms": round(ev, 4), "host_build_s": round(build_s, 3),
-           "c3_rays_Mrays_s": speed, "rays": len(rays)}
+    out = {"refit_ms": round(ev, 4), "host_build_s": round(build_s, 3), "lbvh_build_ms": round(min(builds) * 1e3, 2),
+           "c3_rays_Mrays_s": speed, "rays": len(rays), "lbvh_parity": parity_lb}
+    lb.destroy()
     log("refit", out)
     _line("scene-update Mtris/s (BVH4 refit)", len(idx) / (ev / 1e3) / 1e6, "Mtris/s", ev,
           {"workload": "refit of the BVH4 scene of the %d-triangle displaced icosphere to vertices turned 0.05 rad "
