@@ -18,8 +18,8 @@ using encore::Point3;
 using encore::Vector3;
 
 HipAccel::HipAccel()
-    : m_Kind(RQ_ACCEL_BVH), m_Predicate(RQ_PRED_ENCORE), m_Width(4), m_Ordinal(0), m_Device(0), m_Scene(0), m_dRays(0),
-      m_dOut(0), m_Capacity(0) {}
+    : m_Kind(RQ_ACCEL_BVH), m_Predicate(RQ_PRED_ENCORE), m_Width(4), m_Builder(RQ_BUILD_HOST_SAH), m_Ordinal(0),
+      m_Device(0), m_Scene(0), m_dRays(0), m_dOut(0), m_dXyz(0), m_Capacity(0) {}
 
 HipAccel::~HipAccel() {
     release();
@@ -30,7 +30,9 @@ void HipAccel::usage(void) {
     printf("HipAccel: MI355X ray-query engine\n"
            "  HipAccelKind  BVH | KD      (KD reproduces Kdtree exactly)\n"
            "  HipPredicate  Encore | Watertight\n"
-           "  HipDevice     GPU ordinal\n");
+           "  HipDevice     GPU ordinal\n"
+           "  HipBvhWidth   4 | 2\n"
+           "  HipBvhBuilder SAH | LBVH  (LBVH builds on the GPU)\n");
 }
 
 void HipAccel::configure(Options* o) {
@@ -40,6 +42,8 @@ void HipAccel::configure(Options* o) {
         m_Predicate = (o->getOption("HipPredicate") == "Watertight") ? RQ_PRED_WATERTIGHT : RQ_PRED_ENCORE;
     if (o->doesOptionExist("HipDevice")) m_Ordinal = atoi(o->getOption("HipDevice").c_str());
     if (o->doesOptionExist("HipBvhWidth")) m_Width = (atoi(o->getOption("HipBvhWidth").c_str()) == 2) ? 2u : 4u;
+    if (o->doesOptionExist("HipBvhBuilder"))
+        m_Builder = (o->getOption("HipBvhBuilder") == "LBVH") ? RQ_BUILD_DEVICE_LBVH : RQ_BUILD_HOST_SAH;
 }
 
 bool HipAccel::ok(rq_status s) {
@@ -54,17 +58,20 @@ void HipAccel::release() {
     m_Scene = 0;
     if (m_dRays) (void)hipFree(m_dRays);
     if (m_dOut) (void)hipFree(m_dOut);
-    m_dRays = m_dOut = 0;
+    if (m_dXyz) (void)hipFree(m_dXyz);
+    m_dRays = m_dOut = m_dXyz = 0;
     m_Capacity = 0;
 }
 
-void HipAccel::build(std::list<IModel*>& modelList) {
-    release();
-    m_Prims.clear();
-    m_Xyz.clear();
-    if (!m_Device && !ok(rq_device_open(m_Ordinal, &m_Device))) return;
+void HipAccel::gather(std::list<IModel*>& modelList, std::vector<IPrimitive*>& prims, std::vector<float>& xyz,
+                      float bmin[3], float bmax[3]) {
+    prims.clear();
+    xyz.clear();
     // Model-list bounds, as Kdtree::build gathers them (Kdtree.cpp:92-107).
-    float bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < 3; ++k) {
+        bmin[k] = INFINITY;
+        bmax[k] = -INFINITY;
+    }
     for (std::list<IModel*>::iterator m = modelList.begin(); m != modelList.end(); ++m) {
         for (int k = 0; k < 3; ++k) {
             bmin[k] = std::min((*m)->MinPoint()[k], bmin[k]);
@@ -81,15 +88,22 @@ void HipAccel::build(std::list<IModel*>& modelList) {
             const Vertex* v[3] = {t->getVertex0(), t->getVertex1(), t->getVertex2()};
             for (int j = 0; j < 3; ++j) {
                 const Point3& c = v[j]->getCoordinates();
-                m_Xyz.push_back(c.X());
-                m_Xyz.push_back(c.Y());
-                m_Xyz.push_back(c.Z());
+                xyz.push_back(c.X());
+                xyz.push_back(c.Y());
+                xyz.push_back(c.Z());
             }
             for (std::list<Triangle*>::iterator it = tl->begin(); it != tl->end(); ++it) delete *it;
             delete tl;
-            m_Prims.push_back(*p);
+            prims.push_back(*p);
         }
     }
+}
+
+void HipAccel::build(std::list<IModel*>& modelList) {
+    release();
+    if (!m_Device && !ok(rq_device_open(m_Ordinal, &m_Device))) return;
+    float bmin[3], bmax[3];
+    gather(modelList, m_Prims, m_Xyz, bmin, bmax);
     if (m_Prims.empty()) return;
     const uint32_t nt = (uint32_t)m_Prims.size();
     std::vector<uint32_t> idx(3 * (size_t)nt);
@@ -98,13 +112,43 @@ void HipAccel::build(std::list<IModel*>& modelList) {
     memset(&d, 0, sizeof(d));
     d.accel = m_Kind;
     d.predicate = m_Predicate;
-    d.bvh_width = m_Width;
+    d.bvh_width = m_Builder == RQ_BUILD_DEVICE_LBVH ? 2 : m_Width;
+    d.builder = m_Builder;
     d.has_bounds = 1;
     for (int k = 0; k < 3; ++k) {
         d.bounds_min[k] = bmin[k];
         d.bounds_max[k] = bmax[k];
     }
     ok(rq_scene_create(m_Device, m_Xyz.data(), 3 * nt, idx.data(), nt, &d, &m_Scene));
+}
+
+// AccelerationStructure::update for dynamic models (DynamicModel.cpp:85):
+// the same primitives with moved vertices are refitted in place; anything
+// else is built again.
+void HipAccel::update(std::list<IModel*>& modelList) {
+    if (!m_Scene) {
+        build(modelList);
+        return;
+    }
+    std::vector<IPrimitive*> prims;
+    std::vector<float> xyz;
+    float bmin[3], bmax[3];
+    gather(modelList, prims, xyz, bmin, bmax);
+    if (prims != m_Prims) {
+        build(modelList);
+        return;
+    }
+    if (!m_dXyz && hipMalloc(&m_dXyz, xyz.size() * sizeof(float)) != hipSuccess) {
+        m_dXyz = 0;
+        m_Error = "device vertex buffer allocation failed";
+        return;
+    }
+    if (hipMemcpy(m_dXyz, xyz.data(), xyz.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        m_Error = "vertex upload failed";
+        return;
+    }
+    m_Xyz.swap(xyz);
+    ok(rq_scene_refit(m_Scene, static_cast<const float*>(m_dXyz), (uint32_t)(m_Xyz.size() / 3), 0));
 }
 
 void HipAccel::ensureCapacity(size_t n) {

@@ -14,6 +14,9 @@
 //   intersect(ray, hit) one-ray query; fills HitInfo exactly as
 //                       TrianglePrim::intersect does (TrianglePrim.cpp:226-244)
 //                       from the engine's (t, alpha, beta, prim);
+//   update(modelList)   moved vertices, same primitives: the BVH is refitted
+//                       on the GPU (rq_scene_refit), a KD scene rebuilt;
+//                       a changed primitive list builds afresh;
 //   intersectBatch /    batched extensions for renderers that trace many
 //   occludedBatch       rays at once (one launch per call).
 //
@@ -22,6 +25,8 @@
 //                                          bit for bit, BVH up to ties
 //   HipPredicate   Encore (default) | Watertight
 //   HipDevice      GPU ordinal (default 0)
+//   HipBvhWidth    4 (default) | 2
+//   HipBvhBuilder  SAH (default, host) | LBVH (GPU, BVH2 trees)
 #pragma once
 
 #include <cstdint>
@@ -43,7 +48,7 @@ public:
     void usage(void);
     void configure(Options* l_pOptions);
     void build(std::list<IModel*>& modelList);
-    void update(std::list<IModel*>& modelList) { build(modelList); }
+    void update(std::list<IModel*>& modelList);
 #ifdef WIN32
     // ENCORE's GL-texture GPU path is not used: the engine keeps its own
     // layout in HBM.
@@ -63,13 +68,18 @@ public:
 
 private:
     void release();
+    // Triangles of the model list in primitive-id order (9 floats each) and
+    // the model-list bounds.
+    void gather(std::list<IModel*>& modelList, std::vector<IPrimitive*>& prims, std::vector<float>& xyz,
+                float bmin[3], float bmax[3]);
     void ensureCapacity(size_t n);
     void fillHit(Ray& ray, const rq_hit& h, HitInfo* out) const;
     bool ok(rq_status s);
 
     uint32_t m_Kind;
     uint32_t m_Predicate;
-    uint32_t m_Width;  // BVH children per node: 2 or 4
+    uint32_t m_Width;    // BVH children per node: 2 or 4
+    uint32_t m_Builder;  // RQ_BUILD_HOST_SAH or RQ_BUILD_DEVICE_LBVH
     int m_Ordinal;
     rq_device* m_Device;
     rq_scene* m_Scene;
@@ -77,6 +87,7 @@ private:
     std::vector<float> m_Xyz;          // 9 floats per triangle (v0, v1, v2)
     void* m_dRays;                     // device staging buffers
     void* m_dOut;
+    void* m_dXyz;                      // vertex positions for update()
     size_t m_Capacity;
     std::string m_Error;
 };
