@@ -13,13 +13,17 @@
 //   lb_hierarchy  one thread per internal node finds its key range and split
 //                 from the longest common prefixes (equal keys broken by
 //                 index), so every node is built independently
-//   lb_bottom_up  boxes from the leaves up: the second of two children to
-//                 arrive at a parent forms its box and carries on
-//   lb_emit       one BvhNode per internal node whose range holds more than
-//                 leaf_size triangles; a child range of at most leaf_size
-//                 triangles (contiguous in sorted order) becomes one leaf
 //   lb_pack       triangle records in sorted order, prim -> record map and the
-//                 record vertex indices (for later refits)
+//                 record vertex indices
+//   lb_emit       the refs of one BvhNode per internal node whose range holds
+//                 more than leaf_size triangles (a child range of at most
+//                 leaf_size triangles, contiguous in sorted order, becomes one
+//                 leaf), and the node's depth, counted per level
+//   lb_order      nodes grouped by depth, deepest level first
+//   (refit levels) the boxes, from the leaves up, by the update kernels: one
+//                 launch per level instead of a bottom-up pass whose
+//                 cross-wave handshakes need device-scope fences, which cost
+//                 an L2 write-back each on a GPU with one L2 per XCD
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -47,9 +51,9 @@ struct LbLaunch {
     uint32_t* first;     // per internal node: key range [first, last]
     uint32_t* last;
     uint32_t* parent_in; // per internal node (root: ~0)
-    uint32_t* parent_lf; // per leaf
-    uint32_t* arrive;    // per internal node: children done
-    float* node_box;     // 6 per internal node
+    uint32_t* depth;     // per internal node (emitted ones)
+    uint32_t* level_cnt; // nodes per depth, then the fill cursor per depth
+    uint32_t* order;     // emitted nodes, deepest level first
     BvhNode* nodes;
     uint32_t leaf_size;
     uint32_t* stats;     // [0] leaf refs, [1] deepest emitted node
@@ -156,49 +160,16 @@ __global__ void lb_hierarchy(LbLaunch L) {
     L.child[2 * i + 1] = right;
     L.first[i] = (uint32_t)f;
     L.last[i] = (uint32_t)la;
-    if (left & kLbLeafBit) L.parent_lf[gamma] = (uint32_t)i; else L.parent_in[gamma] = (uint32_t)i;
-    if (right & kLbLeafBit) L.parent_lf[gamma + 1] = (uint32_t)i; else L.parent_in[gamma + 1] = (uint32_t)i;
+    if (!(left & kLbLeafBit)) L.parent_in[gamma] = (uint32_t)i;
+    if (!(right & kLbLeafBit)) L.parent_in[gamma + 1] = (uint32_t)i;
     if (i == 0) L.parent_in[0] = 0xFFFFFFFFu;
-}
-
-// Reads six floats written by another wave: agent-scope loads, so the values
-// come from L2 and not from a stale line of this CU's cache.
-__device__ __forceinline__ void load_box(const float* p, float* b) {
-    for (int k = 0; k < 6; ++k)
-        b[k] = __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(p) + k, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT));
-}
-
-__device__ __forceinline__ void child_box(const LbLaunch& L, uint32_t c, float* b) {
-    if (c & kLbLeafBit)
-        load_box(L.tri_box + 6 * (size_t)L.ids[c & ~kLbLeafBit], b);
-    else
-        load_box(L.node_box + 6 * (size_t)c, b);
-}
-
-__global__ void lb_bottom_up(LbLaunch L) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= L.n) return;
-    uint32_t p = L.parent_lf[s];
-    while (p != 0xFFFFFFFFu) {
-        __threadfence();
-        if (atomicAdd(&L.arrive[p], 1u) == 0) return;  // the sibling is not done yet: it carries on
-        __threadfence();
-        float a[6], b[6];
-        child_box(L, L.child[2 * p], a);
-        child_box(L, L.child[2 * p + 1], b);
-        float* o = L.node_box + 6 * (size_t)p;
-        for (int k = 0; k < 3; ++k) {
-            o[k] = lo_of(a[k], b[k]);
-            o[3 + k] = hi_of(a[3 + k], b[3 + k]);
-        }
-        p = L.parent_in[p];
-    }
 }
 
 __device__ __forceinline__ uint32_t range_size(const LbLaunch& L, uint32_t c) {
     return (c & kLbLeafBit) ? 1u : L.last[c] - L.first[c] + 1;
 }
+
+constexpr uint32_t kLbMaxLevels = 256;
 
 __global__ void lb_emit(LbLaunch L) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -207,14 +178,6 @@ __global__ void lb_emit(LbLaunch L) {
     BvhNode g;
     for (int side = 0; side < 2; ++side) {
         const uint32_t c = L.child[2 * i + side];
-        float b[6];
-        child_box(L, c, b);
-        g.x[2 * side] = b[0];
-        g.x[2 * side + 1] = b[3];
-        g.y[2 * side] = b[1];
-        g.y[2 * side + 1] = b[4];
-        g.z[2 * side] = b[2];
-        g.z[2 * side + 1] = b[5];
         const uint32_t sz = range_size(L, c);
         if (sz <= L.leaf_size) {
             const uint32_t f = (c & kLbLeafBit) ? (c & ~kLbLeafBit) : L.first[c];
@@ -223,13 +186,27 @@ __global__ void lb_emit(LbLaunch L) {
         } else {
             g.ref[side] = c;
         }
+        g.x[2 * side] = g.x[2 * side + 1] = 0.0f;  // boxes: filled by the refit levels
+        g.y[2 * side] = g.y[2 * side + 1] = 0.0f;
+        g.z[2 * side] = g.z[2 * side + 1] = 0.0f;
     }
     g.pad[0] = g.pad[1] = 0;
     L.nodes[i] = g;
-    // depth of this node: hops to the root
-    uint32_t depth = 0;
+    uint32_t depth = 0;  // hops to the root
     for (uint32_t p = L.parent_in[i]; p != 0xFFFFFFFFu; p = L.parent_in[p]) ++depth;
+    L.depth[i] = depth;
     atomicMax(&L.stats[1], depth);
+    if (depth < kLbMaxLevels) atomicAdd(&L.level_cnt[depth], 1u);
+}
+
+// level_cnt[kLbMaxLevels + d] holds, on entry, the first slot of depth d in
+// the deepest-first order and serves as that level's fill cursor.
+__global__ void lb_order(LbLaunch L) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + 1 >= L.n) return;
+    if (L.last[i] - L.first[i] + 1 <= L.leaf_size && i != 0) return;
+    const uint32_t pos = atomicAdd(&L.level_cnt[kLbMaxLevels + L.depth[i]], 1u);
+    L.order[pos] = i;
 }
 
 __global__ void lb_pack(LbLaunch L) {
@@ -276,9 +253,9 @@ void build_bvh_device(rq_scene* sc, const float* xyz, uint32_t nverts, const uin
     RQ_HIP(hipMemcpyAsync(d_xyz.ptr, xyz, d_xyz.bytes, hipMemcpyHostToDevice, s));
     RQ_HIP(hipMemcpyAsync(d_idx.ptr, idx, d_idx.bytes, hipMemcpyHostToDevice, s));
     DevBuf tri_box(6 * n * 4), cen(3 * n * 4), cb(64), keys(n * 8), skeys(n * 8), ids(n * 4), sids(n * 4),
-        child(2 * ni * 4), first(ni * 4), last(ni * 4), parent_in(ni * 4), parent_lf(n * 4), arrive(ni * 4),
-        node_box(6 * ni * 4), stats(64);
-    DevBuf nodes(ni * sizeof(BvhNode)), tris(n * sizeof(TriRec)), slot(n * 4), vidx(3 * n * 4);
+        child(2 * ni * 4), first(ni * 4), last(ni * 4), parent_in(ni * 4), depth(ni * 4),
+        level_cnt(2 * dev::kLbMaxLevels * 4), stats(64);
+    DevBuf nodes(ni * sizeof(BvhNode)), tris(n * sizeof(TriRec)), slot(n * 4), vidx(3 * n * 4), order(ni * 4);
     LbLaunch L{};
     L.xyz = d_xyz.as<float>();
     L.idx = d_idx.as<uint32_t>();
@@ -292,9 +269,9 @@ void build_bvh_device(rq_scene* sc, const float* xyz, uint32_t nverts, const uin
     L.first = first.as<uint32_t>();
     L.last = last.as<uint32_t>();
     L.parent_in = parent_in.as<uint32_t>();
-    L.parent_lf = parent_lf.as<uint32_t>();
-    L.arrive = arrive.as<uint32_t>();
-    L.node_box = node_box.as<float>();
+    L.depth = depth.as<uint32_t>();
+    L.level_cnt = level_cnt.as<uint32_t>();
+    L.order = order.as<uint32_t>();
     L.nodes = nodes.as<BvhNode>();
     L.leaf_size = leaf_size;
     L.stats = stats.as<uint32_t>();
@@ -304,7 +281,7 @@ void build_bvh_device(rq_scene* sc, const float* xyz, uint32_t nverts, const uin
     L.predicate = sc->predicate;
     const uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
     RQ_HIP(hipMemcpyAsync(cb.ptr, init, sizeof(init), hipMemcpyHostToDevice, s));
-    launch_zero_u32(L.arrive, (uint32_t)ni, s);
+    launch_zero_u32(L.level_cnt, dev::kLbMaxLevels, s);
     // Slots of internal nodes that end up inside a leaf are never written by
     // lb_emit: keep them zero rather than stale memory.
     launch_zero_u32(nodes.as<uint32_t>(), (uint32_t)(ni * sizeof(BvhNode) / 4), s);
@@ -323,7 +300,10 @@ void build_bvh_device(rq_scene* sc, const float* xyz, uint32_t nverts, const uin
                                               sids.as<uint32_t>(), (int)n, 0, 63, s));
     L.skeys = skeys.as<unsigned long long>();
     L.ids = sids.as<uint32_t>();
+    hipLaunchKernelGGL(dev::lb_pack, dim3(grid_of(n)), dim3(256), 0, s, L);
+    RQ_HIP(hipGetLastError());
     uint32_t stat[2] = {0, 0};
+    std::vector<uint32_t> levels;  // offsets into order, deepest level first
     if (n == 1) {
         // One triangle: a root whose two slots name the same one-triangle leaf.
         BvhNode g{};
@@ -340,24 +320,58 @@ void build_bvh_device(rq_scene* sc, const float* xyz, uint32_t nverts, const uin
             g.ref[side] = make_leaf_ref(0, 1);
         }
         RQ_HIP(hipMemcpyAsync(nodes.ptr, &g, sizeof(g), hipMemcpyHostToDevice, s));
+        const uint32_t zero = 0;
+        RQ_HIP(hipMemcpyAsync(order.ptr, &zero, 4, hipMemcpyHostToDevice, s));
         stat[0] = 1;
+        levels = {0, 1};
     } else {
         hipLaunchKernelGGL(dev::lb_hierarchy, dim3(grid_of(n - 1)), dim3(256), 0, s, L);
         RQ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(dev::lb_bottom_up, dim3(grid_of(n)), dim3(256), 0, s, L);
-        RQ_HIP(hipGetLastError());
         hipLaunchKernelGGL(dev::lb_emit, dim3(grid_of(n - 1)), dim3(256), 0, s, L);
         RQ_HIP(hipGetLastError());
+        std::vector<uint32_t> cnt(dev::kLbMaxLevels);
+        RQ_HIP(hipMemcpyAsync(stat, L.stats, 8, hipMemcpyDeviceToHost, s));
+        RQ_HIP(hipMemcpyAsync(cnt.data(), L.level_cnt, cnt.size() * 4, hipMemcpyDeviceToHost, s));
+        RQ_HIP(hipStreamSynchronize(s));
+        if (stat[1] >= dev::kLbMaxLevels)
+            throw Error(RQ_ERR_UNSUPPORTED, "linear BVH deeper than 256 levels (use the host builder)");
+        // Deepest level first: the slot where each depth starts.
+        std::vector<uint32_t> start(dev::kLbMaxLevels, 0);
+        uint32_t run = 0;
+        levels.push_back(0);
+        for (int64_t d = stat[1]; d >= 0; --d) {
+            start[d] = run;
+            run += cnt[d];
+            levels.push_back(run);
+        }
+        RQ_HIP(hipMemcpyAsync(L.level_cnt + dev::kLbMaxLevels, start.data(), start.size() * 4, hipMemcpyHostToDevice,
+                              s));
+        hipLaunchKernelGGL(dev::lb_order, dim3(grid_of(n - 1)), dim3(256), 0, s, L);
+        RQ_HIP(hipGetLastError());
+        // Boxes from the leaves up.
+        RefitLaunch R{};
+        R.xyz = L.xyz;
+        R.rec_vidx = L.rec_vidx;
+        R.tris = L.tris;
+        R.nrecs = ntris;
+        R.predicate = sc->predicate;
+        R.nodes = nodes.ptr;
+        R.width = 2;
+        R.order = order.as<uint32_t>();
+        for (size_t l = 0; l + 1 < levels.size(); ++l) {
+            R.begin = levels[l];
+            R.end = levels[l + 1];
+            launch_refit_level(R, s);
+        }
     }
-    hipLaunchKernelGGL(dev::lb_pack, dim3(grid_of(n)), dim3(256), 0, s, L);
-    RQ_HIP(hipGetLastError());
-    if (n > 1) RQ_HIP(hipMemcpyAsync(stat, L.stats, 8, hipMemcpyDeviceToHost, s));
     RQ_HIP(hipStreamSynchronize(s));
     sc->width = 2;
     sc->nodes = std::move(nodes);
     sc->tris = std::move(tris);
     sc->prim_slot = std::move(slot);
     sc->rec_vidx = std::move(vidx);
+    sc->refit_order = std::move(order);
+    sc->refit_levels = std::move(levels);
     sc->stats = BuildStats{};
     sc->stats.nnodes = (uint32_t)ni;  // internal nodes inside leaves stay unused
     sc->stats.nleaves = stat[0];
