@@ -171,42 +171,74 @@ __device__ __forceinline__ uint32_t range_size(const LbLaunch& L, uint32_t c) {
 
 constexpr uint32_t kLbMaxLevels = 256;
 
-__global__ void lb_emit(LbLaunch L) {
+__device__ __forceinline__ bool emitted(const LbLaunch& L, uint32_t i) {
+    return i + 1 < L.n && (i == 0 || L.last[i] - L.first[i] + 1 > L.leaf_size);
+}
+
+// Counts go through LDS: a block's nodes add to its own histogram of depths
+// and the block adds that to the global one, so a few hundred global
+// atomics per level replace one per node.
+__global__ __launch_bounds__(256) void lb_emit(LbLaunch L) {
+    __shared__ uint32_t s_cnt[kLbMaxLevels];
+    __shared__ uint32_t s_leaves, s_deep;
+    for (uint32_t d = threadIdx.x; d < kLbMaxLevels; d += blockDim.x) s_cnt[d] = 0;
+    if (threadIdx.x == 0) s_leaves = s_deep = 0;
+    __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i + 1 >= L.n) return;
-    if (L.last[i] - L.first[i] + 1 <= L.leaf_size && i != 0) return;  // inside a leaf of its parent
-    BvhNode g;
-    for (int side = 0; side < 2; ++side) {
-        const uint32_t c = L.child[2 * i + side];
-        const uint32_t sz = range_size(L, c);
-        if (sz <= L.leaf_size) {
-            const uint32_t f = (c & kLbLeafBit) ? (c & ~kLbLeafBit) : L.first[c];
-            g.ref[side] = make_leaf_ref(f, sz);
-            atomicAdd(&L.stats[0], 1u);
-        } else {
-            g.ref[side] = c;
+    if (emitted(L, i)) {
+        BvhNode g;
+        uint32_t leaves = 0;
+        for (int side = 0; side < 2; ++side) {
+            const uint32_t c = L.child[2 * i + side];
+            const uint32_t sz = range_size(L, c);
+            if (sz <= L.leaf_size) {
+                const uint32_t f = (c & kLbLeafBit) ? (c & ~kLbLeafBit) : L.first[c];
+                g.ref[side] = make_leaf_ref(f, sz);
+                ++leaves;
+            } else {
+                g.ref[side] = c;
+            }
+            g.x[2 * side] = g.x[2 * side + 1] = 0.0f;  // boxes: filled by the refit levels
+            g.y[2 * side] = g.y[2 * side + 1] = 0.0f;
+            g.z[2 * side] = g.z[2 * side + 1] = 0.0f;
         }
-        g.x[2 * side] = g.x[2 * side + 1] = 0.0f;  // boxes: filled by the refit levels
-        g.y[2 * side] = g.y[2 * side + 1] = 0.0f;
-        g.z[2 * side] = g.z[2 * side + 1] = 0.0f;
+        g.pad[0] = g.pad[1] = 0;
+        L.nodes[i] = g;
+        uint32_t depth = 0;  // hops to the root
+        for (uint32_t p = L.parent_in[i]; p != 0xFFFFFFFFu; p = L.parent_in[p]) ++depth;
+        L.depth[i] = depth;
+        if (leaves) atomicAdd(&s_leaves, leaves);
+        atomicMax(&s_deep, depth);
+        if (depth < kLbMaxLevels) atomicAdd(&s_cnt[depth], 1u);
     }
-    g.pad[0] = g.pad[1] = 0;
-    L.nodes[i] = g;
-    uint32_t depth = 0;  // hops to the root
-    for (uint32_t p = L.parent_in[i]; p != 0xFFFFFFFFu; p = L.parent_in[p]) ++depth;
-    L.depth[i] = depth;
-    atomicMax(&L.stats[1], depth);
-    if (depth < kLbMaxLevels) atomicAdd(&L.level_cnt[depth], 1u);
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < kLbMaxLevels; d += blockDim.x)
+        if (s_cnt[d]) atomicAdd(&L.level_cnt[d], s_cnt[d]);
+    if (threadIdx.x == 0) {
+        if (s_leaves) atomicAdd(&L.stats[0], s_leaves);
+        atomicMax(&L.stats[1], s_deep);
+    }
 }
 
 // level_cnt[kLbMaxLevels + d] holds, on entry, the first slot of depth d in
-// the deepest-first order and serves as that level's fill cursor.
-__global__ void lb_order(LbLaunch L) {
+// the deepest-first order and serves as that level's fill cursor: each block
+// reserves one range per depth it holds and places its nodes inside it.
+__global__ __launch_bounds__(256) void lb_order(LbLaunch L) {
+    __shared__ uint32_t s_cnt[kLbMaxLevels], s_base[kLbMaxLevels];
+    for (uint32_t d = threadIdx.x; d < kLbMaxLevels; d += blockDim.x) s_cnt[d] = 0;
+    __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i + 1 >= L.n) return;
-    if (L.last[i] - L.first[i] + 1 <= L.leaf_size && i != 0) return;
-    const uint32_t pos = atomicAdd(&L.level_cnt[kLbMaxLevels + L.depth[i]], 1u);
-    L.order[pos] = i;
+    const bool mine = emitted(L, i);
+    uint32_t depth = 0, rank = 0;
+    if (mine) {
+        depth = L.depth[i];
+        rank = atomicAdd(&s_cnt[depth], 1u);
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < kLbMaxLevels; d += blockDim.x)
+        if (s_cnt[d]) s_base[d] = atomicAdd(&L.level_cnt[kLbMaxLevels + d], s_cnt[d]);
+    __syncthreads();
+    if (mine) L.order[s_base[depth] + rank] = i;
 }
 
 __global__ void lb_pack(LbLaunch L) {
