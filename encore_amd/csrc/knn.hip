@@ -231,7 +231,292 @@ __global__ __launch_bounds__(64) void knn_kernel(KnnLaunch L) {
     }  // queries
 }
 
+// ----- large k, radiance only: one query per wave -------------------------
+// For k in the hundreds a per-lane heap needs k x 8 bytes of LDS per lane,
+// which leaves a CU two or three waves.  Here the 64 lanes of a wave serve
+// one query together:
+//  - the tree is walked 64 ranges per step from a stack of pending ranges
+//    (global memory, one region per wave, popped 64 at a time from the top);
+//  - photons that pass the class and distance tests go to a candidate buffer
+//    in LDS;
+//  - once the buffer holds 2k candidates, the k-th smallest squared distance
+//    is found by a radix select over the float bits, the buffer keeps the k
+//    smallest and that distance becomes the strict bound for new candidates
+//    and for pruning, as the heap's maximum does in knn_kernel;
+//  - at the end the k smallest are kept and their power summed in double.
+// Ties at the k-th distance keep the candidates found first.  The stack keeps
+// its entries in non-decreasing tree depth (children are pushed in the order
+// of their parents), so a depth level never holds more than 128 entries and
+// the stack at most 128 x (depth + 1).
+constexpr int kWaveCand = 1024;
+
+struct WaveStack {
+    uint32_t* lo;
+    uint32_t* hi;
+    uint32_t* dp;
+};
+
+// Stack entries written by this wave's earlier steps: loads that bypass the
+// CU's cache, after the stores were made to complete.
+__device__ __forceinline__ uint32_t stack_load(uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
+}
+
+// The k-th smallest key (1-based rank `need` on entry) among s_key[0, n):
+// returns it, and leaves in `need` how many entries equal to it belong to the
+// k smallest.
+__device__ uint32_t select_kth(const uint32_t* s_key, uint32_t n, uint32_t& need, uint32_t* s_hist) {
+    const uint32_t lane = __lane_id();
+    uint32_t prefix = 0, mask = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (uint32_t b = lane; b < 256; b += 64) s_hist[b] = 0;
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t i = lane; i < n; i += 64) {
+            const uint32_t key = s_key[i];
+            if ((key & mask) == prefix) atomicAdd(&s_hist[(key >> shift) & 255u], 1u);
+        }
+        __builtin_amdgcn_wave_barrier();
+        // each lane sums 4 bins; a scan over lanes finds the digit
+        uint32_t c[4], loc = 0;
+        for (int j = 0; j < 4; ++j) {
+            c[j] = s_hist[4 * lane + j];
+            loc += c[j];
+        }
+        uint32_t incl = loc;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = __shfl_up(incl, off, 64);
+            if (lane >= (uint32_t)off) incl += v;
+        }
+        const uint32_t excl = incl - loc;
+        const bool here = excl < need && incl >= need;
+        const uint64_t hm = __ballot(here);
+        if (hm == 0) break;  // need exceeds the keys left: cannot happen for need <= n
+        const uint32_t src = (uint32_t)__ffsll((unsigned long long)hm) - 1;
+        uint32_t digit = 0, before = 0;
+        if (here) {
+            uint32_t run = excl;
+            for (int j = 0; j < 4; ++j) {
+                if (run + c[j] >= need) {
+                    digit = 4 * lane + j;
+                    before = run;
+                    break;
+                }
+                run += c[j];
+            }
+        }
+        digit = __shfl(digit, src, 64);
+        before = __shfl(before, src, 64);
+        need -= before;
+        prefix |= digit << shift;
+        mask |= 255u << shift;
+        __builtin_amdgcn_wave_barrier();
+    }
+    return prefix;
+}
+
+// Keeps, in buffer order, the keys below t and the first `need_eq` equal to t.
+__device__ uint32_t compact_kth(uint32_t* s_key, uint32_t* s_slot, uint32_t n, uint32_t t, uint32_t need_eq) {
+    uint32_t out = 0, eq_seen = 0;
+    for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+        const uint32_t i = i0 + __lane_id();
+        const bool valid = i < n;
+        const uint32_t key = valid ? s_key[i] : 0xFFFFFFFFu;
+        const uint32_t slot = valid ? s_slot[i] : 0u;
+        const bool lt = valid && key < t, eq = valid && key == t;
+        const uint64_t em = __ballot(eq);
+        const bool keep = lt || (eq && eq_seen + lanes_below(em) < need_eq);
+        const uint64_t km = __ballot(keep);
+        __builtin_amdgcn_wave_barrier();
+        if (keep) {
+            const uint32_t pos = out + lanes_below(km);
+            s_key[pos] = key;
+            s_slot[pos] = slot;
+        }
+        __builtin_amdgcn_wave_barrier();
+        out += (uint32_t)__popcll(km);
+        eq_seen += (uint32_t)__popcll(em);
+    }
+    return out;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(64) void knn_wave_kernel(KnnLaunch L, uint32_t stack_cap) {
+    __shared__ uint32_t s_key[kWaveCand];   // squared distance bits (positive floats order as integers)
+    __shared__ uint32_t s_slot[kWaveCand];
+    __shared__ uint32_t s_hist[256];
+    const uint32_t lane = __lane_id();
+    const uint32_t k = L.k;
+    const uint32_t compact_at = min((uint32_t)kWaveCand - 64u, max(2u * k, k + 64u));
+    WaveStack st;
+    st.lo = L.spill + (size_t)blockIdx.x * 3 * stack_cap;
+    st.hi = st.lo + stack_cap;
+    st.dp = st.hi + stack_cap;
+    for (uint32_t q = blockIdx.x; q < L.nq; q += gridDim.x) {
+        const float qx = L.qpos[3 * (size_t)q], qy = L.qpos[3 * (size_t)q + 1], qz = L.qpos[3 * (size_t)q + 2];
+        const uint32_t cls = lookup_class(L.table, L.table_mask, L.qnrm[3 * (size_t)q], L.qnrm[3 * (size_t)q + 1],
+                                          L.qnrm[3 * (size_t)q + 2]);
+        uint32_t cand = 0;
+        bool overflow = false;
+        if (cls != kClassNever && L.n > 0) {
+            float r2 = L.r2;
+            if (lane == 0) {
+                st.lo[0] = 0;
+                st.hi[0] = L.n - 1;
+                st.dp[0] = 0;
+            }
+            uint32_t sp = 1;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            while (sp > 0) {
+                const uint32_t take = min(sp, 64u);
+                const uint32_t base = sp - take;
+                sp = base;
+                bool act = lane < take;
+                uint32_t lo = 0, hi = 0, dpb = 0;
+                if (act) {
+                    lo = stack_load(st.lo + base + lane);
+                    hi = stack_load(st.hi + base + lane);
+                    dpb = stack_load(st.dp + base + lane);
+                    act = __uint_as_float(dpb) < r2;
+                }
+                bool is_c = false, push_n = false, push_f = false;
+                float d2 = 0.0f, dp2 = 0.0f;
+                int im = 0, nlo = 0, nhi = -1, flo = 0, fhi = -1;
+                if (act) {
+                    const int ilo = (int)lo, ihi = (int)hi;
+                    im = (ilo + ihi) >> 1;
+                    const PhotonHot p = L.hot[im];
+                    const float x = qx - p.x, y = qy - p.y, z = qz - p.z;
+                    d2 = (x * x + y * y) + z * z;
+                    is_c = d2 < r2 && (p.meta >> 2) == cls;
+                    const uint32_t axis = p.meta & 3u;
+                    const float pa = axis == 0 ? p.x : (axis == 1 ? p.y : p.z);
+                    const float qa = axis == 0 ? qx : (axis == 1 ? qy : qz);
+                    const float dp = qa - pa;
+                    dp2 = dp * dp;
+                    if (dp > 0.0f) {  // the query's side (PhotonMap.cpp:276-289)
+                        nlo = im + 1; nhi = ihi; flo = ilo; fhi = im - 1;
+                    } else {
+                        nlo = ilo; nhi = im - 1; flo = im + 1; fhi = ihi;
+                    }
+                    push_n = nlo <= nhi;
+                    push_f = dp2 < r2 && flo <= fhi;
+                }
+                const uint32_t m = (uint32_t)im;
+                // candidates, in lane order
+                const uint64_t cm = __ballot(is_c);
+                if (is_c) {
+                    const uint32_t pos = cand + lanes_below(cm);
+                    s_key[pos] = __float_as_uint(d2);
+                    s_slot[pos] = m;
+                }
+                cand += (uint32_t)__popcll(cm);
+                // children, in the order of their parents
+                const uint32_t np = (push_n ? 1u : 0u) + (push_f ? 1u : 0u);
+                uint32_t incl = np;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const uint32_t v = __shfl_up(incl, off, 64);
+                    if (lane >= (uint32_t)off) incl += v;
+                }
+                const uint32_t total = __shfl(incl, 63, 64);
+                if (sp + total > stack_cap) {
+                    overflow = true;  // cannot happen within the depth bound; reported as NaN
+                    break;
+                }
+                uint32_t w = sp + incl - np;
+                if (push_n) {
+                    st.lo[w] = (uint32_t)nlo;
+                    st.hi[w] = (uint32_t)nhi;
+                    st.dp[w] = 0u;
+                    ++w;
+                }
+                if (push_f) {
+                    st.lo[w] = (uint32_t)flo;
+                    st.hi[w] = (uint32_t)fhi;
+                    st.dp[w] = __float_as_uint(dp2);
+                }
+                sp += total;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                if (cand >= compact_at) {
+                    uint32_t need = k;
+                    const uint32_t t = select_kth(s_key, cand, need, s_hist);
+                    cand = compact_kth(s_key, s_slot, cand, t, need);
+                    r2 = __uint_as_float(t);
+                }
+            }
+        }
+        if (cand > k) {
+            uint32_t need = k;
+            const uint32_t t = select_kth(s_key, cand, need, s_hist);
+            cand = compact_kth(s_key, s_slot, cand, t, need);
+        }
+        // radiance over the kept set (GetRadianceEstimate, PhotonMap.cpp:121-164)
+        uint32_t mx = 0;
+        for (uint32_t i = lane; i < cand; i += 64) mx = max(mx, s_key[i]);
+        for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
+        const double rmax = cand ? (double)__uint_as_float(mx) : 0.0;
+        double R = 0.0, G = 0.0, B = 0.0;
+        for (uint32_t i = lane; i < cand; i += 64) {
+            const float4 pw = *reinterpret_cast<const float4*>(L.power + s_slot[i]);
+            const double dd = (double)__uint_as_float(s_key[i]);
+            const double wgt = L.caustic ? (rmax - dd) / dd : 1.0;
+            R += (double)pw.x * wgt;
+            G += (double)pw.y * wgt;
+            B += (double)pw.z * wgt;
+        }
+        R = wave_sum(R);
+        G = wave_sum(G);
+        B = wave_sum(B);
+        if (lane == 0) {
+            float r = 0.0f, g = 0.0f, b = 0.0f;
+            if (cand) {
+                r = (float)R;
+                g = (float)G;
+                b = (float)B;
+                if (L.caustic) {
+                    const float dens = (float)((double)k / ((double)3.14159f * rmax * rmax));
+                    r *= dens;
+                    g *= dens;
+                    b *= dens;
+                }
+            }
+            if (overflow) r = g = b = __int_as_float(0x7fc00000);
+            if (L.out_cnt) L.out_cnt[q] = cand;
+            L.out_rgb[3 * (size_t)q] = r;
+            L.out_rgb[3 * (size_t)q + 1] = g;
+            L.out_rgb[3 * (size_t)q + 2] = b;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 }  // namespace dev
+
+uint32_t knn_wave_stack_cap(uint32_t n) {
+    uint32_t d = 1;  // levels of the implicit balanced tree over n photons
+    while ((n >> d) != 0) ++d;
+    return 128u * (d + 2);
+}
+
+int knn_wave_grid(uint32_t nq, int num_cus) {
+    int per_cu = 0;
+    RQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)dev::knn_wave_kernel, dev::kKnnWave, 0));
+    return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)nq, (int64_t)std::max(1, per_cu) * num_cus));
+}
+
+void launch_knn_wave(const KnnLaunch& L, int grid, uint32_t stack_cap, hipStream_t s) {
+    if (L.nq == 0) return;
+    hipLaunchKernelGGL(dev::knn_wave_kernel, dim3(grid), dim3(dev::kKnnWave), 0, s, L, stack_cap);
+    RQ_HIP(hipGetLastError());
+}
 
 uint32_t knn_spill_words_per_lane(uint32_t n) {
     uint32_t d = 2;

@@ -532,6 +532,15 @@ rq_status rq_photons_balance_host(const float* pos, uint32_t n, uint32_t layout,
     });
 }
 
+// RQ_KNN_LANE=1 keeps the per-lane heap kernel for every k (A/B checks).
+static bool knn_force_lane() {
+    static const bool force = [] {
+        const char* e = std::getenv("RQ_KNN_LANE");
+        return e && e[0] == '1';
+    }();
+    return force;
+}
+
 static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint32_t nq, uint32_t k, float max_dist,
                        uint32_t* idx, float* d2, uint32_t* cnt, float* rgb, uint32_t caustic, hipStream_t s) {
     require(p != nullptr, "null photon map");
@@ -540,9 +549,14 @@ static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint
     if (nq == 0) return;
     rq_device* dev = p->dev;
     RQ_HIP(hipSetDevice(dev->ordinal));
-    const int grid = knn_grid(nq, dev->num_cus, k);
+    // Radiance without lists at large k: one query per wave.
+    const bool wave = idx == nullptr && rgb != nullptr && k >= kKnnWaveMinK && !knn_force_lane();
+    const int grid = wave ? knn_wave_grid(nq, dev->num_cus) : knn_grid(nq, dev->num_cus, k);
     const uint32_t depth = knn_spill_words_per_lane(p->n);
-    uint32_t* scratch = stream_scratch(dev, s, (size_t)grid * 64 * depth * 3 * sizeof(uint32_t));
+    const uint32_t cap = knn_wave_stack_cap(p->n);
+    const size_t scratch_bytes = wave ? (size_t)grid * 3 * cap * sizeof(uint32_t)
+                                      : (size_t)grid * 64 * depth * 3 * sizeof(uint32_t);
+    uint32_t* scratch = stream_scratch(dev, s, scratch_bytes);
     KnnLaunch L{};
     L.hot = p->hot.as<PhotonHot>();
     L.power = p->power.as<PhotonPower>();
@@ -562,7 +576,10 @@ static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint
     L.caustic = caustic;
     L.spill = scratch + 64;
     L.spill_depth = depth;
-    launch_knn(L, grid, s);
+    if (wave)
+        launch_knn_wave(L, grid, cap, s);
+    else
+        launch_knn(L, grid, s);
 }
 
 rq_status rq_knn(rq_photons* p, const float* d_qpos, const float* d_qnormal, uint32_t nq, uint32_t k, float max_dist,

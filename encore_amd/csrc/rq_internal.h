@@ -213,6 +213,13 @@ int knn_grid(uint32_t nq, int num_cus, uint32_t k);
 size_t knn_lds_bytes(uint32_t k);
 constexpr uint32_t kKnnMaxK = 256;
 uint32_t knn_spill_words_per_lane(uint32_t n);
+// Radiance-only gather for large k, one query per wave (knn.hip): the stack
+// capacity (entries per wave; three words each) for n photons, the grid, the
+// launch.  The kernel is used for k >= kKnnWaveMinK when no lists are asked.
+constexpr uint32_t kKnnWaveMinK = 32;
+uint32_t knn_wave_stack_cap(uint32_t n);
+int knn_wave_grid(uint32_t nq, int num_cus);
+void launch_knn_wave(const KnnLaunch& L, int grid, uint32_t stack_cap, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Kernel launch parameters (by value).
