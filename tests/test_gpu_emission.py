@@ -216,3 +216,27 @@ def test_emission_shadow_chain_bound_matches_oracle(device):
     assert paths == o_paths
     _same(g, o)
     sc.destroy()
+
+
+def test_emission_several_and_tilted_lights_kd(device):
+    """Three lights (the draw rand() % 3 picks one per path): the ceiling
+    light, one on the back wall facing +z and one tilted (the rotation branch
+    of AlignPlaneWithNormal, Sampler.cpp:174-192), with different powers."""
+    xyz, idx, mats, pm, lights = scenes.photon_scene(tess=8)
+    lights = lights + [dict(center=(0.5, -0.5, -2.49), normal=(0.0, 0.0, 1.0), length=0.8, width=0.6,
+                            power=(0.2, 0.4, 0.9)),
+                       dict(center=(-1.0, 1.5, 0.5), normal=(0.3, -0.9, 0.1), length=0.5, width=0.7,
+                            power=(2.0, 1.0, 0.5))]
+    kd = oracle.KdTree(xyz, idx)
+    cap = (8000, 12000, 0)
+    o, o_paths = oracle.emit_photons(kd, mats, pm, lights, cap, seed=99)
+    log("case: photon emission, 3 lights")
+    sc = rq.Scene(device, xyz, idx, accel="kd")
+    sc.set_materials(mats, pm)
+    g, paths = _gpu(sc, lights, cap, seed=99, batch_paths=4096)
+    assert paths == o_paths
+    _same(g, o)
+    # every light contributes, with its own power
+    pw = np.unique(o["direct"]["power"][o["direct"]["shadow"] == 0], axis=0)
+    assert len(pw) == 3
+    sc.destroy()
