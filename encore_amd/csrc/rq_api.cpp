@@ -700,6 +700,52 @@ rq_status rq_host_tree_info(const rq_host_tree* t, rq_scene_info* info) {
     });
 }
 
+rq_status rq_host_tree_export_textures(const rq_host_tree* t, const float* xyz, uint32_t nverts,
+                                       const uint32_t* tri_idx, float* cell, float* v0, float* v1, float* v2,
+                                       uint32_t* ref_prim) {
+    return guarded([&] {
+        require(t != nullptr, "null tree");
+        if (t->accel != RQ_ACCEL_KD) throw Error(RQ_ERR_UNSUPPORTED, "the texture encoding is the kd tree's");
+        require((!v0 && !v1 && !v2) || (xyz && tri_idx), "vertex outputs need the triangle list");
+        const KdHost& h = t->kd;
+        float first = 0.0f;  // currentTriangleIndex, a float in Kdtree::buildGPU
+        size_t r = 0;
+        float* vs[3] = {v0, v1, v2};
+        for (size_t i = 0; i < h.nodes.size(); ++i) {
+            const KdNodeG& nd = h.nodes[i];
+            float c[4];
+            if ((nd.word & 3u) != 3u) {
+                float split;
+                std::memcpy(&split, &nd.word, 4);  // SplitPos(): the word, axis bits included
+                c[0] = (float)nd.child;
+                c[1] = split;
+                c[2] = 0.0f;
+                c[3] = (float)-((int)(nd.word & 3u) + 2);
+            } else {
+                const uint32_t count = nd.word >> 2;
+                c[0] = first;
+                c[1] = 0.0f;
+                c[2] = 0.0f;
+                c[3] = (float)(int)count;
+                for (uint32_t j = 0; j < count; ++j, ++r) {
+                    const uint32_t prim = h.leaf_prims[(size_t)nd.child + j];
+                    first += (float)(size_t)1;  // one triangle per TrianglePrim tessellation
+                    if (ref_prim) ref_prim[r] = prim;
+                    for (int k = 0; k < 3; ++k) {
+                        if (!vs[k]) continue;
+                        const uint32_t v = tri_idx[3 * (size_t)prim + k];
+                        require(v < nverts, "triangle index out of range");
+                        for (int a = 0; a < 3; ++a) vs[k][4 * r + a] = xyz[3 * (size_t)v + a];
+                        vs[k][4 * r + 3] = 0.0f;
+                    }
+                }
+            }
+            if (cell)
+                for (int a = 0; a < 4; ++a) cell[4 * i + a] = c[a];
+        }
+    });
+}
+
 rq_status rq_host_tree_copy(const rq_host_tree* t, void* nodes, uint32_t* leaf_prims, float* root_box) {
     return guarded([&] {
         require(t != nullptr, "null tree");

@@ -170,6 +170,9 @@ def load():
     L.rq_host_tree_build.argtypes = [_vp, _u32, _vp, _u32, ctypes.POINTER(SceneDesc), ctypes.POINTER(_vp)]
     L.rq_host_tree_info.argtypes = [_vp, ctypes.POINTER(SceneInfo)]
     L.rq_host_tree_copy.argtypes = [_vp, _vp, _vp, _vp]
+    if hasattr(L, "rq_host_tree_export_textures"):
+        L.rq_host_tree_export_textures.argtypes = [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.rq_host_tree_export_textures.restype = ctypes.c_int
     L.rq_host_tree_destroy.argtypes = [_vp]
     L.rq_host_tree_destroy.restype = None
     for name in ("rq_device_open", "rq_scene_create", "rq_scene_stats", "rq_intersect_closest", "rq_occluded",
@@ -594,6 +597,36 @@ def host_tree(xyz, idx, accel="kd", leaf_size=0, bounds=None, width=0):
         order = np.zeros(info.ntris, np.uint32)
         check(L.rq_host_tree_copy(h, nodes.ctypes.data, order.ctypes.data, box.ctypes.data))
         return dict(nodes=nodes, tri_order=order, info=inf)
+    finally:
+        L.rq_host_tree_destroy(h)
+
+
+def kd_gpu_textures(xyz, idx, bounds=None):
+    """ENCORE's GPU texture encoding (Kdtree::buildGPU) of the engine's kd
+    tree over the triangles: dict(cell_data0 (nnodes, 4), vertex0/1/2
+    (nrefs, 4), ref_prim (nrefs,)).  Host only."""
+    L = load()
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
+    idx = np.ascontiguousarray(idx, np.uint32).reshape(-1)
+    d = _desc("kd", "encore", 0, bounds)
+    h = _vp()
+    check(L.rq_host_tree_build(xyz.ctypes.data, len(xyz) // 3, idx.ctypes.data, len(idx) // 3, ctypes.byref(d),
+                               ctypes.byref(h)))
+    try:
+        info = SceneInfo()
+        check(L.rq_host_tree_info(h, ctypes.byref(info)))
+        out = dict(cell_data0=np.zeros((info.nnodes, 4), np.float32),
+                   vertex0=np.zeros((max(1, info.nrefs), 4), np.float32),
+                   vertex1=np.zeros((max(1, info.nrefs), 4), np.float32),
+                   vertex2=np.zeros((max(1, info.nrefs), 4), np.float32),
+                   ref_prim=np.zeros(max(1, info.nrefs), np.uint32))
+        check(L.rq_host_tree_export_textures(h, xyz.ctypes.data, len(xyz) // 3, idx.ctypes.data,
+                                             out["cell_data0"].ctypes.data, out["vertex0"].ctypes.data,
+                                             out["vertex1"].ctypes.data, out["vertex2"].ctypes.data,
+                                             out["ref_prim"].ctypes.data))
+        for k in ("vertex0", "vertex1", "vertex2", "ref_prim"):
+            out[k] = out[k][: info.nrefs]
+        return out
     finally:
         L.rq_host_tree_destroy(h)
 

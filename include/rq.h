@@ -470,6 +470,19 @@ rq_status rq_host_tree_info(const rq_host_tree* tree, rq_scene_info* info);
 rq_status rq_host_tree_copy(const rq_host_tree* tree, void* nodes, uint32_t* leaf_prims, float* root_box);
 void rq_host_tree_destroy(rq_host_tree* tree);
 
+/* ENCORE's GPU texture encoding of a kd tree (Kdtree::buildGPU,
+ * Kdtree.cpp:876-1000; read by shaders/Kdtree_restart.cg), from a KD host
+ * tree and the triangle list it was built from.  cell_data0: 4 floats per
+ * node in node order -- inner (left child, split word as float, 0,
+ * -(axis + 2)), leaf (index of its first triangle, 0, 0, count); vertex0/1/2:
+ * 4 floats per leaf reference (x, y, z, 0) in node order; ref_prim: the
+ * triangle id of each reference, for the caller's normals and materials
+ * (m_pNormal*Array, m_pMaterial0Array).  Sizes: info.nnodes and info.nrefs.
+ * Any output may be NULL.  Host only. */
+rq_status rq_host_tree_export_textures(const rq_host_tree* tree, const float* xyz, uint32_t nverts,
+                                       const uint32_t* tri_idx, float* cell_data0, float* vertex0, float* vertex1,
+                                       float* vertex2, uint32_t* ref_prim);
+
 /* Test hook: the n-th device allocation from now fails with RQ_ERR_OOM
  * (n = 0 disables).  For exercising the out-of-memory paths. */
 void rq_debug_fail_alloc_after(int64_t n);

@@ -56,6 +56,7 @@ def lib():
             "orc_ph_rand": (None, [ctypes.c_uint64, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS"), _u32p,
                                    _u32, _u32p]),
             "orc_transmit_ray": (_int, [_f32p, _f32p, _f32p, _int, _f32p, _f32p]),
+            "orc_kd_textures": (None, [_vp, _f32p, _f32p, _f32p, _f32p, _u32p]),
             "orc_emit_photons": (None, [_vp, _f32p, _u32p, _f32p, _u32, _u32, _u32p, _f32p, ctypes.c_uint64,
                                         ctypes.c_uint64, _vp, _u32p, ctypes.POINTER(ctypes.c_uint64)]),
         }
@@ -205,6 +206,20 @@ class KdTree:
         rays = _c32(rays).reshape(-1, 8)
         out = np.zeros((rays.shape[0], 4), np.float32)
         lib().orc_kd_intersect(self.h, rays, rays.shape[0], out, nthreads or threads())
+        return out
+
+    def gpu_textures(self):
+        """Kdtree::buildGPU's texture arrays (Kdtree.cpp:876-1000)."""
+        st = self.stats()
+        out = dict(cell_data0=np.zeros((st["nodes"], 4), np.float32),
+                   vertex0=np.zeros((max(1, st["refs"]), 4), np.float32),
+                   vertex1=np.zeros((max(1, st["refs"]), 4), np.float32),
+                   vertex2=np.zeros((max(1, st["refs"]), 4), np.float32),
+                   ref_prim=np.zeros(max(1, st["refs"]), np.uint32))
+        lib().orc_kd_textures(self.h, out["cell_data0"], out["vertex0"], out["vertex1"], out["vertex2"],
+                              out["ref_prim"])
+        for k in ("vertex0", "vertex1", "vertex2", "ref_prim"):
+            out[k] = out[k][: st["refs"]]
         return out
 
     def occluded(self, rays, nthreads=None):

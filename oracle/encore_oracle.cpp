@@ -1250,3 +1250,39 @@ void orc_emit_photons(void* kd, const void* mats, const uint32_t* prim_mat, cons
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Kdtree::buildGPU (Kdtree.cpp:876-1000): the float4 texture encoding of the
+// tree, node by node, and the vertex arrays of the leaves' triangles.
+// ---------------------------------------------------------------------------
+extern "C" void orc_kd_textures(void* h, float* cell, float* v0, float* v1, float* v2, uint32_t* ref_prim) {
+    const KdTree* T = static_cast<const KdTree*>(h);
+    float currentTriangleIndex = 0;
+    size_t r = 0;
+    float* vs[3] = {v0, v1, v2};
+    for (uint32_t i = 0; i < T->used; ++i) {
+        const KdNode& nd = T->nodes[i];
+        float* c = cell + 4 * (size_t)i;
+        if ((nd.word & 3u) != 3u) {
+            c[0] = (float)nd.child;  // leftChild
+            c[1] = u2f(nd.word);     // SplitPos()
+            c[2] = 0;
+            c[3] = (float)-((int)(nd.word & 3u) + 2);
+        } else {
+            const int ntri = (int)(nd.word >> 2);
+            c[0] = currentTriangleIndex;
+            c[1] = 0;
+            c[2] = 0;
+            c[3] = (float)ntri;
+            const std::vector<const TriObj*>& lst = T->leaves[nd.child];
+            for (int ti = 0; ti < ntri; ++ti, ++r) {
+                currentTriangleIndex += (size_t)1;
+                ref_prim[r] = lst[ti]->id;
+                for (int k = 0; k < 3; ++k) {
+                    for (int a = 0; a < 3; ++a) vs[k][4 * r + a] = lst[ti]->v[k]->p.c[a];
+                    vs[k][4 * r + 3] = 0.0f;
+                }
+            }
+        }
+    }
+}
