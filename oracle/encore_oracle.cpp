@@ -21,6 +21,16 @@
 //   photon balance     implicit-tree median invariant; libstdc++ nth_element
 //   photon kNN         brute-force k-nearest set (tie-aware)
 //   radiance estimate  closed form over the returned set
+//   photon emission    the draws' uniformity; the direction samplers against
+//                      Rodrigues turns and the source's matrix product in
+//                      float64; the emission point's square; TransmitRay's
+//                      closed forms; the emission loop on scenes with closed
+//                      forms (black floor, closed white box, a caustic map
+//                      that cannot fill).  Parity with ENCORE's own photons
+//                      is statistical only: it draws from rand().
+//   kd texture export  the encoding rules read off Kdtree::buildGPU (inner
+//                      w = -(axis + 2), leaf x = running triangle count,
+//                      vertex rows) and the engine's independent export
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
