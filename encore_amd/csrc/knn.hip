@@ -348,6 +348,57 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// The end of a one-query-per-wave estimate: keeps the k nearest of the
+// `cand` candidates in LDS (ties at the k-th distance: those found first) and
+// writes the count and the radiance (GetRadianceEstimate,
+// PhotonMap.cpp:121-164), NaN when the walk overflowed.
+__device__ void wave_finish(const KnnLaunch& L, uint32_t q, uint32_t cand, bool overflow, uint32_t* s_key,
+                            uint32_t* s_slot, uint32_t* s_hist) {
+    const uint32_t lane = __lane_id();
+    const uint32_t k = L.k;
+    if (cand > k) {
+        uint32_t need = k;
+        const uint32_t t = select_kth(s_key, cand, need, s_hist);
+        cand = compact_kth(s_key, s_slot, cand, t, need);
+    }
+    // radiance over the kept set (GetRadianceEstimate, PhotonMap.cpp:121-164)
+    uint32_t mx = 0;
+    for (uint32_t i = lane; i < cand; i += 64) mx = max(mx, s_key[i]);
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
+    const double rmax = cand ? (double)__uint_as_float(mx) : 0.0;
+    double R = 0.0, G = 0.0, B = 0.0;
+    for (uint32_t i = lane; i < cand; i += 64) {
+        const float4 pw = *reinterpret_cast<const float4*>(L.power + s_slot[i]);
+        const double dd = (double)__uint_as_float(s_key[i]);
+        const double wgt = L.caustic ? (rmax - dd) / dd : 1.0;
+        R += (double)pw.x * wgt;
+        G += (double)pw.y * wgt;
+        B += (double)pw.z * wgt;
+    }
+    R = wave_sum(R);
+    G = wave_sum(G);
+    B = wave_sum(B);
+    if (lane == 0) {
+        float r = 0.0f, g = 0.0f, b = 0.0f;
+        if (cand) {
+            r = (float)R;
+            g = (float)G;
+            b = (float)B;
+            if (L.caustic) {
+                const float dens = (float)((double)k / ((double)3.14159f * rmax * rmax));
+                r *= dens;
+                g *= dens;
+                b *= dens;
+            }
+        }
+        if (overflow) r = g = b = __int_as_float(0x7fc00000);
+        if (L.out_cnt) L.out_cnt[q] = cand;
+        L.out_rgb[3 * (size_t)q] = r;
+        L.out_rgb[3 * (size_t)q + 1] = g;
+        L.out_rgb[3 * (size_t)q + 2] = b;
+    }
+}
+
 __global__ __launch_bounds__(64) void knn_wave_kernel(KnnLaunch L, uint32_t stack_cap) {
     __shared__ uint32_t s_key[kWaveCand];   // squared distance bits (positive floats order as integers)
     __shared__ uint32_t s_slot[kWaveCand];
@@ -453,47 +504,142 @@ __global__ __launch_bounds__(64) void knn_wave_kernel(KnnLaunch L, uint32_t stac
                 }
             }
         }
-        if (cand > k) {
-            uint32_t need = k;
-            const uint32_t t = select_kth(s_key, cand, need, s_hist);
-            cand = compact_kth(s_key, s_slot, cand, t, need);
-        }
-        // radiance over the kept set (GetRadianceEstimate, PhotonMap.cpp:121-164)
-        uint32_t mx = 0;
-        for (uint32_t i = lane; i < cand; i += 64) mx = max(mx, s_key[i]);
-        for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
-        const double rmax = cand ? (double)__uint_as_float(mx) : 0.0;
-        double R = 0.0, G = 0.0, B = 0.0;
-        for (uint32_t i = lane; i < cand; i += 64) {
-            const float4 pw = *reinterpret_cast<const float4*>(L.power + s_slot[i]);
-            const double dd = (double)__uint_as_float(s_key[i]);
-            const double wgt = L.caustic ? (rmax - dd) / dd : 1.0;
-            R += (double)pw.x * wgt;
-            G += (double)pw.y * wgt;
-            B += (double)pw.z * wgt;
-        }
-        R = wave_sum(R);
-        G = wave_sum(G);
-        B = wave_sum(B);
-        if (lane == 0) {
-            float r = 0.0f, g = 0.0f, b = 0.0f;
-            if (cand) {
-                r = (float)R;
-                g = (float)G;
-                b = (float)B;
-                if (L.caustic) {
-                    const float dens = (float)((double)k / ((double)3.14159f * rmax * rmax));
-                    r *= dens;
-                    g *= dens;
-                    b *= dens;
+        wave_finish(L, q, cand, overflow, s_key, s_slot, s_hist);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ----- large k, radiance only, from the photon cell grid -------------------
+// One query per wave, as knn_wave_kernel, but the candidates come from the
+// cells within reach of the query instead of a tree walk:
+//  - lane j takes cell (j & 3, (j >> 2) & 3, j >> 4) of the 4 x 4 x 4 block
+//    that covers [q - reach, q + reach] (the cell edge is the reach, so no
+//    axis spans more than 4 cells), drops it when it is outside the grid or
+//    farther than the radius, and looks up its photon range;
+//  - the cells are ordered nearest first, and their photons are read 64 at a
+//    time as one concatenated range;
+//  - candidates, compaction at 2k and the final selection are those of
+//    knn_wave_kernel; once the k-th distance has shrunk below the distance of
+//    the next cell, that cell and every later one are skipped.
+// `near` is a lower bound on the float d2 of every photon of a cell: the
+// distance to the cell's box, widened for the rounding of cell assignment,
+// squared and lowered by 2^-20 for the rounding of the float d2.
+__device__ __forceinline__ uint2 cell_lookup(const PhotonGridView& G, uint64_t key) {
+    uint32_t h = cell_hash(key, G.cell_mask);
+    for (uint32_t probe = 0; probe <= G.cell_mask; ++probe) {
+        const uint4 e = G.cells[h];
+        const uint64_t ek = (uint64_t)e.y << 32 | e.x;
+        if (ek == key) return make_uint2(e.z, e.w);
+        if (ek == ~0ull) break;
+        h = (h + 1) & G.cell_mask;
+    }
+    return make_uint2(0u, 0u);
+}
+
+__global__ __launch_bounds__(64) void knn_cell_kernel(KnnLaunch L) {
+    __shared__ uint32_t s_key[kWaveCand];
+    __shared__ uint32_t s_slot[kWaveCand];
+    __shared__ uint32_t s_hist[256];
+    __shared__ uint32_t s_first[64], s_pre[64];
+    __shared__ double s_near[64];
+    const PhotonGridView& G = L.grid;
+    const uint32_t lane = __lane_id();
+    const uint32_t k = L.k;
+    const uint32_t compact_at = min((uint32_t)kWaveCand - 64u, max(2u * k, k + 64u));
+    for (uint32_t q = blockIdx.x; q < L.nq; q += gridDim.x) {
+        const float qx = L.qpos[3 * (size_t)q], qy = L.qpos[3 * (size_t)q + 1], qz = L.qpos[3 * (size_t)q + 2];
+        const uint32_t cls = lookup_class(L.table, L.table_mask, L.qnrm[3 * (size_t)q], L.qnrm[3 * (size_t)q + 1],
+                                          L.qnrm[3 * (size_t)q + 2]);
+        uint32_t cand = 0;
+        bool overflow = false;
+        if (cls != kClassNever) {
+            float r2 = L.r2;
+            const double qd[3] = {(double)qx, (double)qy, (double)qz};
+            const uint32_t off[3] = {lane & 3u, (lane >> 2) & 3u, lane >> 4};
+            bool v = true;
+            double near2 = 0.0;
+            uint64_t c[3];
+            for (int a = 0; a < 3; ++a) {
+                // widening for assignment and for the rounding of these sums
+                const double m = G.cell * 0x1p-20 + (fabs(G.origin[a]) + fabs(qd[a])) * 0x1p-48;
+                const double w = G.reach + fabs(qd[a]) * 0x1p-40;
+                const int64_t lo = grid_coord(qd[a] - w, G.origin[a], G.inv_cell);
+                const int64_t hi = grid_coord(qd[a] + w, G.origin[a], G.inv_cell);
+                overflow |= hi - lo > 3;  // only for |q| beyond 2^38 cells
+                const int64_t ca = lo + (int64_t)off[a];
+                v = v && ca <= hi && ca >= 0 && ca < (int64_t)G.dims[a];
+                c[a] = (uint64_t)max(ca, (int64_t)0);
+                const double blo = G.origin[a] + (double)ca * G.cell - m;
+                const double bhi = G.origin[a] + (double)(ca + 1) * G.cell + m;
+                const double d = fmax(fmax(blo - qd[a], qd[a] - bhi), 0.0);
+                near2 += d * d;
+            }
+            near2 *= 1.0 - 0x1p-20;
+            v = v && near2 < (double)r2 && !overflow;
+            uint2 range = make_uint2(0u, 0u);
+            if (v) {
+                range = cell_lookup(G, cell_key(c[0], c[1], c[2]));
+                v = range.y > 0;
+            }
+            // nearest first: rank among the kept cells by (near2, lane)
+            const uint64_t vm = __ballot(v);
+            uint32_t rank = 0;
+            for (uint64_t rest = vm; rest; rest &= rest - 1) {
+                const int j = __ffsll((unsigned long long)rest) - 1;
+                const double o = __shfl(near2, j, 64);
+                rank += (o < near2 || (o == near2 && (uint32_t)j < lane)) ? 1u : 0u;
+            }
+            const uint32_t ncell = (uint32_t)__popcll(vm);
+            if (v) {
+                s_first[rank] = range.x;
+                s_pre[rank] = range.y;
+                s_near[rank] = near2;
+            }
+            __builtin_amdgcn_wave_barrier();
+            // photon counts -> exclusive prefix, in rank order
+            const uint32_t cnt = lane < ncell ? s_pre[lane] : 0u;
+            uint32_t incl = cnt;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t t = __shfl_up(incl, o, 64);
+                if (lane >= (uint32_t)o) incl += t;
+            }
+            const uint32_t total = __shfl(incl, 63, 64);
+            __builtin_amdgcn_wave_barrier();
+            if (lane < ncell) s_pre[lane] = incl - cnt;
+            __builtin_amdgcn_wave_barrier();
+            uint32_t ci = 0, c0 = 0;  // cells of this lane's and of the chunk's first element
+            for (uint32_t e0 = 0; e0 < total; e0 += 64) {
+                while (c0 + 1 < ncell && s_pre[c0 + 1] <= e0) ++c0;
+                if (s_near[c0] >= (double)r2) break;  // this cell and all later ones are too far
+                const uint32_t e = e0 + lane;
+                bool is_c = false;
+                float d2 = 0.0f;
+                uint32_t gi = 0;
+                if (e < total) {
+                    while (ci + 1 < ncell && s_pre[ci + 1] <= e) ++ci;
+                    gi = s_first[ci] + (e - s_pre[ci]);
+                    const float4 p = G.pos[gi];
+                    const float x = qx - p.x, y = qy - p.y, z = qz - p.z;
+                    d2 = (x * x + y * y) + z * z;
+                    is_c = d2 < r2 && __float_as_uint(p.w) == cls;
+                }
+                const uint64_t cm = __ballot(is_c);
+                if (is_c) {
+                    const uint32_t pos = cand + lanes_below(cm);
+                    s_key[pos] = __float_as_uint(d2);
+                    s_slot[pos] = G.slot[gi];
+                }
+                cand += (uint32_t)__popcll(cm);
+                __builtin_amdgcn_wave_barrier();
+                if (cand >= compact_at) {
+                    uint32_t need = k;
+                    const uint32_t t = select_kth(s_key, cand, need, s_hist);
+                    cand = compact_kth(s_key, s_slot, cand, t, need);
+                    r2 = __uint_as_float(t);
                 }
             }
-            if (overflow) r = g = b = __int_as_float(0x7fc00000);
-            if (L.out_cnt) L.out_cnt[q] = cand;
-            L.out_rgb[3 * (size_t)q] = r;
-            L.out_rgb[3 * (size_t)q + 1] = g;
-            L.out_rgb[3 * (size_t)q + 2] = b;
         }
+        wave_finish(L, q, cand, overflow, s_key, s_slot, s_hist);
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -515,6 +661,18 @@ int knn_wave_grid(uint32_t nq, int num_cus) {
 void launch_knn_wave(const KnnLaunch& L, int grid, uint32_t stack_cap, hipStream_t s) {
     if (L.nq == 0) return;
     hipLaunchKernelGGL(dev::knn_wave_kernel, dim3(grid), dim3(dev::kKnnWave), 0, s, L, stack_cap);
+    RQ_HIP(hipGetLastError());
+}
+
+int knn_cell_grid(uint32_t nq, int num_cus) {
+    int per_cu = 0;
+    RQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)dev::knn_cell_kernel, dev::kKnnWave, 0));
+    return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)nq, (int64_t)std::max(1, per_cu) * num_cus));
+}
+
+void launch_knn_cells(const KnnLaunch& L, int grid, hipStream_t s) {
+    if (L.nq == 0) return;
+    hipLaunchKernelGGL(dev::knn_cell_kernel, dim3(grid), dim3(dev::kKnnWave), 0, s, L);
     RQ_HIP(hipGetLastError());
 }
 

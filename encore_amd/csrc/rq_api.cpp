@@ -497,6 +497,21 @@ rq_status rq_photons_create(rq_device* dev, const float* pos, const float* norma
             p->perm.upload(h.perm.data(), n * sizeof(uint32_t));
         }
         p->table.upload(h.table.data(), p->table.bytes);
+        // Photons a query can select, and their bounds (the cell grid's).
+        bool any = false;
+        for (uint32_t i = 0; i < n; ++i) {
+            const PhotonHot& ph = h.hot[i];
+            if (!std::isfinite(ph.x) || !std::isfinite(ph.y) || !std::isfinite(ph.z) ||
+                (ph.meta >> 2) == kClassNever)
+                continue;
+            const float c[3] = {ph.x, ph.y, ph.z};
+            for (int a = 0; a < 3; ++a) {
+                p->bmin[a] = any ? std::min(p->bmin[a], c[a]) : c[a];
+                p->bmax[a] = any ? std::max(p->bmax[a], c[a]) : c[a];
+            }
+            any = true;
+            ++p->nvalid;
+        }
         p->perm_host = std::move(h.perm);
         p->axis_host = std::move(h.axis);
         *out = p.release();
@@ -532,13 +547,21 @@ rq_status rq_photons_balance_host(const float* pos, uint32_t n, uint32_t layout,
     });
 }
 
-// RQ_KNN_LANE=1 keeps the per-lane heap kernel for every k (A/B checks).
+// A/B switches: RQ_KNN_LANE=1 keeps the per-lane heap kernel for every k;
+// RQ_KNN_CELLS=0 keeps the tree walk for large-k radiance.
 static bool knn_force_lane() {
     static const bool force = [] {
         const char* e = std::getenv("RQ_KNN_LANE");
         return e && e[0] == '1';
     }();
     return force;
+}
+static bool knn_cells_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("RQ_KNN_CELLS");
+        return !(e && e[0] == '0');
+    }();
+    return on;
 }
 
 static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint32_t nq, uint32_t k, float max_dist,
@@ -549,8 +572,32 @@ static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint
     if (nq == 0) return;
     rq_device* dev = p->dev;
     RQ_HIP(hipSetDevice(dev->ordinal));
-    // Radiance without lists at large k: one query per wave.
+    // Radiance without lists at large k: one query per wave, from the photon
+    // cell grid when one suits the radius, else walking the tree.
     const bool wave = idx == nullptr && rgb != nullptr && k >= kKnnWaveMinK && !knn_force_lane();
+    const float r2 = max_dist * max_dist;  // fl32(maxDistance * maxDistance) (PhotonMap.cpp:89)
+    if (wave && knn_cells_enabled()) {
+        std::lock_guard<std::mutex> g(p->grid_mu);
+        if (const PhotonGrid* pg = photon_grid_for(p, r2, s)) {
+            KnnLaunch L{};
+            L.power = p->power.as<PhotonPower>();
+            L.perm = p->perm.as<uint32_t>();
+            L.table = p->table.as<uint32_t>();
+            L.table_mask = p->table_mask;
+            L.n = p->n;
+            L.qpos = qpos;
+            L.qnrm = qnrm;
+            L.nq = nq;
+            L.k = k;
+            L.r2 = r2;
+            L.out_cnt = cnt;
+            L.out_rgb = rgb;
+            L.caustic = caustic;
+            L.grid = pg->view;
+            launch_knn_cells(L, knn_cell_grid(nq, dev->num_cus), s);
+            return;
+        }
+    }
     const int grid = wave ? knn_wave_grid(nq, dev->num_cus) : knn_grid(nq, dev->num_cus, k);
     const uint32_t depth = knn_spill_words_per_lane(p->n);
     const uint32_t cap = knn_wave_stack_cap(p->n);
@@ -568,7 +615,7 @@ static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint
     L.qnrm = qnrm;
     L.nq = nq;
     L.k = k;
-    L.r2 = max_dist * max_dist;  // fl32(maxDistance * maxDistance) (PhotonMap.cpp:89)
+    L.r2 = r2;
     L.out_idx = idx;
     L.out_d2 = d2;
     L.out_cnt = cnt;

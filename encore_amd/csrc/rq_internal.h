@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
@@ -187,6 +188,45 @@ __host__ __device__ inline uint32_t normal_hash(uint32_t x, uint32_t y, uint32_t
     return h;
 }
 
+// Photon cell grid for radiance queries (photon_grid.hip).  The photons of a
+// map that can ever be selected (finite position, a normal class) are
+// bucketed into cubic cells whose edge is the query reach, stored in Morton
+// order of their cells; an open-addressing table maps a cell to its range.
+// Build and queries compute a cell coordinate with the same double
+// arithmetic, grid_coord, which is monotonic in x.
+struct PhotonGridView {
+    const float4* pos;     // grid order: x, y, z and the normal class (bits)
+    const uint32_t* slot;  // grid order -> balanced-tree slot (power, perm)
+    const uint4* cells;    // (key low, key high, first, count); key ~0 = empty
+    uint32_t cell_mask;    // table entries - 1
+    uint32_t dims[3];      // cells per axis, each below 2^21
+    double origin[3];      // the photons' lower bounds
+    double cell, inv_cell;
+    double reach;          // every photon with float d2 < r2 lies within it
+};
+
+__host__ __device__ inline int64_t grid_coord(double x, double origin, double inv_cell) {
+    return (int64_t)floor((x - origin) * inv_cell);
+}
+
+__host__ __device__ inline uint64_t grid_spread21(uint64_t v) {
+    v &= 0x1FFFFFull;
+    v = (v | (v << 32)) & 0x1F00000000FFFFull;
+    v = (v | (v << 16)) & 0x1F0000FF0000FFull;
+    v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+    v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+__host__ __device__ inline uint64_t cell_key(uint64_t x, uint64_t y, uint64_t z) {
+    return (grid_spread21(x) << 2) | (grid_spread21(y) << 1) | grid_spread21(z);
+}
+
+__host__ __device__ inline uint32_t cell_hash(uint64_t key, uint32_t mask) {
+    return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 32) & mask;
+}
+
 struct KnnLaunch {
     const PhotonHot* hot;
     const PhotonPower* power;
@@ -206,6 +246,7 @@ struct KnnLaunch {
     uint32_t caustic;
     uint32_t* spill;     // per-lane stack spill
     uint32_t spill_depth;
+    PhotonGridView grid;  // knn_cell_kernel only
 };
 
 void launch_knn(const KnnLaunch& L, int grid, hipStream_t s);
@@ -218,6 +259,9 @@ uint32_t knn_spill_words_per_lane(uint32_t n);
 // launch.  The kernel is used for k >= kKnnWaveMinK when no lists are asked.
 constexpr uint32_t kKnnWaveMinK = 32;
 uint32_t knn_wave_stack_cap(uint32_t n);
+// Radiance without lists from the photon cell grid, one query per wave.
+int knn_cell_grid(uint32_t nq, int num_cus);
+void launch_knn_cells(const KnnLaunch& L, int grid, hipStream_t s);
 int knn_wave_grid(uint32_t nq, int num_cus);
 void launch_knn_wave(const KnnLaunch& L, int grid, uint32_t stack_cap, hipStream_t s);
 
@@ -404,6 +448,18 @@ void launch_kd(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int gri
 void launch_count(const TraceLaunch& L, uint32_t accel, uint32_t predicate, TraceKind kind, uint32_t width, int grid,
                   hipStream_t s);
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
+
+// The cell grid of a photon map for squared radius r2 (photon_grid.hip).
+struct PhotonGrid {
+    uint32_t r2_bits = 0;
+    DevBuf pos, slot, cells;
+    PhotonGridView view{};
+};
+// Makes (or keeps) the grid for r2 on stream s and returns it, or null when
+// a grid does not suit this radius (not finite or not positive, more than
+// 2^21 - 1 cells along an axis, or fewer than 4 along every axis).  The
+// caller holds p->grid_mu until the queries that read the grid are queued.
+const PhotonGrid* photon_grid_for(rq_photons* p, float r2, hipStream_t s);
 int occupancy_blocks_per_cu(uint32_t accel, uint32_t predicate, TraceKind kind, uint32_t width,
                             const KernelConfig& kc);
 // Dwords per traversal stack entry: BVH2 1 (ref), BVH4 2 (ref, entry
@@ -438,6 +494,13 @@ struct rq_photons {
     rq::DevBuf hot, power, perm, table;
     std::vector<uint32_t> perm_host;
     std::vector<uint8_t> axis_host;
+    // Photons that a query can select (finite position, a normal class) and
+    // their bounds; the cell grid for the last squared radius it was built
+    // for (made on first use by a radiance query).
+    uint32_t nvalid = 0;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+    std::mutex grid_mu;
+    std::unique_ptr<rq::PhotonGrid> grid;
 };
 
 struct rq_scene {

@@ -289,7 +289,9 @@ typedef enum rq_photon_layout {
 /* Build a balanced photon kd-tree over n photons given in insertion order
  * (PhotonMap::AddPhoton then BalanceAndConstruct, PhotonMap.cpp:46-53,
  * 167-221) and upload it.  Host arrays of 3*n floats: position, surface
- * normal, power (power may be NULL when only k-NN is needed). */
+ * normal, power (power may be NULL when only k-NN is needed).  Positions are
+ * finite, as ENCORE's hit points are: a balanced tree over NaN or infinite
+ * coordinates is not ordered, and gathers over it are not defined. */
 rq_status rq_photons_create(rq_device* dev, const float* pos, const float* normal, const float* power,
                             uint32_t n, uint32_t layout, rq_photons** out);
 void rq_photons_destroy(rq_photons* photons);

@@ -1,0 +1,130 @@
+"""Radiance estimates without neighbour lists at k >= 32 read their
+candidates from the photon cell grid (photon_grid.hip, knn_cell_kernel).
+The grid must never miss a photon in range: counts equal brute force where
+fewer than k photons are in range, whatever the offset of the map, and
+photons that no query can select (NaN normals) stay out.  Estimates equal
+the per-lane heap kernel's (asked with lists, which never uses the grid) up
+to photons tied with the k-th distance, also when one map is queried with
+alternating radii (the grid is rebuilt) and when the radius is too large for
+a grid (the tree walk serves)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from encore_amd import rq
+from helpers import log
+from test_gpu_knn_wave import _radiance
+
+pytestmark = pytest.mark.gpu
+
+
+def _brute_count(P, N, Q, QN, r):
+    """Photons passing the class and distance tests, per query: float32
+    (x*x + y*y) + z*z < fl(r*r); normals equal as floats (-0 matches 0,
+    NaN matches nothing)."""
+    r2 = np.float32(r) * np.float32(r)
+    out = np.zeros(len(Q), np.int64)
+    for i in range(len(Q)):
+        d = (Q[i] - P).astype(np.float32)
+        with np.errstate(invalid="ignore"):
+            d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+            out[i] = np.count_nonzero((d2 < r2) & np.all(N == QN[i], axis=1))
+    return out
+
+
+def _surface_photons(rng, n, offset):
+    """Photons on three axis planes of a unit box (normals +x, +y, +z) and
+    queries on the same planes, shifted by `offset`."""
+    P = rng.uniform(0.0, 1.0, (n, 3)).astype(np.float32)
+    N = np.zeros((n, 3), np.float32)
+    ax = rng.integers(0, 3, n)
+    P[np.arange(n), ax] = 0.0
+    N[np.arange(n), ax] = 1.0
+    W = rng.uniform(0.1, 1.0, (n, 3)).astype(np.float32)
+    nq = 1500
+    Q = rng.uniform(-0.05, 1.05, (nq, 3)).astype(np.float32)
+    QN = np.zeros((nq, 3), np.float32)
+    qa = rng.integers(0, 3, nq)
+    Q[np.arange(nq), qa] = 0.0
+    QN[np.arange(nq), qa] = 1.0
+    off = np.float32(offset)
+    return P + off, N, W, Q + off, QN
+
+
+@pytest.mark.parametrize("offset", [0.0, -3.5, 1000.0])
+def test_cells_find_every_photon_in_range(device, offset):
+    rng = np.random.default_rng(11)
+    P, N, W, Q, QN = _surface_photons(rng, 40000, offset)
+    # photons exactly on cell faces: copies of queries moved by r along an axis
+    r = 0.03
+    edge = Q[:200].copy()
+    edge[:, 1] += np.float32(r)
+    P = np.concatenate([P, edge, Q[200:260]])
+    N = np.concatenate([N, QN[:200], QN[200:260]])
+    W = np.concatenate([W, np.ones((260, 3), np.float32)])
+    pm = rq.PhotonMap(device, P, N, W)
+    k = 256
+    rgb, cnt = _radiance(pm, Q, QN, k, r, False, False)
+    want = _brute_count(P, N, Q, QN, r)
+    log("case: cells offset %g: mean count %.1f, max %d" % (offset, want.mean(), want.max()))
+    assert want.max() < k and want.mean() > 10
+    assert np.array_equal(cnt, want)
+    heap_rgb, heap_cnt = _radiance(pm, Q, QN, k, r, True, False)
+    assert np.array_equal(cnt, heap_cnt)
+    np.testing.assert_allclose(rgb, heap_rgb, rtol=1e-6, atol=0)
+    pm.destroy()
+
+
+@pytest.mark.parametrize("k", [32, 100, 256])
+@pytest.mark.parametrize("caustic", [False, True])
+def test_cells_saturated_equal_heap_kernel_and_oracle(device, k, caustic):
+    rng = np.random.default_rng(12)
+    P, N, W, Q, QN = _surface_photons(rng, 200000, 0.0)
+    pm = rq.PhotonMap(device, P, N, W)
+    r = 0.05  # about 2.5x k = 100 photons in range on a plane
+    rgb, cnt = _radiance(pm, Q, QN, k, r, False, caustic)
+    heap_rgb, heap_cnt = _radiance(pm, Q, QN, k, r, True, caustic)
+    assert np.array_equal(cnt, heap_cnt)
+    assert not np.isnan(rgb).any()
+    same = np.all(rgb.view(np.uint32) == heap_rgb.view(np.uint32), axis=1)
+    log("case: cells k=%d caustic=%s: %.3f full, %d of %d bit-equal" % (k, caustic, (cnt == k).mean(), same.sum(),
+                                                                         len(same)))
+    assert same.mean() >= 0.99
+    np.testing.assert_allclose(rgb, heap_rgb, rtol=1e-5, atol=1e-12)
+    ref = oracle.PhotonMap(P, N, W).radiance(Q[:500], QN[:500], k, r, caustic=caustic)
+    np.testing.assert_allclose(rgb[:500], ref, rtol=1e-5, atol=1e-12)
+    pm.destroy()
+
+
+def test_cells_skip_unselectable_photons(device):
+    """Photon positions are finite (ENCORE stores hit points): a balanced
+    tree over NaN coordinates is not ordered, so only the normals vary here."""
+    rng = np.random.default_rng(13)
+    P, N, W, Q, QN = _surface_photons(rng, 20000, 0.0)
+    N[2::13] = np.nan
+    N[5::19] = [0.6, 0.8, 0.0]  # a normal no query has
+    for far in (False, True):
+        if far:
+            P[3::17] = np.float32(1e30)  # 2^21 cells along an axis: no grid, the tree walk serves
+        pm = rq.PhotonMap(device, P, N, W)
+        rgb, cnt = _radiance(pm, Q, QN, 64, 0.04, False, False)
+        assert np.array_equal(cnt, np.minimum(_brute_count(P, N, Q, QN, 0.04), 64))
+        heap_rgb, _ = _radiance(pm, Q, QN, 64, 0.04, True, False)
+        np.testing.assert_allclose(rgb, heap_rgb, rtol=1e-5)
+        pm.destroy()
+
+
+def test_cells_radius_changes_and_large_radius(device):
+    rng = np.random.default_rng(14)
+    P, N, W, Q, QN = _surface_photons(rng, 60000, 0.0)
+    pm = rq.PhotonMap(device, P, N, W)
+    for r in (0.03, 0.06, 0.03, 0.5, 2.0, 0.03):
+        rgb, cnt = _radiance(pm, Q, QN, 64, r, False, False)
+        heap_rgb, heap_cnt = _radiance(pm, Q, QN, 64, r, True, False)
+        log("case: radius %g: mean count %.1f" % (r, cnt.mean()))
+        assert np.array_equal(cnt, heap_cnt)
+        same = np.all(rgb.view(np.uint32) == heap_rgb.view(np.uint32), axis=1)
+        assert same.mean() >= 0.99
+        np.testing.assert_allclose(rgb, heap_rgb, rtol=1e-5, atol=1e-12)
+    pm.destroy()
