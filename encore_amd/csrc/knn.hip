@@ -15,8 +15,10 @@
 // different banks, and a short ring stack of pending ranges (LDS, spilling to
 // HBM).  Results are written nearest first.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "rq_internal.h"
 
@@ -536,17 +538,24 @@ __device__ __forceinline__ uint2 cell_lookup(const PhotonGridView& G, uint64_t k
     return make_uint2(0u, 0u);
 }
 
+// Candidates: compaction at max(2k, k + 64) <= 512 for k <= 256, plus one
+// chunk of 64.
+constexpr int kCellCand = 2 * (int)kKnnMaxK + 64;
+
 __global__ __launch_bounds__(64) void knn_cell_kernel(KnnLaunch L) {
-    __shared__ uint32_t s_key[kWaveCand];
-    __shared__ uint32_t s_slot[kWaveCand];
+    __shared__ uint32_t s_key[kCellCand];
+    __shared__ uint32_t s_slot[kCellCand];
     __shared__ uint32_t s_hist[256];
     __shared__ uint32_t s_first[64], s_pre[64];
     __shared__ double s_near[64];
     const PhotonGridView& G = L.grid;
     const uint32_t lane = __lane_id();
     const uint32_t k = L.k;
-    const uint32_t compact_at = min((uint32_t)kWaveCand - 64u, max(2u * k, k + 64u));
-    for (uint32_t q = blockIdx.x; q < L.nq; q += gridDim.x) {
+    const uint32_t compact_at = min((uint32_t)kCellCand - 64u, max(2u * k, k + 64u));
+    // Grid-stride over the queries in cell order: the waves running at one
+// time hold a window of consecutive queries, which share cells.
+for (uint32_t qi = blockIdx.x; qi < L.nq; qi += gridDim.x) {
+        const uint32_t q = L.order ? L.order[qi] : qi;
         const float qx = L.qpos[3 * (size_t)q], qy = L.qpos[3 * (size_t)q + 1], qz = L.qpos[3 * (size_t)q + 2];
         const uint32_t cls = lookup_class(L.table, L.table_mask, L.qnrm[3 * (size_t)q], L.qnrm[3 * (size_t)q + 1],
                                           L.qnrm[3 * (size_t)q + 2]);
@@ -644,7 +653,48 @@ __global__ __launch_bounds__(64) void knn_cell_kernel(KnnLaunch L) {
     }
 }
 
+__global__ void knn_cell_keys(KnnLaunch L, unsigned long long* keys, uint32_t* ids) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.nq) return;
+    const PhotonGridView& G = L.grid;
+    unsigned long long key = ~0ull;
+    const float x[3] = {L.qpos[3 * (size_t)i], L.qpos[3 * (size_t)i + 1], L.qpos[3 * (size_t)i + 2]};
+    if (isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2])) {
+        uint64_t c[3];
+        for (int a = 0; a < 3; ++a) {
+            const int64_t v = grid_coord((double)x[a], G.origin[a], G.inv_cell);
+            c[a] = (uint64_t)min(max(v, (int64_t)0), (int64_t)G.dims[a] - 1);
+        }
+        key = cell_key(c[0], c[1], c[2]);
+    }
+    keys[i] = key;
+    ids[i] = i;
+}
+
 }  // namespace dev
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t knn_cell_order_bytes(uint32_t nq) {
+    size_t tmp = 0;
+    RQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (unsigned long long*)nullptr,
+                                              (unsigned long long*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                              (int)nq, 0, 64));
+    return 2 * align256((size_t)nq * 8) + align256((size_t)nq * 4) + align256(std::max<size_t>(tmp, 16));
+}
+
+void knn_cell_order(const KnnLaunch& L, uint32_t* order, void* scratch, size_t scratch_bytes, hipStream_t s) {
+    const uint32_t nq = L.nq;
+    char* w = static_cast<char*>(scratch);
+    auto* keys = reinterpret_cast<unsigned long long*>(w);
+    auto* skeys = reinterpret_cast<unsigned long long*>(w + align256((size_t)nq * 8));
+    auto* ids = reinterpret_cast<uint32_t*>(w + 2 * align256((size_t)nq * 8));
+    void* tmp = w + 2 * align256((size_t)nq * 8) + align256((size_t)nq * 4);
+    size_t tmp_bytes = scratch_bytes - (size_t)(static_cast<char*>(tmp) - w);
+    hipLaunchKernelGGL(dev::knn_cell_keys, dim3((nq + 255) / 256), dim3(256), 0, s, L, keys, ids);
+    RQ_HIP(hipGetLastError());
+    RQ_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, skeys, ids, order, (int)nq, 0, 64, s));
+}
 
 uint32_t knn_wave_stack_cap(uint32_t n) {
     uint32_t d = 1;  // levels of the implicit balanced tree over n photons
@@ -667,6 +717,12 @@ void launch_knn_wave(const KnnLaunch& L, int grid, uint32_t stack_cap, hipStream
 int knn_cell_grid(uint32_t nq, int num_cus) {
     int per_cu = 0;
     RQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)dev::knn_cell_kernel, dev::kKnnWave, 0));
+    // Five waves per SIMD: the 4M-query final gather ran 146 M rays/s with
+    // 20 one-wave blocks per CU and 113 M with 24 (more waves evict each
+    // other's cells).  RQ_KNN_CELL_WAVES lowers the count for sweeps.
+    int cap = 20;
+    if (const char* e = std::getenv("RQ_KNN_CELL_WAVES")) cap = std::atoi(e);
+    per_cu = std::max(1, std::min(per_cu, cap));
     return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)nq, (int64_t)std::max(1, per_cu) * num_cus));
 }
 

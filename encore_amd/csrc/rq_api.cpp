@@ -556,6 +556,16 @@ static bool knn_force_lane() {
     }();
     return force;
 }
+// Queries of a cell-grid gather are sorted by cell from this many on
+// (RQ_KNN_SORT=0 keeps the caller's order).
+constexpr uint32_t kKnnSortMin = 16384;
+static bool knn_cells_sorted() {
+    static const bool on = [] {
+        const char* e = std::getenv("RQ_KNN_SORT");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
 static bool knn_cells_enabled() {
     static const bool on = [] {
         const char* e = std::getenv("RQ_KNN_CELLS");
@@ -594,6 +604,16 @@ static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint
             L.out_rgb = rgb;
             L.caustic = caustic;
             L.grid = pg->view;
+            const bool sorted = nq >= kKnnSortMin && knn_cells_sorted();
+            const size_t sort_bytes = sorted ? knn_cell_order_bytes(nq) : 0;
+            const size_t order_bytes = sorted ? ((size_t)nq * 4 + 255) & ~(size_t)255 : 0;
+            uint32_t* scratch = stream_scratch(dev, s, order_bytes + sort_bytes);
+            char* w = reinterpret_cast<char*>(scratch + 64);
+            if (sorted) {
+                uint32_t* order = reinterpret_cast<uint32_t*>(w);
+                knn_cell_order(L, order, w + order_bytes, sort_bytes, s);
+                L.order = order;
+            }
             launch_knn_cells(L, knn_cell_grid(nq, dev->num_cus), s);
             return;
         }

@@ -246,7 +246,8 @@ struct KnnLaunch {
     uint32_t caustic;
     uint32_t* spill;     // per-lane stack spill
     uint32_t spill_depth;
-    PhotonGridView grid;  // knn_cell_kernel only
+    PhotonGridView grid;    // knn_cell_kernel only
+    const uint32_t* order;  // knn_cell_kernel: queries in this order (null: 0..nq-1)
 };
 
 void launch_knn(const KnnLaunch& L, int grid, hipStream_t s);
@@ -262,6 +263,11 @@ uint32_t knn_wave_stack_cap(uint32_t n);
 // Radiance without lists from the photon cell grid, one query per wave.
 int knn_cell_grid(uint32_t nq, int num_cus);
 void launch_knn_cells(const KnnLaunch& L, int grid, hipStream_t s);
+// Queries ordered by the Morton key of their grid cell, so that the waves
+// running at one time read the same cells: scratch bytes for nq queries, and
+// the sort (order receives nq query indices; scratch holds the rest).
+size_t knn_cell_order_bytes(uint32_t nq);
+void knn_cell_order(const KnnLaunch& L, uint32_t* order, void* scratch, size_t scratch_bytes, hipStream_t s);
 int knn_wave_grid(uint32_t nq, int num_cus);
 void launch_knn_wave(const KnnLaunch& L, int grid, uint32_t stack_cap, hipStream_t s);
 
