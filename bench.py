@@ -845,16 +845,31 @@ def run_workload(args):
         dd = torch.empty((len(Q), k), dtype=torch.float32, device="cuda")
         dc = torch.empty((len(Q),), dtype=torch.int32, device="cuda")
         drgb = torch.empty((len(Q), 3), dtype=torch.float32, device="cuda")
-        for layout in ("spatial", "encore"):
+        # The engine's default (photon cell grid), then the balanced-tree
+        # walks (RQ_KNN_CELLS=0) over the spatial and ENCORE's own layout.
+        # The parity sample below checks the default path's output.
+        for name, layout, tree in (("tree_encore_layout", "encore", True), ("tree_spatial_layout", "spatial", True),
+                                   ("cells", "spatial", False)):
+            if tree:
+                os.environ["RQ_KNN_CELLS"] = "0"
+            else:
+                os.environ.pop("RQ_KNN_CELLS", None)
             t = time.time()
             pm = rq.PhotonMap(dev, P, N, W, layout=layout)
             build = time.time() - t
             fn = lambda: pm.radiance(dq, dqn, k, r, drgb, di, dd, dc)
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            first_ms = (time.perf_counter() - t) * 1e3
             wall, ev = _time_gpu(fn, args.steps, args.warmup)
-            out[layout] = {"Mqueries_s": round(len(Q) / (ev / 1000.0) / 1e6, 3), "launch_ms": round(ev, 3),
-                           "build_s": round(build, 2)}
-            log(layout, out[layout])
+            out[name] = {"Mqueries_s": round(len(Q) / (ev / 1000.0) / 1e6, 3), "launch_ms": round(ev, 3),
+                         "build_s": round(build, 2)}
+            if not tree:
+                out[name]["first_call_ms_incl_grid_build"] = round(first_ms, 2)
+            log(name, out[name])
             pm.destroy()
+        os.environ.pop("RQ_KNN_CELLS", None)
         cnt = dc.cpu().numpy()
         got_i = di.cpu().numpy().view(np.uint32)
         got_d = dd.cpu().numpy()
@@ -881,12 +896,12 @@ def run_workload(args):
         parity = {"sample": int(len(sq)), "count_equal": same_cnt, "set_equal_tie_aware": same_set,
                   "radiance_max_rel": float(rel.max()), "mean_count": float(gc.mean())}
         log("parity", parity)
-        _line("Mqueries/s photon gather (k=50, r=0.05)", out["spatial"]["Mqueries_s"], "Mqueries/s",
-              out["spatial"]["launch_ms"],
+        _line("Mqueries/s photon gather (k=50, r=0.05)", out["cells"]["Mqueries_s"], "Mqueries/s",
+              out["cells"]["launch_ms"],
               {"workload": "radiance estimate at %d gather points, %d photons (3 bounces, albedo 0.7) on the "
                            "Cornell box at 1/20 scale tessellated to %d triangles" % (len(Q), len(P), len(idx)),
                "k": k, "max_dist": r},
-              layouts=out, parity=parity,
+              paths=out, parity=parity,
               cpu_baseline={"value": round(len(sq) / cpu_s / 1e6, 5), "unit": "Mqueries/s", "cores": thr,
                             "kind": "port", "sample": "%d gather points" % len(sq)})
         return 0

@@ -350,10 +350,11 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// The end of a one-query-per-wave estimate: keeps the k nearest of the
-// `cand` candidates in LDS (ties at the k-th distance: those found first) and
-// writes the count and the radiance (GetRadianceEstimate,
-// PhotonMap.cpp:121-164), NaN when the walk overflowed.
+// The end of a one-query-per-wave gather: keeps the k nearest of the `cand`
+// candidates in LDS (ties at the k-th distance: those found first) and
+// writes the count, the list nearest first when asked, and the radiance when
+// asked (GetRadianceEstimate, PhotonMap.cpp:121-164), NaN when the walk
+// overflowed.
 __device__ void wave_finish(const KnnLaunch& L, uint32_t q, uint32_t cand, bool overflow, uint32_t* s_key,
                             uint32_t* s_slot, uint32_t* s_hist) {
     const uint32_t lane = __lane_id();
@@ -362,6 +363,28 @@ __device__ void wave_finish(const KnnLaunch& L, uint32_t q, uint32_t cand, bool 
         uint32_t need = k;
         const uint32_t t = select_kth(s_key, cand, need, s_hist);
         cand = compact_kth(s_key, s_slot, cand, t, need);
+    }
+    if (L.out_idx) {
+        // the list, nearest first: the rank of each kept photon by (d2, buffer position)
+        const size_t base = (size_t)q * k;
+        for (uint32_t i = lane; i < cand; i += 64) {
+            const uint32_t key = s_key[i];
+            uint32_t rank = 0;
+            for (uint32_t j = 0; j < cand; ++j) {
+                const uint32_t o = s_key[j];
+                rank += (o < key || (o == key && j < i)) ? 1u : 0u;
+            }
+            L.out_idx[base + rank] = L.perm[s_slot[i]];
+            L.out_d2[base + rank] = __uint_as_float(key);
+        }
+        for (uint32_t j = cand + lane; j < k; j += 64) {
+            L.out_idx[base + j] = 0xFFFFFFFFu;
+            L.out_d2[base + j] = __int_as_float(0x7f800000);
+        }
+    }
+    if (!L.out_rgb) {
+        if (lane == 0 && L.out_cnt) L.out_cnt[q] = cand;
+        return;
     }
     // radiance over the kept set (GetRadianceEstimate, PhotonMap.cpp:121-164)
     uint32_t mx = 0;

@@ -547,32 +547,19 @@ rq_status rq_photons_balance_host(const float* pos, uint32_t n, uint32_t layout,
     });
 }
 
-// A/B switches: RQ_KNN_LANE=1 keeps the per-lane heap kernel for every k;
-// RQ_KNN_CELLS=0 keeps the tree walk for large-k radiance.
-static bool knn_force_lane() {
-    static const bool force = [] {
-        const char* e = std::getenv("RQ_KNN_LANE");
-        return e && e[0] == '1';
-    }();
-    return force;
+// Kernel switches, read at every call so tests and sweeps can flip them:
+// RQ_KNN_LANE=1 keeps the per-lane heap kernel for every k; RQ_KNN_CELLS=0
+// keeps the tree walks for large k (no cell grid); RQ_KNN_SORT=0 keeps the
+// caller's query order in a cell-grid gather.
+static bool env_is(const char* name, char v) {
+    const char* e = std::getenv(name);
+    return e && e[0] == v;
 }
-// Queries of a cell-grid gather are sorted by cell from this many on
-// (RQ_KNN_SORT=0 keeps the caller's order).
+static bool knn_force_lane() { return env_is("RQ_KNN_LANE", '1'); }
+static bool knn_cells_enabled() { return !env_is("RQ_KNN_CELLS", '0'); }
+static bool knn_cells_sorted() { return !env_is("RQ_KNN_SORT", '0'); }
+// Queries of a cell-grid gather are sorted by cell from this many on.
 constexpr uint32_t kKnnSortMin = 16384;
-static bool knn_cells_sorted() {
-    static const bool on = [] {
-        const char* e = std::getenv("RQ_KNN_SORT");
-        return !(e && e[0] == '0');
-    }();
-    return on;
-}
-static bool knn_cells_enabled() {
-    static const bool on = [] {
-        const char* e = std::getenv("RQ_KNN_CELLS");
-        return !(e && e[0] == '0');
-    }();
-    return on;
-}
 
 static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint32_t nq, uint32_t k, float max_dist,
                        uint32_t* idx, float* d2, uint32_t* cnt, float* rgb, uint32_t caustic, hipStream_t s) {
@@ -582,11 +569,12 @@ static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint
     if (nq == 0) return;
     rq_device* dev = p->dev;
     RQ_HIP(hipSetDevice(dev->ordinal));
-    // Radiance without lists at large k: one query per wave, from the photon
-    // cell grid when one suits the radius, else walking the tree.
+    // Large k: one query per wave, from the photon cell grid when one suits
+    // the radius; else radiance without lists walks the tree, one query per
+    // wave, and lists come from the per-lane heap kernel.
     const bool wave = idx == nullptr && rgb != nullptr && k >= kKnnWaveMinK && !knn_force_lane();
     const float r2 = max_dist * max_dist;  // fl32(maxDistance * maxDistance) (PhotonMap.cpp:89)
-    if (wave && knn_cells_enabled()) {
+    if (k >= kKnnWaveMinK && !knn_force_lane() && knn_cells_enabled()) {
         std::lock_guard<std::mutex> g(p->grid_mu);
         if (const PhotonGrid* pg = photon_grid_for(p, r2, s)) {
             KnnLaunch L{};
@@ -600,6 +588,8 @@ static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint
             L.nq = nq;
             L.k = k;
             L.r2 = r2;
+            L.out_idx = idx;
+            L.out_d2 = d2;
             L.out_cnt = cnt;
             L.out_rgb = rgb;
             L.caustic = caustic;

@@ -3,10 +3,13 @@ candidates from the photon cell grid (photon_grid.hip, knn_cell_kernel).
 The grid must never miss a photon in range: counts equal brute force where
 fewer than k photons are in range, whatever the offset of the map, and
 photons that no query can select (NaN normals) stay out.  Estimates equal
-the per-lane heap kernel's (asked with lists, which never uses the grid) up
-to photons tied with the k-th distance, also when one map is queried with
+the per-lane heap kernel's (RQ_KNN_CELLS=0 with lists) up to photons tied
+with the k-th distance, and lists from the grid equal the heap kernel's as
+tie-aware sets, also when one map is queried with
 alternating radii (the grid is rebuilt) and when the radius is too large for
 a grid (the tree walk serves)."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -15,8 +18,20 @@ import oracle
 from encore_amd import rq
 from helpers import log
 from test_gpu_knn_wave import _radiance
+from test_gpu_photons import _gpu_knn
+from test_oracle_photons import same_knn_set
 
 pytestmark = pytest.mark.gpu
+
+
+def _heap(fn, *args):
+    """Runs fn with the cell grid switched off: lists come from the per-lane
+    heap kernel."""
+    os.environ["RQ_KNN_CELLS"] = "0"
+    try:
+        return fn(*args)
+    finally:
+        del os.environ["RQ_KNN_CELLS"]
 
 
 def _brute_count(P, N, Q, QN, r):
@@ -70,7 +85,7 @@ def test_cells_find_every_photon_in_range(device, offset):
     log("case: cells offset %g: mean count %.1f, max %d" % (offset, want.mean(), want.max()))
     assert want.max() < k and want.mean() > 10
     assert np.array_equal(cnt, want)
-    heap_rgb, heap_cnt = _radiance(pm, Q, QN, k, r, True, False)
+    heap_rgb, heap_cnt = _heap(_radiance, pm, Q, QN, k, r, True, False)
     assert np.array_equal(cnt, heap_cnt)
     np.testing.assert_allclose(rgb, heap_rgb, rtol=1e-6, atol=0)
     pm.destroy()
@@ -84,7 +99,7 @@ def test_cells_saturated_equal_heap_kernel_and_oracle(device, k, caustic):
     pm = rq.PhotonMap(device, P, N, W)
     r = 0.05  # about 2.5x k = 100 photons in range on a plane
     rgb, cnt = _radiance(pm, Q, QN, k, r, False, caustic)
-    heap_rgb, heap_cnt = _radiance(pm, Q, QN, k, r, True, caustic)
+    heap_rgb, heap_cnt = _heap(_radiance, pm, Q, QN, k, r, True, caustic)
     assert np.array_equal(cnt, heap_cnt)
     assert not np.isnan(rgb).any()
     same = np.all(rgb.view(np.uint32) == heap_rgb.view(np.uint32), axis=1)
@@ -110,7 +125,7 @@ def test_cells_skip_unselectable_photons(device):
         pm = rq.PhotonMap(device, P, N, W)
         rgb, cnt = _radiance(pm, Q, QN, 64, 0.04, False, False)
         assert np.array_equal(cnt, np.minimum(_brute_count(P, N, Q, QN, 0.04), 64))
-        heap_rgb, _ = _radiance(pm, Q, QN, 64, 0.04, True, False)
+        heap_rgb, _ = _heap(_radiance, pm, Q, QN, 64, 0.04, True, False)
         np.testing.assert_allclose(rgb, heap_rgb, rtol=1e-5)
         pm.destroy()
 
@@ -121,10 +136,34 @@ def test_cells_radius_changes_and_large_radius(device):
     pm = rq.PhotonMap(device, P, N, W)
     for r in (0.03, 0.06, 0.03, 0.5, 2.0, 0.03):
         rgb, cnt = _radiance(pm, Q, QN, 64, r, False, False)
-        heap_rgb, heap_cnt = _radiance(pm, Q, QN, 64, r, True, False)
+        heap_rgb, heap_cnt = _heap(_radiance, pm, Q, QN, 64, r, True, False)
         log("case: radius %g: mean count %.1f" % (r, cnt.mean()))
         assert np.array_equal(cnt, heap_cnt)
         same = np.all(rgb.view(np.uint32) == heap_rgb.view(np.uint32), axis=1)
         assert same.mean() >= 0.99
         np.testing.assert_allclose(rgb, heap_rgb, rtol=1e-5, atol=1e-12)
+    pm.destroy()
+
+
+@pytest.mark.parametrize("k", [32, 50, 256])
+def test_cell_lists_equal_heap_kernel(device, k):
+    """rq_knn and lists with the radiance: nearest first, ~0 / +inf padding,
+    the heap kernel's set up to ties at the k-th distance."""
+    rng = np.random.default_rng(15)
+    P, N, W, Q, QN = _surface_photons(rng, 100000, 0.0)
+    pm = rq.PhotonMap(device, P, N, W)
+    r = 0.04
+    gi, gd, gc = _gpu_knn(pm, Q, QN, k, r)
+    hi, hd, hc = _heap(_gpu_knn, pm, Q, QN, k, r)
+    assert np.array_equal(gc, hc)
+    log("case: cell lists k=%d: mean count %.1f, %.3f full" % (k, gc.mean(), (gc == k).mean()))
+    for i in range(len(Q)):
+        c = int(gc[i])
+        assert np.all(np.diff(gd[i, :c]) >= 0)
+        assert np.all(gi[i, c:] == 0xFFFFFFFF) and np.all(np.isinf(gd[i, c:]))
+        assert same_knn_set(gi[i, :c], gd[i, :c], hi[i, :c], hd[i, :c]), i
+    gi2, gd2, gc2, rgb = _gpu_knn(pm, Q, QN, k, r, radiance=True)
+    assert np.array_equal(gi2, gi) and np.array_equal(gd2.view(np.uint32), gd.view(np.uint32))
+    rgb_only, cnt_only = _radiance(pm, Q, QN, k, r, False, False)
+    assert np.array_equal(rgb.view(np.uint32), rgb_only.view(np.uint32)) and np.array_equal(gc2, cnt_only)
     pm.destroy()

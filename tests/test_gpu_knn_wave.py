@@ -1,7 +1,11 @@
-"""Radiance estimates without neighbour lists at large k run one query per
-wave (knn_wave_kernel).  They must equal the per-lane heap kernel's estimate
-(the same k-nearest set up to photons tied with the k-th distance) and the
-oracle's GetRadianceEstimate."""
+"""Radiance estimates without neighbour lists at large k, when no photon
+cell grid serves them, run one query per wave walking the tree
+(knn_wave_kernel).  They must equal the per-lane heap kernel's estimate (the
+same k-nearest set up to photons tied with the k-th distance) and the
+oracle's GetRadianceEstimate.  RQ_KNN_CELLS=0 keeps the grid out for the
+whole module."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -11,6 +15,13 @@ from encore_amd import rq, scenes
 from helpers import log
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _tree_kernels():
+    os.environ["RQ_KNN_CELLS"] = "0"
+    yield
+    del os.environ["RQ_KNN_CELLS"]
 
 
 @pytest.fixture(scope="module")

@@ -59,9 +59,14 @@ def _compare(P, N, W, Q, QN, k, r, gi, gd, gc):
 
 
 @pytest.mark.parametrize("layout", ["spatial", "encore"])
-def test_knn_config4(device, c4, layout):
+@pytest.mark.parametrize("path", ["cells", "tree"])
+def test_knn_config4(device, c4, layout, path, monkeypatch):
+    """k = 50: the photon cell grid by default; RQ_KNN_CELLS=0 gives the
+    per-lane heap kernel over the balanced tree."""
     P, N, W, Q, QN = c4
-    log("case: knn config4-lite", layout, len(P), "photons", len(Q), "queries")
+    log("case: knn config4-lite", layout, path, len(P), "photons", len(Q), "queries")
+    if path == "tree":
+        monkeypatch.setenv("RQ_KNN_CELLS", "0")
     pm = rq.PhotonMap(device, P, N, W, layout=layout)
     gi, gd, gc = _gpu_knn(pm, Q, QN, 50, 0.05)
     ties = _compare(P, N, W, Q, QN, 50, 0.05, gi, gd, gc)
