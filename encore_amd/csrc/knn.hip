@@ -570,18 +570,25 @@ __global__ __launch_bounds__(64) void knn_cell_kernel(KnnLaunch L) {
     __shared__ uint32_t s_slot[kCellCand];
     __shared__ uint32_t s_hist[256];
     __shared__ uint32_t s_first[64], s_pre[64];
-    __shared__ double s_near[64];
+    __shared__ float s_near[64];
     const PhotonGridView& G = L.grid;
     const uint32_t lane = __lane_id();
     const uint32_t k = L.k;
     const uint32_t compact_at = min((uint32_t)kCellCand - 64u, max(2u * k, k + 64u));
-    // Grid-stride over the queries in cell order: the waves running at one
-// time hold a window of consecutive queries, which share cells.
-for (uint32_t qi = blockIdx.x; qi < L.nq; qi += gridDim.x) {
-        const uint32_t q = L.order ? L.order[qi] : qi;
-        const float qx = L.qpos[3 * (size_t)q], qy = L.qpos[3 * (size_t)q + 1], qz = L.qpos[3 * (size_t)q + 2];
-        const uint32_t cls = lookup_class(L.table, L.table_mask, L.qnrm[3 * (size_t)q], L.qnrm[3 * (size_t)q + 1],
-                                          L.qnrm[3 * (size_t)q + 2]);
+    // Grid-stride over the prepared queries (cell order when sorted): the
+    // waves running at one time hold a window of consecutive queries, which
+    // share cells.  The next query's record is loaded a query ahead.
+    uint32_t qi = blockIdx.x;
+    float4 nq4 = qi < L.nq ? L.qprep[qi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t nq_id = qi < L.nq ? (L.order ? L.order[qi] : qi) : 0u;
+    for (; qi < L.nq; qi += gridDim.x) {
+        const uint32_t q = nq_id;
+        const float qx = nq4.x, qy = nq4.y, qz = nq4.z;
+        const uint32_t cls = __float_as_uint(nq4.w);
+        if (qi + gridDim.x < L.nq) {
+            nq4 = L.qprep[qi + gridDim.x];
+            nq_id = L.order ? L.order[qi + gridDim.x] : qi + gridDim.x;
+        }
         uint32_t cand = 0;
         bool overflow = false;
         if (cls != kClassNever) {
@@ -613,19 +620,22 @@ for (uint32_t qi = blockIdx.x; qi < L.nq; qi += gridDim.x) {
                 range = cell_lookup(G, cell_key(c[0], c[1], c[2]));
                 v = range.y > 0;
             }
-            // nearest first: rank among the kept cells by (near2, lane)
+            // Nearest first.  The bound is kept as a float rounded down with
+            // its 6 low bits cleared (still a lower bound), and the lane in
+            // those bits makes the 32-bit sort keys distinct.
+            const uint32_t nb = __float_as_uint(__double2float_rd(near2)) & ~63u;
+            const uint32_t key = nb | lane;
             const uint64_t vm = __ballot(v);
             uint32_t rank = 0;
             for (uint64_t rest = vm; rest; rest &= rest - 1) {
                 const int j = __ffsll((unsigned long long)rest) - 1;
-                const double o = __shfl(near2, j, 64);
-                rank += (o < near2 || (o == near2 && (uint32_t)j < lane)) ? 1u : 0u;
+                rank += (uint32_t)__shfl((int)key, j, 64) < key ? 1u : 0u;
             }
             const uint32_t ncell = (uint32_t)__popcll(vm);
             if (v) {
                 s_first[rank] = range.x;
                 s_pre[rank] = range.y;
-                s_near[rank] = near2;
+                s_near[rank] = __uint_as_float(nb);
             }
             __builtin_amdgcn_wave_barrier();
             // photon counts -> exclusive prefix, in rank order
@@ -639,18 +649,31 @@ for (uint32_t qi = blockIdx.x; qi < L.nq; qi += gridDim.x) {
             __builtin_amdgcn_wave_barrier();
             if (lane < ncell) s_pre[lane] = incl - cnt;
             __builtin_amdgcn_wave_barrier();
-            uint32_t ci = 0, c0 = 0;  // cells of this lane's and of the chunk's first element
-            for (uint32_t e0 = 0; e0 < total; e0 += 64) {
-                while (c0 + 1 < ncell && s_pre[c0 + 1] <= e0) ++c0;
-                if (s_near[c0] >= (double)r2) break;  // this cell and all later ones are too far
+            // Chunk e0 covers elements e0 .. e0 + 63 of the concatenated
+            // ranges; its photons (position, class, slot) are loaded one
+            // chunk ahead, so their latency overlaps the previous chunk.
+            uint32_t ci = 0, c0 = 0;  // cells of this lane's next element and of the chunk's first
+            float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            uint32_t sl = 0;
+            auto fetch = [&](uint32_t e0, float4& fp, uint32_t& fs) {
                 const uint32_t e = e0 + lane;
-                bool is_c = false;
-                float d2 = 0.0f;
-                uint32_t gi = 0;
                 if (e < total) {
                     while (ci + 1 < ncell && s_pre[ci + 1] <= e) ++ci;
-                    gi = s_first[ci] + (e - s_pre[ci]);
-                    const float4 p = G.pos[gi];
+                    const uint32_t gi = s_first[ci] + (e - s_pre[ci]);
+                    fp = G.pos[gi];
+                    fs = G.slot[gi];
+                }
+            };
+            if (total) fetch(0, p, sl);
+            for (uint32_t e0 = 0; e0 < total; e0 += 64) {
+                while (c0 + 1 < ncell && s_pre[c0 + 1] <= e0) ++c0;
+                if (s_near[c0] >= r2) break;  // this cell and all later ones are too far
+                float4 pn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                uint32_t sn = 0;
+                if (e0 + 64 < total) fetch(e0 + 64, pn, sn);
+                bool is_c = false;
+                float d2 = 0.0f;
+                if (e0 + lane < total) {
                     const float x = qx - p.x, y = qy - p.y, z = qz - p.z;
                     d2 = (x * x + y * y) + z * z;
                     is_c = d2 < r2 && __float_as_uint(p.w) == cls;
@@ -659,7 +682,7 @@ for (uint32_t qi = blockIdx.x; qi < L.nq; qi += gridDim.x) {
                 if (is_c) {
                     const uint32_t pos = cand + lanes_below(cm);
                     s_key[pos] = __float_as_uint(d2);
-                    s_slot[pos] = G.slot[gi];
+                    s_slot[pos] = sl;
                 }
                 cand += (uint32_t)__popcll(cm);
                 __builtin_amdgcn_wave_barrier();
@@ -669,11 +692,25 @@ for (uint32_t qi = blockIdx.x; qi < L.nq; qi += gridDim.x) {
                     cand = compact_kth(s_key, s_slot, cand, t, need);
                     r2 = __uint_as_float(t);
                 }
+                p = pn;
+                sl = sn;
             }
         }
         wave_finish(L, q, cand, overflow, s_key, s_slot, s_hist);
         __builtin_amdgcn_wave_barrier();
     }
+}
+
+// Prepared queries for knn_cell_kernel: position and normal class in the
+// order it serves them.
+__global__ void knn_cell_prep(KnnLaunch L, float4* prep) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.nq) return;
+    const uint32_t q = L.order ? L.order[i] : i;
+    const uint32_t cls = lookup_class(L.table, L.table_mask, L.qnrm[3 * (size_t)q], L.qnrm[3 * (size_t)q + 1],
+                                      L.qnrm[3 * (size_t)q + 2]);
+    prep[i] = make_float4(L.qpos[3 * (size_t)q], L.qpos[3 * (size_t)q + 1], L.qpos[3 * (size_t)q + 2],
+                          __uint_as_float(cls));
 }
 
 __global__ void knn_cell_keys(KnnLaunch L, unsigned long long* keys, uint32_t* ids) {
@@ -749,9 +786,13 @@ int knn_cell_grid(uint32_t nq, int num_cus) {
     return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)nq, (int64_t)std::max(1, per_cu) * num_cus));
 }
 
-void launch_knn_cells(const KnnLaunch& L, int grid, hipStream_t s) {
+void launch_knn_cells(const KnnLaunch& L, float4* prep, int grid, hipStream_t s) {
     if (L.nq == 0) return;
-    hipLaunchKernelGGL(dev::knn_cell_kernel, dim3(grid), dim3(dev::kKnnWave), 0, s, L);
+    hipLaunchKernelGGL(dev::knn_cell_prep, dim3((L.nq + 255) / 256), dim3(256), 0, s, L, prep);
+    RQ_HIP(hipGetLastError());
+    KnnLaunch C = L;
+    C.qprep = prep;
+    hipLaunchKernelGGL(dev::knn_cell_kernel, dim3(grid), dim3(dev::kKnnWave), 0, s, C);
     RQ_HIP(hipGetLastError());
 }
 

@@ -595,16 +595,18 @@ static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint
             L.caustic = caustic;
             L.grid = pg->view;
             const bool sorted = nq >= kKnnSortMin && knn_cells_sorted();
+            const size_t prep_bytes = ((size_t)nq * 16 + 255) & ~(size_t)255;
             const size_t sort_bytes = sorted ? knn_cell_order_bytes(nq) : 0;
             const size_t order_bytes = sorted ? ((size_t)nq * 4 + 255) & ~(size_t)255 : 0;
-            uint32_t* scratch = stream_scratch(dev, s, order_bytes + sort_bytes);
+            uint32_t* scratch = stream_scratch(dev, s, prep_bytes + order_bytes + sort_bytes);
             char* w = reinterpret_cast<char*>(scratch + 64);
+            float4* prep = reinterpret_cast<float4*>(w);
             if (sorted) {
-                uint32_t* order = reinterpret_cast<uint32_t*>(w);
-                knn_cell_order(L, order, w + order_bytes, sort_bytes, s);
+                uint32_t* order = reinterpret_cast<uint32_t*>(w + prep_bytes);
+                knn_cell_order(L, order, w + prep_bytes + order_bytes, sort_bytes, s);
                 L.order = order;
             }
-            launch_knn_cells(L, knn_cell_grid(nq, dev->num_cus), s);
+            launch_knn_cells(L, prep, knn_cell_grid(nq, dev->num_cus), s);
             return;
         }
     }

@@ -248,6 +248,7 @@ struct KnnLaunch {
     uint32_t spill_depth;
     PhotonGridView grid;    // knn_cell_kernel only
     const uint32_t* order;  // knn_cell_kernel: queries in this order (null: 0..nq-1)
+    const float4* qprep;    // knn_cell_kernel: position and class per served query
 };
 
 void launch_knn(const KnnLaunch& L, int grid, hipStream_t s);
@@ -262,7 +263,8 @@ constexpr uint32_t kKnnWaveMinK = 32;
 uint32_t knn_wave_stack_cap(uint32_t n);
 // Radiance without lists from the photon cell grid, one query per wave.
 int knn_cell_grid(uint32_t nq, int num_cus);
-void launch_knn_cells(const KnnLaunch& L, int grid, hipStream_t s);
+// prep: nq float4 of scratch for the prepared queries.
+void launch_knn_cells(const KnnLaunch& L, float4* prep, int grid, hipStream_t s);
 // Queries ordered by the Morton key of their grid cell, so that the waves
 // running at one time read the same cells: scratch bytes for nq queries, and
 // the sort (order receives nq query indices; scratch holds the rest).
