@@ -590,8 +590,10 @@ __global__ __launch_bounds__(64) void knn_cell_kernel(KnnLaunch L) {
             nq_id = L.order ? L.order[qi + gridDim.x] : qi + gridDim.x;
         }
         uint32_t cand = 0;
-        bool overflow = false;
-        if (cls != kClassNever) {
+        bool far = false;
+        // A query with a coordinate that is not finite has no photon in range
+        // (photon positions are finite).
+        if (cls != kClassNever && isfinite(qx) && isfinite(qy) && isfinite(qz)) {
             float r2 = L.r2;
             const double qd[3] = {(double)qx, (double)qy, (double)qz};
             const uint32_t off[3] = {lane & 3u, (lane >> 2) & 3u, lane >> 4};
@@ -604,7 +606,10 @@ __global__ __launch_bounds__(64) void knn_cell_kernel(KnnLaunch L) {
                 const double w = G.reach + fabs(qd[a]) * 0x1p-40;
                 const int64_t lo = grid_coord(qd[a] - w, G.origin[a], G.inv_cell);
                 const int64_t hi = grid_coord(qd[a] + w, G.origin[a], G.inv_cell);
-                overflow |= hi - lo > 3;  // only for |q| beyond 2^38 cells
+                // More than 4 cells only when |q| exceeds 2^38 cells; the grid
+                // is refused for maps whose coordinates come near that, so such
+                // a query is farther than the reach from every photon.
+                far |= hi - lo > 3;
                 const int64_t ca = lo + (int64_t)off[a];
                 v = v && ca <= hi && ca >= 0 && ca < (int64_t)G.dims[a];
                 c[a] = (uint64_t)max(ca, (int64_t)0);
@@ -614,7 +619,7 @@ __global__ __launch_bounds__(64) void knn_cell_kernel(KnnLaunch L) {
                 near2 += d * d;
             }
             near2 *= 1.0 - 0x1p-20;
-            v = v && near2 < (double)r2 && !overflow;
+            v = v && near2 < (double)r2 && !far;
             uint2 range = make_uint2(0u, 0u);
             if (v) {
                 range = cell_lookup(G, cell_key(c[0], c[1], c[2]));
@@ -696,7 +701,7 @@ __global__ __launch_bounds__(64) void knn_cell_kernel(KnnLaunch L) {
                 sl = sn;
             }
         }
-        wave_finish(L, q, cand, overflow, s_key, s_slot, s_hist);
+        wave_finish(L, q, cand, false, s_key, s_slot, s_hist);
         __builtin_amdgcn_wave_barrier();
     }
 }

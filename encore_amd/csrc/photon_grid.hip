@@ -131,6 +131,11 @@ const PhotonGrid* photon_grid_for(rq_photons* p, float r2, hipStream_t s) {
         widest = std::max(widest, dims[a]);
     }
     if (widest < 4) return nullptr;
+    // Queries widen their reach by |q| 2^-40 for rounding; near the map that
+    // must stay below a quarter cell.
+    double far = 0.0;
+    for (int a = 0; a < 3; ++a) far = std::max({far, std::fabs((double)p->bmin[a]), std::fabs((double)p->bmax[a])});
+    if ((far + 2.0 * reach) * 0x1p-40 >= 0.25 * cell) return nullptr;
     const uint32_t n = p->n, nv = p->nvalid;
     uint32_t cap = 64;
     while (cap < 2 * (uint64_t)nv) cap <<= 1;

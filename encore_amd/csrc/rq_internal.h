@@ -465,7 +465,8 @@ struct PhotonGrid {
 };
 // Makes (or keeps) the grid for r2 on stream s and returns it, or null when
 // a grid does not suit this radius (not finite or not positive, more than
-// 2^21 - 1 cells along an axis, or fewer than 4 along every axis).  The
+// 2^21 - 1 cells along an axis, fewer than 4 along every axis, or map
+// coordinates near 2^38 cells from the origin).  The
 // caller holds p->grid_mu until the queries that read the grid are queued.
 const PhotonGrid* photon_grid_for(rq_photons* p, float r2, hipStream_t s);
 int occupancy_blocks_per_cu(uint32_t accel, uint32_t predicate, TraceKind kind, uint32_t width,

@@ -167,3 +167,24 @@ def test_cell_lists_equal_heap_kernel(device, k):
     rgb_only, cnt_only = _radiance(pm, Q, QN, k, r, False, False)
     assert np.array_equal(rgb.view(np.uint32), rgb_only.view(np.uint32)) and np.array_equal(gc2, cnt_only)
     pm.destroy()
+
+
+def test_cells_degenerate_query_positions(device):
+    """Queries at NaN, infinite or far positions find no photons on the grid
+    path, as on the heap kernel's; lists and radiance stay consistent."""
+    rng = np.random.default_rng(16)
+    P, N, W, Q, QN = _surface_photons(rng, 20000, 0.0)
+    pm = rq.PhotonMap(device, P, N, W)
+    bad = np.array([[np.nan, 0.5, 0.0], [0.5, np.inf, 0.0], [-np.inf, 0.5, 0.0], [1e30, 0.5, 0.0],
+                    [-1e30, -1e30, 0.0], [3e38, 3e38, 0.0], [0.5, 0.5, 1e-40]], np.float32)
+    bad_n = np.tile(np.float32([0, 0, 1]), (len(bad), 1))
+    Qm = np.concatenate([Q[:100], bad, Q[100:200]]).astype(np.float32)
+    QNm = np.concatenate([QN[:100], bad_n, QN[100:200]]).astype(np.float32)
+    rgb, cnt = _radiance(pm, Qm, QNm, 64, 0.04, False, False)
+    heap_rgb, heap_cnt = _heap(_radiance, pm, Qm, QNm, 64, 0.04, True, False)
+    assert np.array_equal(cnt, heap_cnt)
+    assert cnt[100:106].tolist() == [0] * 6 and np.all(rgb[100:106] == 0)
+    np.testing.assert_allclose(rgb, heap_rgb, rtol=1e-5)
+    gi, gd, gc = _gpu_knn(pm, Qm, QNm, 64, 0.04)
+    assert np.array_equal(gc, cnt) and np.all(gi[100:106] == 0xFFFFFFFF)
+    pm.destroy()
