@@ -90,7 +90,10 @@ static int grid_for(rq_device* dev, uint32_t accel, uint32_t pred, TraceKind kin
     int per_cu = kc.blocks_per_cu > 0 ? kc.blocks_per_cu : occupancy_blocks_per_cu(accel, pred, kind, width, kc);
     int64_t grid = (int64_t)per_cu * dev->num_cus;
     const int64_t bt = block_threads(accel);
-    const int64_t need = ((int64_t)n + bt - 1) / bt;
+    // kd: up to a wave per ray (the kernel spreads a small launch over every
+    // wave and tests long leaves with the idle lanes); BVH: a lane per ray.
+    const int64_t per_block = accel == RQ_ACCEL_KD ? bt / 64 : bt;
+    const int64_t need = ((int64_t)n + per_block - 1) / per_block;
     grid = std::max<int64_t>(1, std::min(grid, need));
     return (int)grid;
 }
@@ -123,6 +126,10 @@ static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, ui
     }
     L.counts = counts;
     L.fetch_threshold = (uint32_t)(sc->accel == RQ_ACCEL_KD ? dev->kc.kd_fetch_threshold : dev->kc.fetch_threshold);
+    // kd launches claim rays along a Weyl sequence (RQ_KD_SCATTER=0 keeps
+    // the input order, for A/B checks); results do not depend on the order.
+    const char* sc_env = std::getenv("RQ_KD_SCATTER");
+    L.scatter = (sc->accel == RQ_ACCEL_KD && !(sc_env && sc_env[0] == '0')) ? 1u : 0u;
     launch_zero_u32(scratch, 2, s);
     if (counts)
         launch_count(L, sc->accel, sc->predicate, kind, sc->width, grid, s);

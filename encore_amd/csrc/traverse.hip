@@ -718,6 +718,57 @@ __device__ __forceinline__ bool root_clip(const RayState& r, const float* pos, c
     return true;
 }
 
+// Claims for the kd kernel: idle lanes take rays only while the wave holds
+// fewer than `cap` of them, and once the counter has run out every idle lane
+// is done.  With cap = ceil(n / waves) a launch of fewer rays than lanes is
+// spread over every wave of the grid instead of filling the first ones, and
+// the idle lanes serve the cooperative leaf tests below.
+__device__ __forceinline__ uint32_t fetch_capped(uint32_t cur, uint32_t* counter, uint32_t n, uint32_t cap,
+                                                 bool drained) {
+    const bool idle = (cur == kNone);
+    const uint64_t m = __ballot(idle);
+    if (m == 0) return cur;
+    if (drained) return idle ? kDone : cur;
+    const uint32_t busy = (uint32_t)__popcll(__ballot(cur != kNone && cur != kDone));
+    const uint32_t want = min((uint32_t)__popcll(m), cap > busy ? cap - busy : 0u);
+    if (want == 0) return cur;
+    // the first `want` idle lanes claim
+    uint64_t take = m;
+    for (uint32_t k = (uint32_t)__popcll(m); k > want; --k) take &= ~(1ull << (63 - __clzll(take)));
+    const int leader = __ffsll((unsigned long long)take) - 1;
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(counter, want);
+    base = __shfl(base, leader);
+    if (base >= n) return idle ? kDone : cur;
+    if (!((take >> lane_id()) & 1ull)) return cur;
+    const uint32_t id = base + lanes_below(take);
+    return id < n ? id : kDone;
+}
+
+// Cooperative leaf testing in the kd tail: at most this many lanes in
+// leaves with more than kKdCoopMin triangles left.
+constexpr uint32_t kKdCoopLanes = 16;
+constexpr uint32_t kKdCoopMin = 16;
+
+// An odd stride near n / phi that is coprime with n: j -> (j * stride) mod n
+// is then a bijection of [0, n) (a Weyl sequence), and every run of
+// consecutive claims samples the whole ray array.
+__device__ uint32_t weyl_stride(uint32_t n) {
+    if (n < 3) return 1;
+    uint32_t s = (uint32_t)(((uint64_t)n * 2654435769ull) >> 32) | 1u;
+    for (;;) {
+        uint32_t a = s, b = n;
+        while (b) {
+            const uint32_t t = a % b;
+            a = b;
+            b = t;
+        }
+        if (a == 1) return s;
+        s += 2;
+        if (s >= n) s = 1;
+    }
+}
+
 // Lanes run a state machine -- one node step, or up to four triangle tests of
 // the current leaf, per iteration -- and idle lanes are refilled from the
 // counter whenever fewer than fetch_threshold lanes of the wave are busy, so
@@ -745,9 +796,20 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
     float rtmax = 0, best = 0, bu = 0, bv = 0, tmin = 0, tmax = 0;
     uint32_t bprim = kNone, cur = 0, lpos = 0, lend = 0;
     uint32_t n_nodes = 0, n_tris = 0;  // COUNT only
+    // Rays of ENCORE's tree differ in cost by three orders of magnitude (a
+    // ray along the ground plane tests tens of thousands of triangles), and
+    // costly rays come in runs of the input order; claimed in that order they
+    // end the launch with a long tail.  A Weyl sequence spreads them.
+    const uint32_t stride = L.scatter ? weyl_stride(nrays) : 1u;
+    bool drained = false;  // the work counter has run out (wave-uniform)
+    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    const uint32_t cap = COUNT ? (uint32_t)kWave
+                               : (uint32_t)min((uint64_t)kWave, max(1ull, ((uint64_t)nrays + waves - 1) / waves));
 
     for (;;) {
-        ray = fetch_work(ray, L.counter, nrays);
+        ray = fetch_capped(ray, L.counter, nrays, cap, drained);
+        if (ray != kNone && ray != kDone && !active && stride != 1u)
+            ray = (uint32_t)(((uint64_t)ray * stride) % nrays);
         if (ray != kNone && ray != kDone && !active) {
             float rtmin;
             init_ray(r, L.rays, ray, rtmin, rtmax);
@@ -768,6 +830,108 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
             }
         }
         if (__ballot(ray != kDone) == 0) break;
+        if (!COUNT) {
+            // Once the rays have run out (or the wave holds all it may), the
+            // few in the wave can hold leaves of thousands of triangles,
+            // tested four per step by one lane.  Then the wave splits into
+            // one group per such ray, and
+            // each group tests its ray's leaf gs triangles per step, keeping
+            // the first (in leaf order) of the closest hits that beat the
+            // ray's record -- the record the lane's sequential strict-< loop
+            // keeps.  The leaf's last triangle is left to the lane, whose
+            // state machine then pops its stack or finishes as usual.
+            drained = drained || __ballot(ray == kDone) != 0;
+            // The wave cannot take more rays now (the counter has run out, or
+            // it holds its cap) and few lanes are busy.
+            const uint32_t busy = (uint32_t)__popcll(__ballot(active));
+            const bool coop = (drained || busy >= cap) && busy <= kKdCoopLanes;
+            const uint64_t longm = coop ? __ballot(active && lend - lpos > kKdCoopMin) : 0ull;
+            const uint32_t nl = (uint32_t)__popcll(longm);
+            if (nl != 0 && nl <= kKdCoopLanes) {
+                // The wave splits into nl groups of gs lanes (gs = 64 / nl
+                // rounded down to a power of two); group g serves the ray of
+                // the g-th lane in longm.
+                uint32_t np2 = 1;
+                while (np2 < nl) np2 <<= 1;
+                const uint32_t gs = kWave / np2, g = lane / gs, j = lane % gs;
+                const bool in_group = g < nl;
+                uint64_t rest = longm;
+                for (uint32_t q = 0; q < g && rest; ++q) rest &= rest - 1;
+                const int src = (in_group && rest) ? __ffsll((unsigned long long)rest) - 1 : lane;
+                RayState c;
+                c.ox = __shfl(r.ox, src);
+                c.oy = __shfl(r.oy, src);
+                c.oz = __shfl(r.oz, src);
+                c.dx = __shfl(r.dx, src);
+                c.dy = __shfl(r.dy, src);
+                c.dz = __shfl(r.dz, src);
+                c.ix = __shfl(r.ix, src);
+                c.iy = __shfl(r.iy, src);
+                c.iz = __shfl(r.iz, src);
+                c.cnx = __shfl(r.cnx, src);
+                c.cny = __shfl(r.cny, src);
+                c.cnz = __shfl(r.cnz, src);
+                c.cfx = __shfl(r.cfx, src);
+                c.cfy = __shfl(r.cfy, src);
+                c.cfz = __shfl(r.cfz, src);
+                c.kx = __shfl(r.kx, src);
+                c.ky = __shfl(r.ky, src);
+                c.kz = __shfl(r.kz, src);
+                c.sx = __shfl(r.sx, src);
+                c.sy = __shfl(r.sy, src);
+                c.sz = __shfl(r.sz, src);
+                float cbest = __shfl(best, src), cbu = __shfl(bu, src), cbv = __shfl(bv, src);
+                uint32_t cprim = __shfl(bprim, src), cpos = __shfl(lpos, src);
+                // every lane takes part in the shuffle (a source lane can lie
+                // outside the grouped lanes); lanes outside a group get no work
+                const uint32_t send = __shfl(lend, src);
+                const uint32_t cend = in_group ? send : cpos;
+                const uint64_t gmask = gs == kWave ? ~0ull : ((1ull << gs) - 1ull) << (g * gs);
+                while (__ballot(in_group && cend - cpos > 1u) != 0) {
+                    const bool work = in_group && cend - cpos > 1u;
+                    const uint32_t cnt = work ? min(gs, cend - cpos - 1u) : 0u;
+                    bool hit = false;
+                    float t = kInf, u = 0.0f, v = 0.0f;
+                    uint32_t prim = kNone;
+                    if (j < cnt) {
+                        const TriRec* tp = tris + cpos + j;
+                        const float4 ta = ldg4(tp->a), tb = ldg4(tp->b), tc = ldg4(tp->c);
+                        hit = Pred::test(c, ta, tb, tc, 0.0f, t, u, v);
+                        prim = __float_as_uint(ta.w);
+                    }
+                    const bool better = hit && t < cbest;
+                    float mn = better ? t : kInf;
+                    for (uint32_t o = gs >> 1; o > 0; o >>= 1) mn = fminf(mn, __shfl_xor(mn, (int)o, kWave));
+                    // the lowest lane holding the minimum is the first in leaf order
+                    const uint64_t win = __ballot(better && t == mn) & gmask;
+                    const uint64_t missed = __ballot(j < cnt && !hit) & gmask;
+                    const int w = win ? __ffsll((unsigned long long)win) - 1 : lane;
+                    const float wu = __shfl(u, w), wv = __shfl(v, w);
+                    const uint32_t wp = __shfl(prim, w);
+                    if (work && win) {
+                        cbest = mn;  // the first closest hit that beats the record, in leaf order
+                        cbu = wu;
+                        cbv = wv;
+                        cprim = wp;
+                    } else if (work && cbest == kInf && missed) {
+                        cbest = 3.40282347e+38f;  // an empty record, as after a miss
+                    }
+                    cpos += cnt;
+                }
+                // each source lane takes its group's record from the group's first lane
+                const uint32_t rank = (uint32_t)__popcll(longm & ((1ull << lane) - 1ull));
+                const int from = ((longm >> lane) & 1ull) ? (int)(rank * gs) : lane;
+                const float nb = __shfl(cbest, from), nu = __shfl(cbu, from), nv = __shfl(cbv, from);
+                const uint32_t npr = __shfl(cprim, from), npos = __shfl(cpos, from);
+                if ((longm >> lane) & 1ull) {
+                    best = nb;
+                    bu = nu;
+                    bv = nv;
+                    bprim = npr;
+                    lpos = npos;
+                }
+            }
+        }
 
         while (active) {
             bool finish = false;
