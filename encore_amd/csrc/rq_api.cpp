@@ -126,10 +126,6 @@ static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, ui
     }
     L.counts = counts;
     L.fetch_threshold = (uint32_t)(sc->accel == RQ_ACCEL_KD ? dev->kc.kd_fetch_threshold : dev->kc.fetch_threshold);
-    // kd launches claim rays along a Weyl sequence (RQ_KD_SCATTER=0 keeps
-    // the input order, for A/B checks); results do not depend on the order.
-    const char* sc_env = std::getenv("RQ_KD_SCATTER");
-    L.scatter = (sc->accel == RQ_ACCEL_KD && !(sc_env && sc_env[0] == '0')) ? 1u : 0u;
     launch_zero_u32(scratch, 2, s);
     if (counts)
         launch_count(L, sc->accel, sc->predicate, kind, sc->width, grid, s);

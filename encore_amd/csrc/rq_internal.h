@@ -309,7 +309,6 @@ struct TraceLaunch {
     uint32_t nitems;
     uint32_t fetch_threshold;    // refill idle lanes when fewer than this many are busy
     float root_pos[3], root_size[3];  // kd only
-    uint32_t scatter;        // kd: claim rays along a Weyl sequence instead of in order
 };
 
 enum class TraceKind { Closest = 0, Any = 1 };

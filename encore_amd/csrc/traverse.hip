@@ -750,25 +750,6 @@ __device__ __forceinline__ uint32_t fetch_capped(uint32_t cur, uint32_t* counter
 constexpr uint32_t kKdCoopLanes = 16;
 constexpr uint32_t kKdCoopMin = 16;
 
-// An odd stride near n / phi that is coprime with n: j -> (j * stride) mod n
-// is then a bijection of [0, n) (a Weyl sequence), and every run of
-// consecutive claims samples the whole ray array.
-__device__ uint32_t weyl_stride(uint32_t n) {
-    if (n < 3) return 1;
-    uint32_t s = (uint32_t)(((uint64_t)n * 2654435769ull) >> 32) | 1u;
-    for (;;) {
-        uint32_t a = s, b = n;
-        while (b) {
-            const uint32_t t = a % b;
-            a = b;
-            b = t;
-        }
-        if (a == 1) return s;
-        s += 2;
-        if (s >= n) s = 1;
-    }
-}
-
 // Lanes run a state machine -- one node step, or up to four triangle tests of
 // the current leaf, per iteration -- and idle lanes are refilled from the
 // counter whenever fewer than fetch_threshold lanes of the wave are busy, so
@@ -797,10 +778,9 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
     uint32_t bprim = kNone, cur = 0, lpos = 0, lend = 0;
     uint32_t n_nodes = 0, n_tris = 0;  // COUNT only
     // Rays of ENCORE's tree differ in cost by three orders of magnitude (a
-    // ray along the ground plane tests tens of thousands of triangles), and
-    // costly rays come in runs of the input order; claimed in that order they
-    // end the launch with a long tail.  A Weyl sequence spreads them.
-    const uint32_t stride = L.scatter ? weyl_stride(nrays) : 1u;
+    // ray along the ground plane tests tens of thousands of triangles).  A
+    // launch ends with its costliest rays; the per-wave cap and the
+    // cooperative leaf tests below shorten that tail.
     bool drained = false;  // the work counter has run out (wave-uniform)
     const uint32_t waves = gridDim.x * (blockDim.x / kWave);
     const uint32_t cap = COUNT ? (uint32_t)kWave
@@ -808,8 +788,6 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
 
     for (;;) {
         ray = fetch_capped(ray, L.counter, nrays, cap, drained);
-        if (ray != kNone && ray != kDone && !active && stride != 1u)
-            ray = (uint32_t)(((uint64_t)ray * stride) % nrays);
         if (ray != kNone && ray != kDone && !active) {
             float rtmin;
             init_ray(r, L.rays, ray, rtmin, rtmax);
@@ -843,7 +821,7 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
             drained = drained || __ballot(ray == kDone) != 0;
             // The wave cannot take more rays now (the counter has run out, or
             // it holds its cap) and few lanes are busy.
-            const uint32_t busy = (uint32_t)__popcll(__ballot(active));
+            const uint32_t busy = (drained || cap < (uint32_t)kWave) ? (uint32_t)__popcll(__ballot(active)) : 0xFFu;
             const bool coop = (drained || busy >= cap) && busy <= kKdCoopLanes;
             const uint64_t longm = coop ? __ballot(active && lend - lpos > kKdCoopMin) : 0ull;
             const uint32_t nl = (uint32_t)__popcll(longm);
