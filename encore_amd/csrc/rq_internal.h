@@ -309,9 +309,14 @@ struct TraceLaunch {
     uint32_t nitems;
     uint32_t fetch_threshold;    // refill idle lanes when fewer than this many are busy
     float root_pos[3], root_size[3];  // kd only
+    uint32_t coop_lanes, coop_min;    // kd: cooperative leaf tests (see kd_trace)
 };
 
 enum class TraceKind { Closest = 0, Any = 1 };
+// kd cooperative leaf tests: at most this many busy lanes, and leaves with
+// more than this many triangles left.
+constexpr uint32_t kKdCoopLanesDefault = 16;
+constexpr uint32_t kKdCoopMinDefault = 16;
 
 struct KernelConfig {
     int block = 256;

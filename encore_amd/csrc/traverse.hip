@@ -745,10 +745,6 @@ __device__ __forceinline__ uint32_t fetch_capped(uint32_t cur, uint32_t* counter
     return id < n ? id : kDone;
 }
 
-// Cooperative leaf testing in the kd tail: at most this many lanes in
-// leaves with more than kKdCoopMin triangles left.
-constexpr uint32_t kKdCoopLanes = 16;
-constexpr uint32_t kKdCoopMin = 16;
 
 // Lanes run a state machine -- one node step, or up to four triangle tests of
 // the current leaf, per iteration -- and idle lanes are refilled from the
@@ -822,10 +818,10 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
             // The wave cannot take more rays now (the counter has run out, or
             // it holds its cap) and few lanes are busy.
             const uint32_t busy = (drained || cap < (uint32_t)kWave) ? (uint32_t)__popcll(__ballot(active)) : 0xFFu;
-            const bool coop = (drained || busy >= cap) && busy <= kKdCoopLanes;
-            const uint64_t longm = coop ? __ballot(active && lend - lpos > kKdCoopMin) : 0ull;
+            const bool coop = (drained || busy >= cap) && busy <= L.coop_lanes;
+            const uint64_t longm = coop ? __ballot(active && lend - lpos > L.coop_min) : 0ull;
             const uint32_t nl = (uint32_t)__popcll(longm);
-            if (nl != 0 && nl <= kKdCoopLanes) {
+            if (nl != 0 && nl <= L.coop_lanes) {
                 // The wave splits into nl groups of gs lanes (gs = 64 / nl
                 // rounded down to a power of two); group g serves the ray of
                 // the g-th lane in longm.

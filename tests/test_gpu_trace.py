@@ -107,6 +107,26 @@ def test_sphere_kd_path_is_bit_exact(device, sphere):
     assert np.array_equal(occ.astype(np.uint32), s["shadow_ref"])
 
 
+@pytest.mark.parametrize("coop", ["16,16", "0,16", "64,1"])
+def test_sphere_kd_small_launches_are_bit_exact(device, sphere, coop, monkeypatch):
+    """Launches of fewer rays than the GPU has lanes: each wave holds at most
+    ceil(n / waves) rays and tests long leaves in lane groups (any group
+    size, or none: RQ_KD_COOP_LANES, RQ_KD_COOP_MIN).  Results stay those of
+    the oracle's kd-tree, bit for bit."""
+    s = sphere
+    lanes, mn = coop.split(",")
+    monkeypatch.setenv("RQ_KD_COOP_LANES", lanes)
+    monkeypatch.setenv("RQ_KD_COOP_MIN", mn)
+    log("case: sphere L5 kd small launches, cooperative lanes %s, min %s" % (lanes, mn))
+    sc = rq.Scene(device, s["xyz"], s["idx"], accel="kd")
+    rays, ref = s["bounce"], s["bounce_ref"]
+    for lo, hi in ((0, 1), (1, 8), (8, 72), (72, 1072), (1072, 9072)):
+        got = _trace(sc, np.ascontiguousarray(rays[lo:hi]))
+        assert np.array_equal(got.view(np.uint32), ref[lo:hi].view(np.uint32)), (lo, hi)
+    occ = _trace(sc, np.ascontiguousarray(s["shadow"][:3000]), "any")
+    assert np.array_equal(occ.astype(np.uint32), s["shadow_ref"][:3000])
+
+
 @pytest.mark.parametrize("width", [2, 4])
 def test_sphere_bvh_occluded(device, sphere, width):
     s = sphere
