@@ -401,6 +401,81 @@ rq_status rq_scene_refit(rq_scene* sc, const float* d_xyz, uint32_t nverts, void
     });
 }
 
+rq_status rq_scene_create_kd(rq_device* dev, const float* xyz, uint32_t nverts, const uint32_t* tri_idx,
+                             uint32_t ntris, const void* nodes, uint32_t nnodes, const uint32_t* leaf_prims,
+                             uint32_t nrefs, const float* root_box, uint32_t predicate, rq_scene** out) {
+    return guarded([&] {
+        require(dev && xyz && tri_idx && nodes && root_box && out, "null argument");
+        require(ntris > 0 && nverts > 0, "empty triangle list");
+        require(nnodes > 0, "a kd-tree has at least its root");
+        require(nrefs == 0 || leaf_prims, "null leaf list");
+        require(predicate == RQ_PRED_ENCORE || predicate == RQ_PRED_WATERTIGHT, "unknown predicate");
+        for (size_t i = 0; i < 3 * (size_t)ntris; ++i) require(tri_idx[i] < nverts, "triangle index out of range");
+        for (uint32_t j = 0; j < nrefs; ++j) require(leaf_prims[j] < ntris, "leaf triangle out of range");
+        const KdNodeG* nd = static_cast<const KdNodeG*>(nodes);
+        // Walk the tree from the root: every node reached once, children and
+        // leaf ranges in bounds; the depth sizes the traversal stack.
+        std::vector<uint32_t> depth(nnodes, 0xFFFFFFFFu);
+        std::vector<uint32_t> todo{0};
+        depth[0] = 0;
+        uint32_t maxd = 0, nleaves = 0;
+        while (!todo.empty()) {
+            const uint32_t i = todo.back();
+            todo.pop_back();
+            maxd = std::max(maxd, depth[i]);
+            if ((nd[i].word & 3u) == 3u) {
+                const uint64_t cnt = nd[i].word >> 2;
+                require((uint64_t)nd[i].child + cnt <= nrefs, "leaf range outside leaf_prims");
+                ++nleaves;
+                continue;
+            }
+            const uint32_t c = nd[i].child;
+            require((uint64_t)c + 1 < nnodes, "child index out of range");
+            for (uint32_t k = c; k <= c + 1; ++k) {
+                require(depth[k] == 0xFFFFFFFFu, "kd node array is not a tree");
+                depth[k] = depth[i] + 1;
+                todo.push_back(k);
+            }
+        }
+        RQ_HIP(hipSetDevice(dev->ordinal));
+        auto sc = std::make_unique<rq_scene>();
+        sc->dev = dev;
+        sc->accel = RQ_ACCEL_KD;
+        sc->predicate = predicate;
+        sc->ntris = ntris;
+        std::vector<TriRec> tris;
+        pack_triangles(xyz, tri_idx, leaf_prims, nrefs, predicate, tris);
+        std::vector<uint32_t> slot(ntris, 0xFFFFFFFFu);
+        for (uint32_t j = 0; j < nrefs; ++j)
+            if (slot[leaf_prims[j]] == 0xFFFFFFFFu) slot[leaf_prims[j]] = j;
+        for (uint32_t& v : slot)
+            if (v == 0xFFFFFFFFu) v = 0;  // in no leaf: never hit
+        for (int k = 0; k < 3; ++k) {
+            sc->root_pos[k] = root_box[k];
+            sc->root_size[k] = root_box[3 + k];
+        }
+        sc->stats = BuildStats{};
+        sc->stats.nnodes = nnodes;
+        sc->stats.nleaves = nleaves;
+        sc->stats.nrefs = nrefs;
+        sc->stats.max_depth = maxd + 1;
+        sc->stats.stack_bound = maxd + 2;
+        sc->nodes = DevBuf((size_t)nnodes * sizeof(KdNodeG));
+        sc->nodes.upload(nodes, sc->nodes.bytes);
+        sc->tris = DevBuf(std::max<size_t>(1, tris.size()) * sizeof(TriRec));
+        if (!tris.empty()) sc->tris.upload(tris.data(), tris.size() * sizeof(TriRec));
+        sc->prim_slot = DevBuf(slot.size() * sizeof(uint32_t));
+        sc->prim_slot.upload(slot.data(), sc->prim_slot.bytes);
+        sc->nverts = nverts;
+        sc->idx_host.assign(tri_idx, tri_idx + 3 * (size_t)ntris);
+        sc->desc = rq_scene_desc{};
+        sc->desc.accel = RQ_ACCEL_KD;
+        sc->desc.predicate = predicate;
+        sc->device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->prim_slot.bytes;
+        *out = sc.release();
+    });
+}
+
 void rq_scene_destroy(rq_scene* scene) {
     if (!scene) return;
     (void)hipSetDevice(scene->dev->ordinal);

@@ -158,6 +158,10 @@ def load():
         L.rq_final_gather.restype = ctypes.c_int
     if hasattr(L, "rq_scene_refit"):
         L.rq_scene_refit.argtypes = [_vp, _vp, _u32, _vp]
+    if hasattr(L, "rq_scene_create_kd"):
+        L.rq_scene_create_kd.argtypes = [_vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32,
+                                         ctypes.POINTER(_vp)]
+        L.rq_scene_create_kd.restype = ctypes.c_int
         L.rq_scene_refit.restype = ctypes.c_int
     if hasattr(L, "rq_emit_photons"):
         L.rq_scene_set_materials.argtypes = [_vp, _vp, _u32, _vp]
@@ -283,6 +287,27 @@ class Scene:
     def save(self, path):
         """Write the scene to a packed scene file."""
         check(load().rq_scene_save(self.h, os.fsencode(path)))
+
+    @classmethod
+    def from_kd(cls, device, xyz, idx, words, childs, leaf_prims, root_box, predicate="encore"):
+        """A kd scene over a kd-tree built elsewhere (rq_scene_create_kd):
+        node words and children in Kdtree's numbering, a leaf's child being
+        the offset of its triangles in leaf_prims; root_box = pos, size."""
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
+        idx = np.ascontiguousarray(idx, np.uint32).reshape(-1)
+        nodes = np.ascontiguousarray(np.stack([np.asarray(words, np.uint32), np.asarray(childs, np.uint32)], axis=1))
+        lp = np.ascontiguousarray(leaf_prims, np.uint32).reshape(-1)
+        box = np.ascontiguousarray(root_box, np.float32).reshape(6)
+        h = _vp()
+        check(load().rq_scene_create_kd(device.h, xyz.ctypes.data, len(xyz) // 3, idx.ctypes.data, len(idx) // 3,
+                                        nodes.ctypes.data, len(nodes), lp.ctypes.data if len(lp) else None, len(lp),
+                                        box.ctypes.data, PRED[predicate], ctypes.byref(h)))
+        sc = cls.__new__(cls)
+        sc.h = h
+        sc.device = device
+        sc.accel = "kd"
+        sc.predicate = predicate
+        return sc
 
     @classmethod
     def load(cls, device, path):

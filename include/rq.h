@@ -144,6 +144,21 @@ void rq_device_close(rq_device* dev);
 rq_status rq_scene_create(rq_device* dev, const float* xyz, uint32_t nverts,
                           const uint32_t* tri_idx, uint32_t ntris,
                           const rq_scene_desc* desc, rq_scene** out);
+/* A kd scene over a kd-tree built elsewhere -- ENCORE's own Kdtree, kept
+ * instead of built again (Kdtree.h:25-58, 144-176: m_pTree[0..m_NodeUsed),
+ * each leaf's m_pPrimList, m_Bounds) -- in the form rq_host_tree_copy
+ * writes: nnodes nodes of 8 bytes {word, child}.  A leaf's word is
+ * (count << 2) | 3 and its child the offset of its triangles in leaf_prims;
+ * an inner node's word is the split's float bits with the axis in the low
+ * two bits and its child the left child, the right one following it.
+ * leaf_prims holds nrefs triangle indices into tri_idx; root_box = pos[3],
+ * size[3] of the root cell.  The tree is checked (children and leaf ranges
+ * in bounds, no node reached twice) and traversed as the kd scenes
+ * rq_scene_create builds.  rq_scene_refit rebuilds such a scene with the
+ * engine's own builder, as Kdtree::update rebuilds. */
+rq_status rq_scene_create_kd(rq_device* dev, const float* xyz, uint32_t nverts, const uint32_t* tri_idx,
+                             uint32_t ntris, const void* nodes, uint32_t nnodes, const uint32_t* leaf_prims,
+                             uint32_t nrefs, const float* root_box, uint32_t predicate, rq_scene** out);
 void rq_scene_destroy(rq_scene* scene);
 rq_status rq_scene_stats(const rq_scene* scene, rq_scene_info* info);
 
