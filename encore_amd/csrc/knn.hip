@@ -565,12 +565,24 @@ __device__ __forceinline__ uint2 cell_lookup(const PhotonGridView& G, uint64_t k
 // chunk of 64.
 constexpr int kCellCand = 2 * (int)kKnnMaxK + 64;
 
-__global__ __launch_bounds__(64) void knn_cell_kernel(KnnLaunch L) {
-    __shared__ uint32_t s_key[kCellCand];
-    __shared__ uint32_t s_slot[kCellCand];
-    __shared__ uint32_t s_hist[256];
-    __shared__ uint32_t s_first[64], s_pre[64];
-    __shared__ float s_near[64];
+// Workgroups of kCellWaves waves, one query per wave: a workgroup's waves
+// are spread over the CU's SIMDs, so a cap of whole workgroups per CU puts
+// the same number of waves on every SIMD.
+constexpr int kCellWaves = 4;
+
+__global__ __launch_bounds__(64 * kCellWaves) void knn_cell_kernel(KnnLaunch L) {
+    __shared__ uint32_t s_key_w[kCellWaves][kCellCand];
+    __shared__ uint32_t s_slot_w[kCellWaves][kCellCand];
+    __shared__ uint32_t s_hist_w[kCellWaves][256];
+    __shared__ uint32_t s_first_w[kCellWaves][64], s_pre_w[kCellWaves][64];
+    __shared__ float s_near_w[kCellWaves][64];
+    const uint32_t wv = threadIdx.x / kKnnWave;
+    uint32_t* s_key = s_key_w[wv];
+    uint32_t* s_slot = s_slot_w[wv];
+    uint32_t* s_hist = s_hist_w[wv];
+    uint32_t* s_first = s_first_w[wv];
+    uint32_t* s_pre = s_pre_w[wv];
+    float* s_near = s_near_w[wv];
     const PhotonGridView& G = L.grid;
     const uint32_t lane = __lane_id();
     const uint32_t k = L.k;
@@ -578,16 +590,17 @@ __global__ __launch_bounds__(64) void knn_cell_kernel(KnnLaunch L) {
     // Grid-stride over the prepared queries (cell order when sorted): the
     // waves running at one time hold a window of consecutive queries, which
     // share cells.  The next query's record is loaded a query ahead.
-    uint32_t qi = blockIdx.x;
+    const uint32_t nwaves = gridDim.x * kCellWaves;
+    uint32_t qi = blockIdx.x * kCellWaves + wv;
     float4 nq4 = qi < L.nq ? L.qprep[qi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     uint32_t nq_id = qi < L.nq ? (L.order ? L.order[qi] : qi) : 0u;
-    for (; qi < L.nq; qi += gridDim.x) {
+    for (; qi < L.nq; qi += nwaves) {
         const uint32_t q = nq_id;
         const float qx = nq4.x, qy = nq4.y, qz = nq4.z;
         const uint32_t cls = __float_as_uint(nq4.w);
-        if (qi + gridDim.x < L.nq) {
-            nq4 = L.qprep[qi + gridDim.x];
-            nq_id = L.order ? L.order[qi + gridDim.x] : qi + gridDim.x;
+        if (qi + nwaves < L.nq) {
+            nq4 = L.qprep[qi + nwaves];
+            nq_id = L.order ? L.order[qi + nwaves] : qi + nwaves;
         }
         uint32_t cand = 0;
         bool far = false;
@@ -781,14 +794,18 @@ void launch_knn_wave(const KnnLaunch& L, int grid, uint32_t stack_cap, hipStream
 
 int knn_cell_grid(uint32_t nq, int num_cus) {
     int per_cu = 0;
-    RQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)dev::knn_cell_kernel, dev::kKnnWave, 0));
-    // Five waves per SIMD: the 4M-query final gather ran 146 M rays/s with
-    // 20 one-wave blocks per CU and 113 M with 24 (more waves evict each
-    // other's cells).  RQ_KNN_CELL_WAVES lowers the count for sweeps.
+    RQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)dev::knn_cell_kernel,
+                                                        dev::kKnnWave * dev::kCellWaves, 0));
+    // Five waves per SIMD (20 per CU): with one-wave workgroups the 4M-query
+    // final gather ran 160 M rays/s at 5 waves per SIMD, 146-149 M when 20
+    // workgroups per CU could land 6, 6, 6, 2 on the SIMDs, and 113 M with
+    // 24 per CU (more waves evict each other's cells).  RQ_KNN_CELL_WAVES
+    // lowers the count for sweeps.
     int cap = 20;
     if (const char* e = std::getenv("RQ_KNN_CELL_WAVES")) cap = std::atoi(e);
-    per_cu = std::max(1, std::min(per_cu, cap));
-    return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)nq, (int64_t)std::max(1, per_cu) * num_cus));
+    per_cu = std::max(1, std::min(per_cu, std::max(1, cap / dev::kCellWaves)));
+    const int64_t need = ((int64_t)nq + dev::kCellWaves - 1) / dev::kCellWaves;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)per_cu * num_cus));
 }
 
 void launch_knn_cells(const KnnLaunch& L, float4* prep, int grid, hipStream_t s) {
@@ -797,7 +814,7 @@ void launch_knn_cells(const KnnLaunch& L, float4* prep, int grid, hipStream_t s)
     RQ_HIP(hipGetLastError());
     KnnLaunch C = L;
     C.qprep = prep;
-    hipLaunchKernelGGL(dev::knn_cell_kernel, dim3(grid), dim3(dev::kKnnWave), 0, s, C);
+    hipLaunchKernelGGL(dev::knn_cell_kernel, dim3(grid), dim3(dev::kKnnWave * dev::kCellWaves), 0, s, C);
     RQ_HIP(hipGetLastError());
 }
 
