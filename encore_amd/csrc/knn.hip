@@ -10,10 +10,16 @@
 // as the splitting plane, so the selected set is the same up to photons tied
 // with the k-th distance.
 //
-// One lane per query.  Each lane keeps a bounded max-heap of (d2, slot) in
-// LDS, laid out [entry][lane] so lanes at different heap positions still hit
-// different banks, and a short ring stack of pending ranges (LDS, spilling to
-// HBM).  Results are written nearest first.
+// Three kernels give that answer:
+//  - knn_kernel (k < 32, or no cell grid): one lane per query, a bounded
+//    max-heap of (d2, slot) in LDS laid out [entry][lane] so lanes at
+//    different heap positions still hit different banks, and a short ring
+//    stack of pending ranges (LDS, spilling to HBM);
+//  - knn_cell_kernel (k >= 32): one query per wave over the photon cell grid
+//    of photon_grid.hip, lists and/or radiance;
+//  - knn_wave_kernel (k >= 32, radiance only, when no grid suits the radius):
+//    one query per wave walking the tree.
+// Lists are written nearest first.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 

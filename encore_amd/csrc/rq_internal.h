@@ -258,7 +258,8 @@ constexpr uint32_t kKnnMaxK = 256;
 uint32_t knn_spill_words_per_lane(uint32_t n);
 // Radiance-only gather for large k, one query per wave (knn.hip): the stack
 // capacity (entries per wave; three words each) for n photons, the grid, the
-// launch.  The kernel is used for k >= kKnnWaveMinK when no lists are asked.
+// launch.  From k = kKnnWaveMinK the cell-grid kernel serves queries; this
+// one serves radiance without lists when no grid suits the radius.
 constexpr uint32_t kKnnWaveMinK = 32;
 uint32_t knn_wave_stack_cap(uint32_t n);
 // Radiance without lists from the photon cell grid, one query per wave.
