@@ -125,6 +125,16 @@ def test_sphere_kd_small_launches_are_bit_exact(device, sphere, coop, monkeypatc
         assert np.array_equal(got.view(np.uint32), ref[lo:hi].view(np.uint32)), (lo, hi)
     occ = _trace(sc, np.ascontiguousarray(s["shadow"][:3000]), "any")
     assert np.array_equal(occ.astype(np.uint32), s["shadow_ref"][:3000])
+    # the watertight predicate through the same groups: equal to the lanes alone
+    wt = rq.Scene(device, s["xyz"], s["idx"], accel="kd", predicate="watertight")
+    parts = ((0, 7), (7, 1007), (1007, 9007))
+    got = [_trace(wt, np.ascontiguousarray(rays[lo:hi])) for lo, hi in parts]
+    monkeypatch.setenv("RQ_KD_COOP_LANES", "0")
+    ref_wt = [_trace(wt, np.ascontiguousarray(rays[lo:hi])) for lo, hi in parts]
+    for a, b in zip(got, ref_wt):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    wt.destroy()
+    sc.destroy()
 
 
 @pytest.mark.parametrize("width", [2, 4])
