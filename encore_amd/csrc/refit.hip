@@ -112,33 +112,34 @@ __device__ __forceinline__ Box child_box(const RefitLaunch& L, uint32_t ref) {
     return (ref & kRefLeaf) ? leaf_box(L, ref) : node_box(L, ref);
 }
 
+// One thread per child slot of the level's nodes (4 per BVH4 node, 2 per
+// BVH2 node): each computes one child box and writes its own columns.
 __global__ void refit_level(RefitLaunch L) {
-    const uint32_t i = L.begin + blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t w = L.width == 4 ? 4u : 2u;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t i = L.begin + t / w;
     if (i >= L.end) return;
+    const int c = (int)(t % w);
     const uint32_t node = L.order[i];
     if (L.width == 4) {
         BvhNode4& n = static_cast<BvhNode4*>(L.nodes)[node];
-        for (int c = 0; c < 4; ++c) {
-            if (n.ref[c] == kRefEmpty) continue;
-            const Box b = child_box(L, n.ref[c]);
-            n.lox[c] = b.lo[0];
-            n.hix[c] = b.hi[0];
-            n.loy[c] = b.lo[1];
-            n.hiy[c] = b.hi[1];
-            n.loz[c] = b.lo[2];
-            n.hiz[c] = b.hi[2];
-        }
+        if (n.ref[c] == kRefEmpty) return;
+        const Box b = child_box(L, n.ref[c]);
+        n.lox[c] = b.lo[0];
+        n.hix[c] = b.hi[0];
+        n.loy[c] = b.lo[1];
+        n.hiy[c] = b.hi[1];
+        n.loz[c] = b.lo[2];
+        n.hiz[c] = b.hi[2];
     } else {
         BvhNode& n = static_cast<BvhNode*>(L.nodes)[node];
-        for (int c = 0; c < 2; ++c) {
-            const Box b = child_box(L, n.ref[c]);
-            n.x[2 * c] = b.lo[0];
-            n.x[2 * c + 1] = b.hi[0];
-            n.y[2 * c] = b.lo[1];
-            n.y[2 * c + 1] = b.hi[1];
-            n.z[2 * c] = b.lo[2];
-            n.z[2 * c + 1] = b.hi[2];
-        }
+        const Box b = child_box(L, n.ref[c]);
+        n.x[2 * c] = b.lo[0];
+        n.x[2 * c + 1] = b.hi[0];
+        n.y[2 * c] = b.lo[1];
+        n.y[2 * c + 1] = b.hi[1];
+        n.z[2 * c] = b.lo[2];
+        n.z[2 * c + 1] = b.hi[2];
     }
 }
 
@@ -152,7 +153,8 @@ void launch_refit_tris(const RefitLaunch& L, hipStream_t s) {
 
 void launch_refit_level(const RefitLaunch& L, hipStream_t s) {
     if (L.end <= L.begin) return;
-    hipLaunchKernelGGL(dev::refit_level, dim3((L.end - L.begin + 255) / 256), dim3(256), 0, s, L);
+    const uint64_t threads = (uint64_t)(L.end - L.begin) * (L.width == 4 ? 4u : 2u);
+    hipLaunchKernelGGL(dev::refit_level, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, L);
     RQ_HIP(hipGetLastError());
 }
 
