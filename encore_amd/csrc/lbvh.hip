@@ -274,6 +274,16 @@ __global__ void lb_pack(LbLaunch L) {
 
 static unsigned grid_of(uint64_t n, unsigned block = 256) { return (unsigned)((n + block - 1) / block); }
 
+// A piece of a larger device allocation.
+struct Carved {
+    void* ptr;
+    size_t bytes;
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
 void build_bvh_device(rq_scene* sc, const float* xyz, uint32_t nverts, const uint32_t* idx, uint32_t ntris,
                       uint32_t leaf_size, hipStream_t s) {
     using dev::LbLaunch;
@@ -281,12 +291,31 @@ void build_bvh_device(rq_scene* sc, const float* xyz, uint32_t nverts, const uin
     if (leaf_size > kLeafMaxCount) throw Error(RQ_ERR_INVALID_ARG, "leaf_size must be 1..16");
     if (ntris > kLeafMaxFirst) throw Error(RQ_ERR_CAPACITY, "BVH supports at most 2^27-1 triangles");
     const size_t n = ntris, ni = std::max<size_t>(1, n - 1);
-    DevBuf d_xyz(3 * (size_t)nverts * 4), d_idx(3 * n * 4);
+    // The temporaries share one allocation (a hipMalloc and a hipFree each
+    // cost tens of microseconds); the arrays the scene keeps have their own.
+    size_t sort_bytes = 0;
+    RQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (unsigned long long*)nullptr,
+                                              (unsigned long long*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                              (int)n, 0, 63, s));
+    const size_t sizes[] = {3 * (size_t)nverts * 4, 3 * n * 4, 6 * n * 4, 3 * n * 4, 64, n * 8, n * 8, n * 4, n * 4,
+                            2 * ni * 4, ni * 4, ni * 4, ni * 4, ni * 4, 2 * dev::kLbMaxLevels * 4, 64,
+                            std::max<size_t>(sort_bytes, 16)};
+    size_t total = 0;
+    for (size_t b : sizes) total += (b + 255) & ~(size_t)255;
+    DevBuf work(total);
+    size_t at = 0;
+    auto take = [&](size_t b) {
+        Carved c{static_cast<char*>(work.ptr) + at, b};
+        at += (b + 255) & ~(size_t)255;
+        return c;
+    };
+    const Carved d_xyz = take(sizes[0]), d_idx = take(sizes[1]), tri_box = take(sizes[2]), cen = take(sizes[3]),
+                 cb = take(sizes[4]), keys = take(sizes[5]), skeys = take(sizes[6]), ids = take(sizes[7]),
+                 sids = take(sizes[8]), child = take(sizes[9]), first = take(sizes[10]), last = take(sizes[11]),
+                 parent_in = take(sizes[12]), depth = take(sizes[13]), level_cnt = take(sizes[14]),
+                 stats = take(sizes[15]), tmp = take(sizes[16]);
     RQ_HIP(hipMemcpyAsync(d_xyz.ptr, xyz, d_xyz.bytes, hipMemcpyHostToDevice, s));
     RQ_HIP(hipMemcpyAsync(d_idx.ptr, idx, d_idx.bytes, hipMemcpyHostToDevice, s));
-    DevBuf tri_box(6 * n * 4), cen(3 * n * 4), cb(64), keys(n * 8), skeys(n * 8), ids(n * 4), sids(n * 4),
-        child(2 * ni * 4), first(ni * 4), last(ni * 4), parent_in(ni * 4), depth(ni * 4),
-        level_cnt(2 * dev::kLbMaxLevels * 4), stats(64);
     DevBuf nodes(ni * sizeof(BvhNode)), tris(n * sizeof(TriRec)), slot(n * 4), vidx(3 * n * 4), order(ni * 4);
     LbLaunch L{};
     L.xyz = d_xyz.as<float>();
@@ -324,10 +353,7 @@ void build_bvh_device(rq_scene* sc, const float* xyz, uint32_t nverts, const uin
     RQ_HIP(hipGetLastError());
     // Sort (key, triangle) pairs by key; the sort is stable, so equal keys
     // keep triangle order.
-    size_t tmp_bytes = 0;
-    RQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, L.keys, skeys.as<unsigned long long>(), L.ids,
-                                              sids.as<uint32_t>(), (int)n, 0, 63, s));
-    DevBuf tmp(std::max<size_t>(tmp_bytes, 16));
+    size_t tmp_bytes = tmp.bytes;
     RQ_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tmp_bytes, L.keys, skeys.as<unsigned long long>(), L.ids,
                                               sids.as<uint32_t>(), (int)n, 0, 63, s));
     L.skeys = skeys.as<unsigned long long>();

@@ -330,7 +330,7 @@ rq_status rq_scene_create(rq_device* dev, const float* xyz, uint32_t nverts, con
         require(dd.builder == RQ_BUILD_HOST_SAH || dd.builder == RQ_BUILD_DEVICE_LBVH, "unknown builder");
         if (dd.accel == RQ_ACCEL_BVH && dd.builder == RQ_BUILD_DEVICE_LBVH) {
             if (dd.bvh_width == 4) throw Error(RQ_ERR_UNSUPPORTED, "the device builder makes BVH2 scenes");
-            for (size_t i = 0; i < 3 * (size_t)ntris; ++i) require(tri_idx[i] < nverts, "triangle index out of range");
+            require(*std::max_element(tri_idx, tri_idx + 3 * (size_t)ntris) < nverts, "triangle index out of range");
             build_bvh_device(sc.get(), xyz, nverts, tri_idx, ntris, dd.leaf_size, nullptr);
         } else if (dd.accel == RQ_ACCEL_BVH) {
             require(dd.bvh_width == 0 || dd.bvh_width == 2 || dd.bvh_width == 4, "bvh_width must be 0, 2 or 4");
@@ -410,8 +410,8 @@ rq_status rq_scene_create_kd(rq_device* dev, const float* xyz, uint32_t nverts, 
         require(nnodes > 0, "a kd-tree has at least its root");
         require(nrefs == 0 || leaf_prims, "null leaf list");
         require(predicate == RQ_PRED_ENCORE || predicate == RQ_PRED_WATERTIGHT, "unknown predicate");
-        for (size_t i = 0; i < 3 * (size_t)ntris; ++i) require(tri_idx[i] < nverts, "triangle index out of range");
-        for (uint32_t j = 0; j < nrefs; ++j) require(leaf_prims[j] < ntris, "leaf triangle out of range");
+        require(*std::max_element(tri_idx, tri_idx + 3 * (size_t)ntris) < nverts, "triangle index out of range");
+        require(nrefs == 0 || *std::max_element(leaf_prims, leaf_prims + nrefs) < ntris, "leaf triangle out of range");
         const KdNodeG* nd = static_cast<const KdNodeG*>(nodes);
         // Walk the tree from the root: every node reached once, children and
         // leaf ranges in bounds; the depth sizes the traversal stack.
