@@ -159,6 +159,7 @@ void emit(rq_scene* sc, const rq_rect_light* lights, uint32_t nlights, const rq_
     for (int m = 0; m < 3; ++m) {
         require(std::isfinite(P.scale[m]), "map scales must be finite");
         if (P.capacity[m]) require(out && out[m].pos && out[m].normal && out[m].power, "null output buffer");
+        if (P.capacity[m] > kMaxCount) throw Error(RQ_ERR_CAPACITY, "at most 2^30 photons per map");
     }
     rq_photon_result r{};
     const uint32_t nslots = P.max_bounces + 2;

@@ -102,6 +102,7 @@ static int grid_for(rq_device* dev, uint32_t accel, uint32_t pred, TraceKind kin
 static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, uint32_t* occ, TraceKind kind,
                   hipStream_t s, unsigned long long* counts = nullptr, const uint32_t* n_dev = nullptr) {
     if (n == 0) return;
+    if (n > kMaxCount) throw Error(RQ_ERR_CAPACITY, "at most 2^30 rays per call");
     rq_device* dev = sc->dev;
     RQ_HIP(hipSetDevice(dev->ordinal));
     const int grid = grid_for(dev, sc->accel, sc->predicate, kind, sc->width, n);
@@ -561,6 +562,7 @@ rq_status rq_photons_create(rq_device* dev, const float* pos, const float* norma
     return guarded([&] {
         require(dev && out, "null argument");
         require(n == 0 || (pos && normal), "null photon arrays");
+        if (n > kMaxCount) throw Error(RQ_ERR_CAPACITY, "at most 2^30 photons per map");
         require(layout == RQ_PHOTONS_ENCORE_ORDER || layout == RQ_PHOTONS_SPATIAL, "unknown photon layout");
         RQ_HIP(hipSetDevice(dev->ordinal));
         PhotonHost h;
@@ -621,6 +623,7 @@ rq_status rq_photons_balance_host(const float* pos, uint32_t n, uint32_t layout,
     return guarded([&] {
         require(n == 0 || (pos && perm && axis), "null argument");
         require(layout == RQ_PHOTONS_ENCORE_ORDER || layout == RQ_PHOTONS_SPATIAL, "unknown photon layout");
+        if (n > kMaxCount) throw Error(RQ_ERR_CAPACITY, "at most 2^30 photons per map");
         std::vector<float> nrm(3 * (size_t)n, 0.0f);
         PhotonHost h;
         build_photons(pos, nrm.data(), nullptr, n, layout, h);
@@ -651,6 +654,7 @@ static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint
     require(k >= 1 && k <= kKnnMaxK, "k must be 1..256");
     require(nq == 0 || (qpos && qnrm), "null query arrays");
     if (nq == 0) return;
+    if (nq > kMaxCount) throw Error(RQ_ERR_CAPACITY, "at most 2^30 queries per call");
     rq_device* dev = p->dev;
     RQ_HIP(hipSetDevice(dev->ordinal));
     // Large k: one query per wave, from the photon cell grid when one suits
@@ -760,7 +764,7 @@ rq_status rq_final_gather(rq_scene* scene, rq_photons* indirect, uint32_t k_ind,
         const uint64_t max_rays = std::max<uint64_t>(nper, 1u << 22);
         const uint32_t chunk_pts = (uint32_t)std::max<uint64_t>(1, max_rays / nper);
         const uint64_t cap = (uint64_t)std::min<uint64_t>(chunk_pts, npoints) * nper;
-        require(cap <= 0xFFFFFFF0ull, "too many samples per point");
+        if (cap > kMaxCount) throw Error(RQ_ERR_CAPACITY, "at most 2^30 sample rays per shading point");
         const size_t per_ray = sizeof(rq_ray) + sizeof(rq_hit) + 4 * 3 * sizeof(float);
         DevBuf work(cap * per_ray + 1024);
         char* w = static_cast<char*>(work.ptr);

@@ -99,6 +99,9 @@ constexpr uint32_t kRefLeaf = 0x80000000u;
 constexpr uint32_t kRefSentinel = 0x7FFFFFFFu;
 constexpr uint32_t kLeafMaxCount = 16;
 constexpr uint32_t kLeafMaxFirst = (1u << 27) - 1;
+// Rays, queries or sample rays per call and photons per map or emission
+// buffer: the work counters and radix-sort sizes are 32-bit.
+constexpr uint32_t kMaxCount = 1u << 30;
 
 __host__ __device__ inline uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
     return kRefLeaf | ((count - 1) << 27) | first;

@@ -18,6 +18,9 @@
  *  - Scenes and photon maps change only through rq_scene_refit and
  *    rq_scene_set_materials; between such calls any number of threads may
  *    issue queries on one object concurrently on distinct streams.
+ *  - One call takes at most 2^30 rays, queries or sample rays, and a photon
+ *    map or emission buffer holds at most 2^30 photons (work counters and
+ *    sort sizes are 32-bit); larger counts return RQ_ERR_CAPACITY.
  */
 #ifndef ENCORE_RQ_H
 #define ENCORE_RQ_H

@@ -40,6 +40,19 @@ def test_invalid_arguments_are_reported():
     assert e.value.status == 1 and "range" in str(e.value)
 
 
+def test_counts_beyond_2_30_are_refused_before_any_read():
+    """Work counters and sort sizes are 32-bit: a photon map of more than
+    2^30 photons is refused with RQ_ERR_CAPACITY before the arrays are read
+    (the buffers here hold one photon)."""
+    pos = np.zeros(3, np.float32)
+    perm = np.zeros(1, np.uint32)
+    axis = np.zeros(1, np.uint8)
+    L = rq.load()
+    st = L.rq_photons_balance_host(pos.ctypes.data, (1 << 30) + 1, 1, perm.ctypes.data, axis.ctypes.data)
+    assert st == 5 and b"2^30" in L.rq_last_error()
+    assert L.rq_photons_balance_host(pos.ctypes.data, 1, 1, perm.ctypes.data, axis.ctypes.data) == 0
+
+
 @pytest.mark.parametrize("level", [None, 3, 5])
 def test_engine_kd_tree_is_encore_tree(level):
     """The engine's kd builder reproduces ENCORE's tree node for node."""
