@@ -9,6 +9,8 @@
 // 1M-ray batch share the GPU without a launch each.
 #include <algorithm>
 #include <chrono>
+#include <utility>
+#include <vector>
 
 #include "rq_internal.h"
 
@@ -232,26 +234,45 @@ rq_status rq_batch_autotune(rq_batch* b, void* stream, rq_kernel_config* chosen,
         float best_t = 1e30f;
         const int stacks[] = {8, 16, 32};
         const int thresholds[] = {32, 48, 56, 60};
+        // Mean time of `reps` back-to-back launches with configuration kc.
+        auto time_config = [&](const KernelConfig& kc, int reps) {
+            b->kc = kc;
+            launch(b, false, s);  // warm
+            RQ_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) launch(b, false, s);
+            RQ_HIP(hipEventRecord(e1, s));
+            RQ_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            RQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+            return ms / (float)reps;
+        };
         try {
+            // One launch per configuration, then the fastest few again over
+            // several launches: a single launch varies by a few percent, and
+            // the minimum of one-shot times favours a lucky launch.
+            std::vector<std::pair<float, KernelConfig>> sweep;
             for (int st : stacks) {
                 const int occ = occupancy_bvh_batched(b->predicate, kind_of(b), b->width, st);
                 for (int bpc = 1; bpc <= occ; bpc = (bpc < 4 ? bpc + 1 : bpc + 2)) {
                     for (int th : thresholds) {
-                        b->kc.stack = st;
-                        b->kc.blocks_per_cu = bpc;
-                        b->kc.fetch_threshold = th;
-                        launch(b, false, s);  // warm
-                        RQ_HIP(hipEventRecord(e0, s));
-                        launch(b, false, s);
-                        RQ_HIP(hipEventRecord(e1, s));
-                        RQ_HIP(hipEventSynchronize(e1));
-                        float ms = 0;
-                        RQ_HIP(hipEventElapsedTime(&ms, e0, e1));
-                        if (ms < best_t) {
-                            best_t = ms;
-                            best = b->kc;
-                        }
+                        KernelConfig kc = saved;
+                        kc.stack = st;
+                        kc.blocks_per_cu = bpc;
+                        kc.fetch_threshold = th;
+                        sweep.emplace_back(time_config(kc, 1), kc);
                     }
+                }
+            }
+            std::stable_sort(sweep.begin(), sweep.end(),
+                             [](const std::pair<float, KernelConfig>& x, const std::pair<float, KernelConfig>& y) {
+                                 return x.first < y.first;
+                             });
+            const size_t finalists = std::min<size_t>(sweep.size(), 6);
+            for (size_t i = 0; i < finalists; ++i) {
+                const float ms = time_config(sweep[i].second, 5);
+                if (ms < best_t) {
+                    best_t = ms;
+                    best = sweep[i].second;
                 }
             }
         } catch (...) {

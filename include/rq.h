@@ -233,8 +233,10 @@ rq_status rq_batch_set(rq_batch* batch, const rq_batch_item* items, uint32_t cou
 rq_status rq_batch_launch(rq_batch* batch, uint32_t use_graph, void* stream);
 rq_status rq_batch_get_config(const rq_batch* batch, rq_kernel_config* cfg);
 rq_status rq_batch_set_config(rq_batch* batch, const rq_kernel_config* cfg);
-/* Time every stack x blocks-per-CU x fetch-threshold candidate on the
- * batch's current items (outputs are overwritten) and keep the fastest. */
+/* Time every stack x blocks-per-CU x fetch-threshold candidate once on the
+ * batch's current items (outputs are overwritten), time the six fastest
+ * again over five launches each, and keep the fastest of those; best_ms is
+ * its mean launch time. */
 rq_status rq_batch_autotune(rq_batch* batch, void* stream, rq_kernel_config* chosen, double* best_ms);
 void rq_batch_destroy(rq_batch* batch);
 
