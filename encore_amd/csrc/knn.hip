@@ -582,6 +582,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void knn_cell_kernel(KnnLaunch L) 
     __shared__ uint32_t s_hist_w[kCellWaves][256];
     __shared__ uint32_t s_first_w[kCellWaves][64], s_pre_w[kCellWaves][64];
     __shared__ float s_near_w[kCellWaves][64];
+    __shared__ uint32_t s_mark_w[kCellWaves][64];
     const uint32_t wv = threadIdx.x / kKnnWave;
     uint32_t* s_key = s_key_w[wv];
     uint32_t* s_slot = s_slot_w[wv];
@@ -589,6 +590,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void knn_cell_kernel(KnnLaunch L) 
     uint32_t* s_first = s_first_w[wv];
     uint32_t* s_pre = s_pre_w[wv];
     float* s_near = s_near_w[wv];
+    uint32_t* s_mark = s_mark_w[wv];
     const PhotonGridView& G = L.grid;
     const uint32_t lane = __lane_id();
     const uint32_t k = L.k;
@@ -676,25 +678,51 @@ __global__ __launch_bounds__(64 * kCellWaves) void knn_cell_kernel(KnnLaunch L) 
             // Chunk e0 covers elements e0 .. e0 + 63 of the concatenated
             // ranges; its photons (position, class, slot) are loaded one
             // chunk ahead, so their latency overlaps the previous chunk.
-            uint32_t ci = 0, c0 = 0;  // cells of this lane's next element and of the chunk's first
+            //
+            // Every kept cell holds a photon, so the starts increase with the
+            // cell and an element's cell is the last one starting at or
+            // before it.  When few cells start inside a chunk, each lane
+            // steps its cell forward; when many do (sparse cells), the chunk
+            // marks their starts in LDS and takes a running maximum over the
+            // lanes, carried from the previous chunk's last element.
+            const uint32_t my_start = lane < ncell ? s_pre[lane] : 0xFFFFFFFFu;
+            uint32_t ci = 0, carry = 0;  // this lane's cell; the cell of the chunk's previous element
             float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             uint32_t sl = 0;
-            auto fetch = [&](uint32_t e0, float4& fp, uint32_t& fs) {
+            // loads chunk e0; returns the cell of its first element
+            auto fetch = [&](uint32_t e0, float4& fp, uint32_t& fs) -> uint32_t {
                 const uint32_t e = e0 + lane;
+                const bool starts_here = lane < ncell && my_start - e0 < 64u;
+                if (__popcll(__ballot(starts_here)) <= 3) {
+                    if (e < total)
+                        while (ci + 1 < ncell && s_pre[ci + 1] <= e) ++ci;
+                } else {
+                    s_mark[lane] = 0u;
+                    __builtin_amdgcn_wave_barrier();
+                    if (starts_here) s_mark[my_start - e0] = lane;
+                    __builtin_amdgcn_wave_barrier();
+                    uint32_t c = max(s_mark[lane], carry);
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const uint32_t t = __shfl_up(c, o, 64);
+                        if (lane >= (uint32_t)o) c = max(c, t);
+                    }
+                    ci = c;
+                }
+                carry = __shfl(ci, 63, 64);
                 if (e < total) {
-                    while (ci + 1 < ncell && s_pre[ci + 1] <= e) ++ci;
                     const uint32_t gi = s_first[ci] + (e - s_pre[ci]);
                     fp = G.pos[gi];
                     fs = G.slot[gi];
                 }
+                return __shfl(ci, 0, 64);
             };
-            if (total) fetch(0, p, sl);
+            uint32_t c0 = total ? fetch(0, p, sl) : 0u;  // the cell of the chunk's first element
             for (uint32_t e0 = 0; e0 < total; e0 += 64) {
-                while (c0 + 1 < ncell && s_pre[c0 + 1] <= e0) ++c0;
                 if (s_near[c0] >= r2) break;  // this cell and all later ones are too far
                 float4 pn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 uint32_t sn = 0;
-                if (e0 + 64 < total) fetch(e0 + 64, pn, sn);
+                uint32_t c0n = c0;
+                if (e0 + 64 < total) c0n = fetch(e0 + 64, pn, sn);
                 bool is_c = false;
                 float d2 = 0.0f;
                 if (e0 + lane < total) {
@@ -718,6 +746,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void knn_cell_kernel(KnnLaunch L) 
                 }
                 p = pn;
                 sl = sn;
+                c0 = c0n;
             }
         }
         wave_finish(L, q, cand, false, s_key, s_slot, s_hist);
