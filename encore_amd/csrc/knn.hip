@@ -274,9 +274,14 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
     return (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
 }
 
-// The k-th smallest key (1-based rank `need` on entry) among s_key[0, n):
-// returns it, and leaves in `need` how many entries equal to it belong to the
-// k smallest.
+// Radix select of the k smallest keys (k = the 1-based rank `need` on entry)
+// among s_key[0, n), 8 bits per pass from the top: returns a threshold t and
+// leaves in `need` how many entries equal to t belong to the k smallest, so
+// that they are the keys below t and the first `need` keys equal to t.  t is
+// the k-th smallest key, or, once every key of the k-th key's bucket belongs
+// to the k smallest, the end of that bucket with need = 0 (the remaining
+// passes would change nothing).  Either way no key of the k smallest exceeds
+// t, so t bounds the search radius from above.
 __device__ uint32_t select_kth(const uint32_t* s_key, uint32_t n, uint32_t& need, uint32_t* s_hist) {
     const uint32_t lane = __lane_id();
     uint32_t prefix = 0, mask = 0;
@@ -304,13 +309,14 @@ __device__ uint32_t select_kth(const uint32_t* s_key, uint32_t n, uint32_t& need
         const uint64_t hm = __ballot(here);
         if (hm == 0) break;  // need exceeds the keys left: cannot happen for need <= n
         const uint32_t src = (uint32_t)__ffsll((unsigned long long)hm) - 1;
-        uint32_t digit = 0, before = 0;
+        uint32_t digit = 0, before = 0, in_bucket = 0;
         if (here) {
             uint32_t run = excl;
             for (int j = 0; j < 4; ++j) {
                 if (run + c[j] >= need) {
                     digit = 4 * lane + j;
                     before = run;
+                    in_bucket = c[j];
                     break;
                 }
                 run += c[j];
@@ -318,10 +324,16 @@ __device__ uint32_t select_kth(const uint32_t* s_key, uint32_t n, uint32_t& need
         }
         digit = __shfl(digit, src, 64);
         before = __shfl(before, src, 64);
+        in_bucket = __shfl(in_bucket, src, 64);
         need -= before;
         prefix |= digit << shift;
         mask |= 255u << shift;
         __builtin_amdgcn_wave_barrier();
+        const uint64_t end = (uint64_t)prefix + (1ull << shift);
+        if (need == in_bucket && end <= 0xFFFFFFFFull) {
+            need = 0;
+            return (uint32_t)end;
+        }
     }
     return prefix;
 }
