@@ -655,7 +655,10 @@ __global__ __launch_bounds__(64 * kCellWaves) void knn_cell_kernel(KnnLaunch L) 
             v = v && near2 < (double)r2 && !far;
             uint2 range = make_uint2(0u, 0u);
             if (v) {
-                range = cell_lookup(G, cell_key(c[0], c[1], c[2]));
+                if (G.dense)
+                    range = G.dense[c[0] + (uint64_t)G.dims[0] * (c[1] + (uint64_t)G.dims[1] * c[2])];
+                else
+                    range = cell_lookup(G, cell_key(c[0], c[1], c[2]));
                 v = range.y > 0;
             }
             // Nearest first.  The bound is kept as a float rounded down with

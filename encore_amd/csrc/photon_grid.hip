@@ -15,11 +15,15 @@
 //   pg_starts  the first photon of each cell inserts the cell into the table;
 //   pg_ends    the last photon of each cell writes its count;
 //   pg_gather  positions and classes in grid order.
+// The table is a dense array over the photons' bounding box when that takes
+// at most kGridDenseMaxBytes (one load per cell, no hashing or probing),
+// otherwise an open-addressing hash table keyed by the cell's Morton code.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "rq_internal.h"
 
@@ -37,9 +41,28 @@ struct PgBuild {
     uint32_t* slot;
     uint4* cells;
     uint32_t cell_mask;
+    uint2* dense;  // null: the hash table
+    uint32_t dims[3];
     double origin[3];
     double inv_cell;
 };
+
+// Dense index of photon p's cell (within the box by construction; clamped
+// so that a write can never leave the table).
+__device__ __forceinline__ uint64_t pg_dense_index(const PgBuild& B, const PhotonHot& p) {
+    const float x[3] = {p.x, p.y, p.z};
+    uint64_t c[3];
+    for (int a = 0; a < 3; ++a) {
+        const int64_t v = grid_coord((double)x[a], B.origin[a], B.inv_cell);
+        c[a] = (uint64_t)min(max(v, (int64_t)0), (int64_t)B.dims[a] - 1);
+    }
+    return c[0] + (uint64_t)B.dims[0] * (c[1] + (uint64_t)B.dims[1] * c[2]);
+}
+
+__global__ void pg_clear_dense(uint2* dense, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dense[i] = make_uint2(0u, 0u);
+}
 
 constexpr unsigned long long kEmptyKey = ~0ull;
 
@@ -69,6 +92,10 @@ __global__ void pg_starts(PgBuild B) {
     if (i >= B.nvalid) return;
     const unsigned long long key = B.skeys[i];
     if (i > 0 && B.skeys[i - 1] == key) return;
+    if (B.dense) {
+        B.dense[pg_dense_index(B, B.hot[B.sids[i]])].x = i;
+        return;
+    }
     uint32_t h = cell_hash(key, B.cell_mask);
     // The table has at least twice as many entries as cells: a free entry
     // is always found.
@@ -88,6 +115,11 @@ __global__ void pg_ends(PgBuild B) {
     if (i >= B.nvalid) return;
     const unsigned long long key = B.skeys[i];
     if (i + 1 < B.nvalid && B.skeys[i + 1] == key) return;
+    if (B.dense) {
+        uint2* e = B.dense + pg_dense_index(B, B.hot[B.sids[i]]);
+        e->y = i + 1 - e->x;
+        return;
+    }
     uint32_t h = cell_hash(key, B.cell_mask);
     for (uint32_t probe = 0; probe <= B.cell_mask; ++probe) {
         const uint4 e = B.cells[h];
@@ -137,13 +169,17 @@ const PhotonGrid* photon_grid_for(rq_photons* p, float r2, hipStream_t s) {
     for (int a = 0; a < 3; ++a) far = std::max({far, std::fabs((double)p->bmin[a]), std::fabs((double)p->bmax[a])});
     if ((far + 2.0 * reach) * 0x1p-40 >= 0.25 * cell) return nullptr;
     const uint32_t n = p->n, nv = p->nvalid;
-    uint32_t cap = 64;
-    while (cap < 2 * (uint64_t)nv) cap <<= 1;
+    const uint64_t ncells = (uint64_t)dims[0] * dims[1] * dims[2];
+    const char* force_hash = std::getenv("RQ_KNN_GRID_HASH");  // A/B checks of the hash table
+    const bool dense = ncells <= kGridDenseMaxBytes / sizeof(uint2) && !(force_hash && force_hash[0] == '1');
+    uint32_t cap = dense ? 1 : 64;
+    while (!dense && cap < 2 * (uint64_t)nv) cap <<= 1;
     auto grid = std::make_unique<PhotonGrid>();
     grid->r2_bits = bits;
     grid->pos = DevBuf((size_t)nv * sizeof(float4));
     grid->slot = DevBuf((size_t)nv * sizeof(uint32_t));
     grid->cells = DevBuf((size_t)cap * sizeof(uint4));
+    if (dense) grid->dense = DevBuf(ncells * sizeof(uint2));
     DevBuf keys((size_t)n * 8), skeys((size_t)n * 8), ids((size_t)n * 4), sids((size_t)n * 4);
     dev::PgBuild B{};
     B.hot = p->hot.as<PhotonHot>();
@@ -157,10 +193,18 @@ const PhotonGrid* photon_grid_for(rq_photons* p, float r2, hipStream_t s) {
     B.slot = grid->slot.as<uint32_t>();
     B.cells = grid->cells.as<uint4>();
     B.cell_mask = cap - 1;
-    for (int a = 0; a < 3; ++a) B.origin[a] = (double)p->bmin[a];
+    B.dense = dense ? grid->dense.as<uint2>() : nullptr;
+    for (int a = 0; a < 3; ++a) {
+        B.dims[a] = dims[a];
+        B.origin[a] = (double)p->bmin[a];
+    }
     B.inv_cell = inv_cell;
     hipLaunchKernelGGL(dev::pg_clear, dim3(blocks_for(cap)), dim3(256), 0, s, B.cells, cap);
     RQ_HIP(hipGetLastError());
+    if (dense) {
+        hipLaunchKernelGGL(dev::pg_clear_dense, dim3(blocks_for(ncells)), dim3(256), 0, s, B.dense, ncells);
+        RQ_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(dev::pg_keys, dim3(blocks_for(n)), dim3(256), 0, s, B);
     RQ_HIP(hipGetLastError());
     // Stable sort: photons of one cell keep their slot order.
@@ -184,6 +228,7 @@ const PhotonGrid* photon_grid_for(rq_photons* p, float r2, hipStream_t s) {
     v.slot = grid->slot.as<uint32_t>();
     v.cells = grid->cells.as<uint4>();
     v.cell_mask = cap - 1;
+    v.dense = B.dense;
     for (int a = 0; a < 3; ++a) {
         v.dims[a] = dims[a];
         v.origin[a] = (double)p->bmin[a];

@@ -202,6 +202,9 @@ struct PhotonGridView {
     const uint32_t* slot;  // grid order -> balanced-tree slot (power, perm)
     const uint4* cells;    // (key low, key high, first, count); key ~0 = empty
     uint32_t cell_mask;    // table entries - 1
+    // When the grid's box is small enough: (first, count) of every cell,
+    // x fastest (count 0 = empty), read instead of the hash table.
+    const uint2* dense;
     uint32_t dims[3];      // cells per axis, each below 2^21
     double origin[3];      // the photons' lower bounds
     double cell, inv_cell;
@@ -467,9 +470,11 @@ void launch_count(const TraceLaunch& L, uint32_t accel, uint32_t predicate, Trac
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
 
 // The cell grid of a photon map for squared radius r2 (photon_grid.hip).
+// Largest dense cell table (bytes); larger grids use the hash table only.
+constexpr uint64_t kGridDenseMaxBytes = 256ull << 20;
 struct PhotonGrid {
     uint32_t r2_bits = 0;
-    DevBuf pos, slot, cells;
+    DevBuf pos, slot, cells, dense;
     PhotonGridView view{};
 };
 // Makes (or keeps) the grid for r2 on stream s and returns it, or null when

@@ -67,8 +67,13 @@ def _surface_photons(rng, n, offset):
     return P + off, N, W, Q + off, QN
 
 
+@pytest.mark.parametrize("table", ["dense", "hash"])
 @pytest.mark.parametrize("offset", [0.0, -3.5, 1000.0])
-def test_cells_find_every_photon_in_range(device, offset):
+def test_cells_find_every_photon_in_range(device, offset, table, monkeypatch):
+    """Both cell tables: the dense array (the default for a box of at most
+    256 MB of cells) and the hash table (RQ_KNN_GRID_HASH=1)."""
+    if table == "hash":
+        monkeypatch.setenv("RQ_KNN_GRID_HASH", "1")
     rng = np.random.default_rng(11)
     P, N, W, Q, QN = _surface_photons(rng, 40000, offset)
     # photons exactly on cell faces: copies of queries moved by r along an axis
@@ -82,7 +87,7 @@ def test_cells_find_every_photon_in_range(device, offset):
     k = 256
     rgb, cnt = _radiance(pm, Q, QN, k, r, False, False)
     want = _brute_count(P, N, Q, QN, r)
-    log("case: cells offset %g: mean count %.1f, max %d" % (offset, want.mean(), want.max()))
+    log("case: cells offset %g, %s table: mean count %.1f, max %d" % (offset, table, want.mean(), want.max()))
     assert want.max() < k and want.mean() > 10
     assert np.array_equal(cnt, want)
     heap_rgb, heap_cnt = _heap(_radiance, pm, Q, QN, k, r, True, False)
@@ -91,9 +96,12 @@ def test_cells_find_every_photon_in_range(device, offset):
     pm.destroy()
 
 
+@pytest.mark.parametrize("table", ["dense", "hash"])
 @pytest.mark.parametrize("k", [32, 100, 256])
 @pytest.mark.parametrize("caustic", [False, True])
-def test_cells_saturated_equal_heap_kernel_and_oracle(device, k, caustic):
+def test_cells_saturated_equal_heap_kernel_and_oracle(device, k, caustic, table, monkeypatch):
+    if table == "hash":
+        monkeypatch.setenv("RQ_KNN_GRID_HASH", "1")
     rng = np.random.default_rng(12)
     P, N, W, Q, QN = _surface_photons(rng, 200000, 0.0)
     pm = rq.PhotonMap(device, P, N, W)
@@ -103,8 +111,8 @@ def test_cells_saturated_equal_heap_kernel_and_oracle(device, k, caustic):
     assert np.array_equal(cnt, heap_cnt)
     assert not np.isnan(rgb).any()
     same = np.all(rgb.view(np.uint32) == heap_rgb.view(np.uint32), axis=1)
-    log("case: cells k=%d caustic=%s: %.3f full, %d of %d bit-equal" % (k, caustic, (cnt == k).mean(), same.sum(),
-                                                                         len(same)))
+    log("case: cells k=%d caustic=%s %s table: %.3f full, %d of %d bit-equal" % (k, caustic, table, (cnt == k).mean(),
+                                                                                  same.sum(), len(same)))
     assert same.mean() >= 0.99
     np.testing.assert_allclose(rgb, heap_rgb, rtol=1e-5, atol=1e-12)
     ref = oracle.PhotonMap(P, N, W).radiance(Q[:500], QN[:500], k, r, caustic=caustic)
