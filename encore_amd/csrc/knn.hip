@@ -585,7 +585,10 @@ constexpr int kCellCand = 2 * (int)kKnnMaxK + 64;
 
 // Workgroups of kCellWaves waves, one query per wave: a workgroup's waves
 // are spread over the CU's SIMDs, so a cap of whole workgroups per CU puts
-// the same number of waves on every SIMD.
+// the same number of waves on every SIMD.  The kernel compiles to 96 VGPRs,
+// the most that still fits 5 waves per SIMD: a change that needs more costs
+// a wave per SIMD (a variant at 98 VGPRs ran 8% slower), and asking for 5
+// waves with __launch_bounds__ makes the compiler spill to scratch instead.
 constexpr int kCellWaves = 4;
 
 __global__ __launch_bounds__(64 * kCellWaves) void knn_cell_kernel(KnnLaunch L) {
