@@ -258,3 +258,37 @@ def test_axis_parallel_rays_keep_culling(device, width):
     got = _trace(sc, rays)
     c = _check_vs_oracle(xyz, idx, rays, got, kd.intersect(rays))
     assert c["bad"] == 0, c["bad_idx"][:10]
+
+
+def test_counts_beyond_2_30_are_refused(device):
+    """More than 2^30 rays or queries in one call return RQ_ERR_CAPACITY
+    before anything is read or launched (the buffers here hold one ray and
+    one query); the scene and map stay usable."""
+    xyz, idx = scenes.cornell()
+    sc = rq.Scene(device, xyz, idx, accel="bvh")
+    rays = torch.zeros((1, 8), device="cuda")
+    rays[0, 4] = 1.0
+    rays[0, 7] = float("inf")
+    hits = torch.zeros((1, 4), device="cuda")
+    occ = torch.zeros((1,), dtype=torch.int32, device="cuda")
+    big = (1 << 30) + 1
+    for call in (lambda: sc.intersect(rays, hits, n=big), lambda: sc.occluded(rays, occ, n=big)):
+        with pytest.raises(rq.RQError) as e:
+            call()
+        assert e.value.status == 5 and "2^30" in str(e.value)
+    pos = np.zeros((4, 3), np.float32)
+    nrm = np.tile(np.float32([0, 1, 0]), (4, 1))
+    pm = rq.PhotonMap(device, pos, nrm, np.ones_like(pos))
+    q = torch.zeros((1, 3), device="cuda")
+    qn = torch.tensor([[0.0, 1.0, 0.0]], device="cuda")
+    out_i = torch.zeros((1, 2), dtype=torch.int32, device="cuda")
+    out_d = torch.zeros((1, 2), device="cuda")
+    st = rq.load().rq_knn(pm.h, q.data_ptr(), qn.data_ptr(), big, 2, 1.0, out_i.data_ptr(), out_d.data_ptr(), None,
+                          None)
+    assert st == 5
+    log("case: counts beyond 2^30 refused")
+    sc.intersect(rays, hits)
+    torch.cuda.synchronize()
+    assert float(hits[0, 0]) > 0.0  # a hit distance, or tmax on a miss
+    pm.destroy()
+    sc.destroy()
