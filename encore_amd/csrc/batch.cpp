@@ -233,7 +233,7 @@ rq_status rq_batch_autotune(rq_batch* b, void* stream, rq_kernel_config* chosen,
         KernelConfig best = saved;
         float best_t = 1e30f;
         const int stacks[] = {8, 16, 32};
-        const int thresholds[] = {32, 48, 56, 60};
+        const int thresholds[] = {32, 44, 48, 56, 60};
         // Mean time of `reps` back-to-back launches with configuration kc.
         auto time_config = [&](const KernelConfig& kc, int reps) {
             b->kc = kc;
