@@ -4,14 +4,18 @@
 Workload: 8,388,608 cosine-weighted bounce rays from the primary hits of a
 1920x1080 view of a displaced icosphere (level 8, 1,310,720 triangles) above
 a 2-triangle ground plane, split into 64 lognormal batches (median 65,536,
-clipped to 4k..1M).  A step traces every batch once through the BVH2 with
-ENCORE's predicate; rays and hit buffers are resident in HBM before timing.
+clipped to 4k..1M).  A step traces every batch once, in one persistent
+hipGraph launch, through the SAH BVH4 with ENCORE's predicate; rays and hit
+buffers are resident in HBM before timing.
 
-Prints one JSON line: Mrays/s, the roofline of the traversal kernel
-(algorithmic bytes per ray = 32 + 16 + 64 N_node + 48 N_tri, counted by an
-instrumented copy of the kernel), the CPU baseline (ENCORE's kd-tree
-algorithm restated in oracle/, all host threads, on a strided ray sample)
-and hit-ID agreement of the GPU with that baseline on the same sample.
+Prints one JSON line: Mrays/s; the roofline of the traversal kernel
+(algorithmic bytes per ray = 32 + 16 + 128 N_node + 48 N_tri, counted by an
+instrumented copy of the kernel); the CPU baseline (ENCORE's kd-tree
+algorithm restated in oracle/, all host threads, on a strided ray sample);
+parity of the timed launch's own hits with that baseline on the sample
+(every difference adjudicated) and bitwise against one launch per batch on
+every ray; the FP64 error of a ~1% sample; the exact path (ENCORE's kd-tree
+on the GPU, bit for bit) in the same persistent launch.
 
     python bench.py [--steps K] [--warmup W] [--mode plain|batched] [--no-cpu]
 """
@@ -63,6 +67,9 @@ def parse():
 
 
 def cpu_threads(req):
+    """Threads of the CPU baseline: all CPUs in this process's affinity, up
+    to the 16 host cores a one-GPU share of the MI355X box provides (the
+    line also states the affinity count)."""
     n = len(os.sched_getaffinity(0))
     if req > 0:
         return req
@@ -108,18 +115,22 @@ def main():
                                                                  time.time() - t))
     N = len(rays)
     d_rays = torch.from_numpy(rays).cuda()
-    d_hits = torch.empty((N, 4), dtype=torch.float32, device="cuda")
+    # The timed launch writes d_hits; the plain per-batch leg has its own
+    # buffer, so parity below reads what the timed launch produced.
+    d_hits = torch.full((N, 4), -7.0, dtype=torch.float32, device="cuda")
+    d_hits_plain = torch.full((N, 4), -9.0, dtype=torch.float32, device="cuda")
     offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     stream = torch.cuda.current_stream()
     nb = len(sizes)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * nb)]
 
-    def plain_step(timed=False):
+    def plain_step(timed=False, out=None):
+        out = d_hits if out is None else out
         for b in range(nb):
             o, n = int(offs[b]), int(sizes[b])
             if timed:
                 ev[2 * b].record(stream)
-            scene.intersect(d_rays[o:o + n], d_hits[o:o + n], stream=stream)
+            scene.intersect(d_rays[o:o + n], out[o:o + n], stream=stream)
             if timed:
                 ev[2 * b + 1].record(stream)
         return nb
@@ -179,15 +190,23 @@ def main():
     launches_per_step = len(launch_ms) // args.steps
     log("timed (%s): %.2f ms/step, %.1f Mrays/s; launch time %.2f ms/step over %d launches" % (
         args.mode, ms_per_step, value, kernel_ms_total, launches_per_step))
-    plain_value = None
-    if args.mode == "batched":
-        psecs, _ = run(plain_step, max(1, min(args.steps, 3)), 1)
-        plain_value = N * max(1, min(args.steps, 3)) / psecs / 1e6
-        log("plain per-batch launches: %.1f Mrays/s" % plain_value)
     hits = d_hits.cpu().numpy()
+    plain_value = None
+    batched_equals_plain = None
+    if args.mode == "batched":
+        pstep = lambda timed=False: plain_step(timed, out=d_hits_plain)
+        psecs, _ = run(pstep, max(1, min(args.steps, 3)), 1)
+        plain_value = N * max(1, min(args.steps, 3)) / psecs / 1e6
+        # Bitwise, every ray: the timed persistent launch against one launch
+        # per batch of the same kernel family.
+        batched_equals_plain = bool(torch.equal(d_hits.view(torch.int32), d_hits_plain.view(torch.int32)))
+        log("plain per-batch launches: %.1f Mrays/s; batched == plain on all %d rays: %s" % (
+            plain_value, N, batched_equals_plain))
+    del d_hits_plain
 
     exact = None
     kd_hits = None
+    kd_batched_hits = None
     if not args.no_exact:
         # The bit-exact drop-in: ENCORE's own kd-tree traversed on the GPU.
         t = time.time()
@@ -216,14 +235,39 @@ def main():
         torch.cuda.synchronize()
         k1secs = time.perf_counter() - kt1
         assert np.array_equal(d_kh.cpu().numpy().view(np.uint32), kd_hits.view(np.uint32))
+        # The 64 batches in one persistent launch (replayed hipGraph,
+        # autotuned), as the BVH headline runs them.
+        d_kb = torch.full((N, 4), -5.0, dtype=torch.float32, device="cuda")
+        kb = rq.Batch(dev)
+        kb.set([(kscene, d_rays[int(offs[b]):int(offs[b + 1])], d_kb[int(offs[b]):int(offs[b + 1])])
+                for b in range(nb)])
+        t = time.time()
+        ktuned = kb.autotune(stream=stream)
+        log("kd batch config in %.1fs: %s" % (time.time() - t, ktuned))
+        d_kb.fill_(-5.0)
+        kb.launch(graph=True, stream=stream)
+        torch.cuda.synchronize()
+        kt2 = time.perf_counter()
+        for _ in range(ksteps):
+            kb.launch(graph=True, stream=stream)
+        torch.cuda.synchronize()
+        kbsecs = time.perf_counter() - kt2
+        kd_batched_hits = d_kb.cpu().numpy()
+        kd_batched_equals_plain = bool(np.array_equal(kd_batched_hits.view(np.uint32), kd_hits.view(np.uint32)))
+        kb.destroy()
+        del d_kb
         kn = kscene.counts(d_rays) if not args.no_counts else None
-        exact = {"accel": "kd", "value": round(N * ksteps / ksecs / 1e6, 2), "unit": "Mrays/s",
-                 "ms_per_step": round(ksecs * 1000 / ksteps, 2), "mode": "one launch per batch",
+        exact = {"accel": "kd", "value": round(N * ksteps / kbsecs / 1e6, 2), "unit": "Mrays/s",
+                 "ms_per_step": round(kbsecs * 1000 / ksteps, 2),
+                 "mode": "one persistent hipGraph launch per step, autotuned", "autotuned": ktuned,
+                 "batched_equals_per_batch_launches": kd_batched_equals_plain,
+                 "per_batch_launches_value": round(N * ksteps / ksecs / 1e6, 2),
                  "one_launch_value": round(N * ksteps / k1secs / 1e6, 2)}
         if kn:
             exact["nodes_per_ray"] = round(kn[1] / kn[0], 2)
             exact["tris_per_ray"] = round(kn[2] / kn[0], 2)
-        log("kd exact path: %.1f Mrays/s per batch, %.1f in one launch" % (exact["value"], exact["one_launch_value"]))
+        log("kd exact path: %.1f Mrays/s batched (graph), %.1f per batch, %.1f in one launch; batched == per-batch: %s"
+            % (exact["value"], exact["per_batch_launches_value"], exact["one_launch_value"], kd_batched_equals_plain))
         del d_kh
         kscene.destroy()
 
@@ -255,6 +299,7 @@ def main():
 
     cpu = None
     parity = None
+    fp64 = None
     if not args.no_cpu:
         import oracle
 
@@ -274,56 +319,34 @@ def main():
         allc = len(sample) / (time.perf_counter() - t) / 1e6
         log("cpu kd: %.4f Mrays/s on 1 thread, %.4f Mrays/s on %d threads" % (one, allc, thr))
         cpu = {"value": round(allc, 5), "unit": "Mrays/s", "cores": thr, "kind": "port",
+               "host_cpus_in_affinity": len(os.sched_getaffinity(0)),
                "sample": "%d of the %d bounce rays (every %dth), ENCORE kd-tree (Kdtree.cpp) restated, %d threads; "
                          "1 thread: %.5f Mrays/s on %d rays" % (len(sample), N, stride, thr, one, len(small)),
                "one_core": round(one, 5)}
+        # Parity of the timed launch's own output on the sample, every
+        # difference adjudicated (brute force over all triangles, then FP64).
         g = hits[::stride][: args.cpu_sample]
-        gp = g[:, 3].view(np.uint32)
-        rp = ref[:, 3].view(np.uint32)
-        same = gp == rp
-        bits = same & np.all(g.view(np.uint32) == ref.view(np.uint32), axis=1)
-        diff = np.nonzero(~same)[0]
-        tie = 0
-        for i in diff:
-            if gp[i] != 0xFFFFFFFF and rp[i] != 0xFFFFFFFF and abs(g[i, 0] - ref[i, 0]) <= 1e-6 * abs(ref[i, 0]):
-                tie += 1
-        # Adjudicate the remaining differences by brute force over all
-        # triangles (ENCORE's predicate): "kd_miss" = the kd-tree misses the
-        # brute-force closest hit and the GPU finds it.
-        other = [i for i in diff if not (gp[i] != 0xFFFFFFFF and rp[i] != 0xFFFFFFFF and
-                                          abs(g[i, 0] - ref[i, 0]) <= 1e-6 * abs(ref[i, 0]))]
-        kd_miss, unexplained = 0, []
-        if other:
-            other = np.array(other[:256])
-            br = oracle.brute_closest(xyz, idx, sample[other], nthreads=thr)
-            bp = br[:, 3].view(np.uint32)
-            for j, i in enumerate(other):
-                if gp[i] == bp[j] or (gp[i] != 0xFFFFFFFF and bp[j] != 0xFFFFFFFF and
-                                      abs(g[i, 0] - br[j, 0]) <= 1e-6 * abs(br[j, 0])):
-                    kd_miss += 1
-                else:
-                    unexplained.append(int(i))
-            if args.dump:
-                np.savez(args.dump, rays=sample[other], gpu=g[other], kd=ref[other], brute=br)
-        # Differences left after brute force: the reference predicate's Cross()
-        # flush reports a hit the exact geometry does not have.  FP64 decides.
-        phantom = 0
-        still = []
-        if unexplained:
-            ua = np.array(unexplained)
-            rk = np.ascontiguousarray(ref[ua, 3]).view(np.uint32)
-            f = oracle.fp64_eval(xyz, idx, sample[ua], rk)
-            off = ~((f[:, 1] >= 0) & (f[:, 2] >= 0) & (f[:, 1] + f[:, 2] <= 1) & (f[:, 0] > 0))
-            phantom = int(off.sum())
-            still = ua[~off].tolist()
-        parity = {"sample": len(sample), "bvh": {"prim_equal": int(same.sum()), "bit_equal": int(bits.sum()),
-                                                 "ties_1e-6t": tie, "kd_miss_gpu_matches_brute": kd_miss,
-                                                 "reference_flush_phantom_fp64_off_triangle": phantom,
-                                                 "unexplained": len(still)}}
+        parity = {"sample": len(sample), "checked": "batched (timed) launch output" if args.mode == "batched"
+                  else "plain launch output"}
+        parity["bvh"] = _classify(g, ref, xyz, idx, sample, thr, dump=args.dump)
+        parity["bvh"]["prim_equal"] = int((g[:, 3].view(np.uint32) == ref[:, 3].view(np.uint32)).sum())
+        if batched_equals_plain is not None:
+            parity["batched_equals_plain"] = batched_equals_plain
         if kd_hits is not None:
             kg = kd_hits[::stride][: args.cpu_sample]
             parity["kd"] = {"bit_equal": int(np.all(kg.view(np.uint32) == ref.view(np.uint32), axis=1).sum())}
+        if kd_batched_hits is not None:
+            kg = kd_batched_hits[::stride][: args.cpu_sample]
+            parity["kd_batched"] = {"bit_equal": int(np.all(kg.view(np.uint32) == ref.view(np.uint32), axis=1).sum())}
         log("parity on sample:", parity)
+        # FP64 brute force (box-culled, identical results) on a ~1% sample:
+        # every 6th ray of the parity sample.
+        sub = slice(None, None, 6)
+        t = time.time()
+        fp64 = {"sample": "every %dth ray (%d of %d)" % (6 * stride, len(sample[sub]), N),
+                "gpu": _fp64_report(xyz, idx, np.ascontiguousarray(sample[sub]), g[sub], thr),
+                "reference_cpu": _fp64_report(xyz, idx, np.ascontiguousarray(sample[sub]), ref[sub], thr)}
+        log("fp64 1%% sample (%.1fs): %s" % (time.time() - t, fp64))
 
     out = {
         "metric": "Mrays/s incoherent @1M tris",
@@ -349,6 +372,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "parity": parity,
+        "fp64_1pct": fp64,
     }
     if plain_value is not None:
         out["plain_per_batch_value"] = round(plain_value, 2)
@@ -424,11 +448,11 @@ def _predicate_hit(xyz, idx, rays, g):
     return res
 
 
-def _adjudicate(xyz, idx, rays, g, ref, thr, limit=600, dump=""):
-    """Closest-hit differences that are not ties, decided by brute force over
-    every triangle (ENCORE's predicate) and FP64: counts of kd misses the GPU
-    finds, reference flush phantoms (the reference's triangle is not crossed
-    in FP64), and unexplained rows."""
+def _adjudicate(xyz, idx, rays, g, ref, thr, dump=""):
+    """Closest-hit differences that are not ties, every one decided by brute
+    force over every triangle (ENCORE's predicate) and FP64: counts of kd
+    misses the GPU finds, reference flush phantoms (the reference's triangle
+    is not crossed in FP64), and unexplained rows."""
     import numpy as np
 
     import oracle
@@ -437,8 +461,7 @@ def _adjudicate(xyz, idx, rays, g, ref, thr, limit=600, dump=""):
            "reference_phantom_outside_its_triangle_box": 0, "unexplained": 0, "adjudicated": 0}
     if len(rays) == 0:
         return out
-    sel = np.arange(min(len(rays), limit))
-    rays, g, ref = np.ascontiguousarray(rays[sel]), g[sel], ref[sel]
+    rays = np.ascontiguousarray(rays)
     gp = np.ascontiguousarray(g[:, 3]).view(np.uint32)
     rp = np.ascontiguousarray(ref[:, 3]).view(np.uint32)
     br = oracle.brute_closest(xyz, idx, rays, nthreads=thr)
@@ -478,19 +501,57 @@ def _adjudicate(xyz, idx, rays, g, ref, thr, limit=600, dump=""):
 
 
 def _classify(g, ref, xyz=None, idx=None, rays=None, thr=1, dump=""):
+    """Every ray of the sample in exactly one class: bit-equal, same prim with
+    other bits (never expected: the predicate is ENCORE's), a tie (another
+    genuine prim within 1e-6 t), or -- adjudicated ray by ray when the scene
+    is given -- a kd miss, a reference flush phantom, a phantom outside its
+    triangle's box, or unexplained.  The classes are checked to sum to n."""
     import numpy as np
 
     gp = np.ascontiguousarray(g[:, 3]).view(np.uint32)
     rp = np.ascontiguousarray(ref[:, 3]).view(np.uint32)
     bits = np.all(g.view(np.uint32) == ref.view(np.uint32), axis=1)
     diff = np.nonzero(~bits)[0]
-    is_tie = np.array([gp[i] != rp[i] and gp[i] != 0xFFFFFFFF and rp[i] != 0xFFFFFFFF and
-                       abs(g[i, 0] - ref[i, 0]) <= 1e-6 * abs(ref[i, 0]) for i in diff], bool)
-    out = {"n": int(len(g)), "bit_equal": int(bits.sum()), "ties_1e-6t": int(is_tie.sum()),
-           "other": int(len(diff) - is_tie.sum())}
-    if xyz is not None and out["other"]:
-        o = diff[~is_tie]
-        out.update(_adjudicate(xyz, idx, rays[o], g[o], ref[o], thr, dump=dump))
+    same_prim = gp[diff] == rp[diff]
+    both = (gp[diff] != 0xFFFFFFFF) & (rp[diff] != 0xFFFFFFFF)
+    with np.errstate(invalid="ignore"):
+        is_tie = ~same_prim & both & (np.abs(g[diff, 0] - ref[diff, 0]) <= 1e-6 * np.abs(ref[diff, 0]))
+    out = {"n": int(len(g)), "bit_equal": int(bits.sum()), "same_prim_other_bits": int(same_prim.sum()),
+           "ties_1e-6t": int(is_tie.sum())}
+    rest = diff[~same_prim & ~is_tie]
+    if xyz is not None:
+        out.update(_adjudicate(xyz, idx, rays[rest], g[rest], ref[rest], thr, dump=dump))
+        parts = ("bit_equal", "same_prim_other_bits", "ties_1e-6t", "kd_miss_gpu_matches_brute",
+                 "reference_flush_phantom_fp64_off_triangle", "reference_phantom_outside_its_triangle_box",
+                 "unexplained")
+        assert sum(out[k] for k in parts) == out["n"], out
+    else:
+        out["other"] = int(len(rest))
+    return out
+
+
+def _fp64_report(xyz, idx, rays, hits, thr):
+    """Error of float32 hits against the FP64 closest hit over every triangle
+    (no flush): for the rays whose prim equals the FP64 prim, the largest
+    relative t error and the largest |du|, |dv|, also relative to
+    max(|u|, 1e-3) as the 1e-5-relative rule reads near zero."""
+    import numpy as np
+
+    import oracle
+
+    f, fp = oracle.fp64_closest(xyz, idx, rays, nthreads=thr, culled=True)
+    gp = np.ascontiguousarray(hits[:, 3]).view(np.uint32)
+    same = (fp != 0xFFFFFFFF) & (gp == fp)
+    out = {"rays": int(len(rays)), "fp64_hits": int((fp != 0xFFFFFFFF).sum()), "prim_equal_fp64": int(same.sum()),
+           "both_miss": int(((fp == 0xFFFFFFFF) & (gp == 0xFFFFFFFF)).sum())}
+    if same.any():
+        t, u, v = (hits[same, k].astype(np.float64) for k in range(3))
+        du, dv = np.abs(u - f[same, 1]), np.abs(v - f[same, 2])
+        rel_t = np.abs(t - f[same, 0]) / np.abs(f[same, 0])
+        out.update({"max_rel_t": float(rel_t.max()), "max_abs_du": float(du.max()), "max_abs_dv": float(dv.max()),
+                    "max_rel_uv_floor_1e-3": float(max((du / np.maximum(np.abs(f[same, 1]), 1e-3)).max(),
+                                                       (dv / np.maximum(np.abs(f[same, 2]), 1e-3)).max())),
+                    "p99_rel_t": float(np.percentile(rel_t, 99))})
     return out
 
 

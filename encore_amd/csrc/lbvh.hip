@@ -56,7 +56,7 @@ struct LbLaunch {
     uint32_t* order;     // emitted nodes, deepest level first
     BvhNode* nodes;
     uint32_t leaf_size;
-    uint32_t* stats;     // [0] leaf refs, [1] deepest emitted node
+    uint32_t* stats;     // [0] leaf refs, [1] deepest emitted node, [2] emitted nodes, [3] lb_order overflow
     TriRec* tris;
     uint32_t* prim_slot;
     uint32_t* rec_vidx;
@@ -180,9 +180,9 @@ __device__ __forceinline__ bool emitted(const LbLaunch& L, uint32_t i) {
 // atomics per level replace one per node.
 __global__ __launch_bounds__(256) void lb_emit(LbLaunch L) {
     __shared__ uint32_t s_cnt[kLbMaxLevels];
-    __shared__ uint32_t s_leaves, s_deep;
+    __shared__ uint32_t s_leaves, s_deep, s_emitted;
     for (uint32_t d = threadIdx.x; d < kLbMaxLevels; d += blockDim.x) s_cnt[d] = 0;
-    if (threadIdx.x == 0) s_leaves = s_deep = 0;
+    if (threadIdx.x == 0) s_leaves = s_deep = s_emitted = 0;
     __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (emitted(L, i)) {
@@ -208,6 +208,7 @@ __global__ __launch_bounds__(256) void lb_emit(LbLaunch L) {
         for (uint32_t p = L.parent_in[i]; p != 0xFFFFFFFFu; p = L.parent_in[p]) ++depth;
         L.depth[i] = depth;
         if (leaves) atomicAdd(&s_leaves, leaves);
+        atomicAdd(&s_emitted, 1u);
         atomicMax(&s_deep, depth);
         if (depth < kLbMaxLevels) atomicAdd(&s_cnt[depth], 1u);
     }
@@ -216,13 +217,17 @@ __global__ __launch_bounds__(256) void lb_emit(LbLaunch L) {
         if (s_cnt[d]) atomicAdd(&L.level_cnt[d], s_cnt[d]);
     if (threadIdx.x == 0) {
         if (s_leaves) atomicAdd(&L.stats[0], s_leaves);
+        if (s_emitted) atomicAdd(&L.stats[2], s_emitted);
         atomicMax(&L.stats[1], s_deep);
     }
 }
 
 // level_cnt[kLbMaxLevels + d] holds, on entry, the first slot of depth d in
 // the deepest-first order and serves as that level's fill cursor: each block
-// reserves one range per depth it holds and places its nodes inside it.
+// reserves one range per depth it holds and places its nodes inside it.  The
+// host has checked the level counts against the emitted nodes (at most
+// n - 1), so every slot lies in order[0, n - 1); a slot beyond it is not
+// written but flagged in stats[3], which the host turns into an error.
 __global__ __launch_bounds__(256) void lb_order(LbLaunch L) {
     __shared__ uint32_t s_cnt[kLbMaxLevels], s_base[kLbMaxLevels];
     for (uint32_t d = threadIdx.x; d < kLbMaxLevels; d += blockDim.x) s_cnt[d] = 0;
@@ -238,7 +243,13 @@ __global__ __launch_bounds__(256) void lb_order(LbLaunch L) {
     for (uint32_t d = threadIdx.x; d < kLbMaxLevels; d += blockDim.x)
         if (s_cnt[d]) s_base[d] = atomicAdd(&L.level_cnt[kLbMaxLevels + d], s_cnt[d]);
     __syncthreads();
-    if (mine) L.order[s_base[depth] + rank] = i;
+    if (mine) {
+        const uint32_t at = s_base[depth] + rank;
+        if (at < L.n - 1)
+            L.order[at] = i;
+        else
+            atomicOr(&L.stats[3], 1u);
+    }
 }
 
 __global__ void lb_pack(LbLaunch L) {
@@ -346,7 +357,7 @@ void build_bvh_device(rq_scene* sc, const float* xyz, uint32_t nverts, const uin
     // Slots of internal nodes that end up inside a leaf are never written by
     // lb_emit: keep them zero rather than stale memory.
     launch_zero_u32(nodes.as<uint32_t>(), (uint32_t)(ni * sizeof(BvhNode) / 4), s);
-    launch_zero_u32(L.stats, 2, s);
+    launch_zero_u32(L.stats, 4, s);
     hipLaunchKernelGGL(dev::lb_boxes, dim3(grid_of(n)), dim3(256), 0, s, L);
     RQ_HIP(hipGetLastError());
     hipLaunchKernelGGL(dev::lb_morton, dim3(grid_of(n)), dim3(256), 0, s, L);
@@ -360,11 +371,15 @@ void build_bvh_device(rq_scene* sc, const float* xyz, uint32_t nverts, const uin
     L.ids = sids.as<uint32_t>();
     hipLaunchKernelGGL(dev::lb_pack, dim3(grid_of(n)), dim3(256), 0, s, L);
     RQ_HIP(hipGetLastError());
-    uint32_t stat[2] = {0, 0};
+    uint32_t stat[4] = {0, 0, 0, 0};
     std::vector<uint32_t> levels;  // offsets into order, deepest level first
+    // Host sources of asynchronous copies stay alive until the stream
+    // synchronizes at the end of the build.
+    BvhNode g{};
+    const uint32_t zero = 0;
+    std::vector<uint32_t> start(dev::kLbMaxLevels, 0);
     if (n == 1) {
         // One triangle: a root whose two slots name the same one-triangle leaf.
-        BvhNode g{};
         float b[6];
         RQ_HIP(hipMemcpyAsync(b, L.tri_box, 24, hipMemcpyDeviceToHost, s));
         RQ_HIP(hipStreamSynchronize(s));
@@ -378,7 +393,6 @@ void build_bvh_device(rq_scene* sc, const float* xyz, uint32_t nverts, const uin
             g.ref[side] = make_leaf_ref(0, 1);
         }
         RQ_HIP(hipMemcpyAsync(nodes.ptr, &g, sizeof(g), hipMemcpyHostToDevice, s));
-        const uint32_t zero = 0;
         RQ_HIP(hipMemcpyAsync(order.ptr, &zero, 4, hipMemcpyHostToDevice, s));
         stat[0] = 1;
         levels = {0, 1};
@@ -388,13 +402,26 @@ void build_bvh_device(rq_scene* sc, const float* xyz, uint32_t nverts, const uin
         hipLaunchKernelGGL(dev::lb_emit, dim3(grid_of(n - 1)), dim3(256), 0, s, L);
         RQ_HIP(hipGetLastError());
         std::vector<uint32_t> cnt(dev::kLbMaxLevels);
-        RQ_HIP(hipMemcpyAsync(stat, L.stats, 8, hipMemcpyDeviceToHost, s));
+        RQ_HIP(hipMemcpyAsync(stat, L.stats, 16, hipMemcpyDeviceToHost, s));
         RQ_HIP(hipMemcpyAsync(cnt.data(), L.level_cnt, cnt.size() * 4, hipMemcpyDeviceToHost, s));
         RQ_HIP(hipStreamSynchronize(s));
         if (stat[1] >= dev::kLbMaxLevels)
             throw Error(RQ_ERR_UNSUPPORTED, "linear BVH deeper than 256 levels (use the host builder)");
+        // Every count the host uses below comes from the device: check that
+        // they describe a binary tree of at most n - 1 inner nodes before they
+        // become offsets (levels, the slots lb_order writes).
+        uint64_t counted = 0;
+        for (uint32_t d = 0; d < dev::kLbMaxLevels; ++d) {
+            if (d > stat[1] && cnt[d] != 0)
+                throw Error(RQ_ERR_DEVICE, "linear BVH build: nodes counted below the deepest level");
+            counted += cnt[d];
+        }
+        if (stat[2] == 0 || stat[2] > n - 1 || counted != stat[2] || stat[0] != stat[2] + 1)
+            throw Error(RQ_ERR_DEVICE, "linear BVH build: inconsistent node counts (" + std::to_string(stat[2]) +
+                                           " inner nodes, " + std::to_string(counted) + " by level, " +
+                                           std::to_string(stat[0]) + " leaves, " + std::to_string(n) +
+                                           " triangles)");
         // Deepest level first: the slot where each depth starts.
-        std::vector<uint32_t> start(dev::kLbMaxLevels, 0);
         uint32_t run = 0;
         levels.push_back(0);
         for (int64_t d = stat[1]; d >= 0; --d) {
@@ -421,8 +448,10 @@ void build_bvh_device(rq_scene* sc, const float* xyz, uint32_t nverts, const uin
             R.end = levels[l + 1];
             launch_refit_level(R, s);
         }
+        RQ_HIP(hipMemcpyAsync(&stat[3], L.stats + 3, 4, hipMemcpyDeviceToHost, s));
     }
     RQ_HIP(hipStreamSynchronize(s));
+    if (stat[3] != 0) throw Error(RQ_ERR_DEVICE, "linear BVH build: a node was placed outside the level order");
     sc->width = 2;
     sc->nodes = std::move(nodes);
     sc->tris = std::move(tris);

@@ -317,11 +317,14 @@ struct TraceLaunch {
     uint32_t fetch_threshold;    // refill idle lanes when fewer than this many are busy
     float root_pos[3], root_size[3];  // kd only
     uint32_t coop_lanes, coop_min;    // kd: cooperative leaf tests (see kd_trace)
+    const float* item_boxes;          // batched kd launch: root pos[3], pad, size[3], pad per item
 };
 
 enum class TraceKind { Closest = 0, Any = 1 };
 // kd cooperative leaf tests: at most this many busy lanes, and leaves with
 // more than this many triangles left.
+// kd traversal ring: entries per lane in LDS (older ones spill to HBM).
+constexpr int kKdStackEntries = 8;
 constexpr uint32_t kKdCoopLanesDefault = 16;
 constexpr uint32_t kKdCoopMinDefault = 16;
 
@@ -465,6 +468,8 @@ void launch_bvh_batched(const TraceLaunch& L, uint32_t predicate, TraceKind kind
                         const KernelConfig& kc, hipStream_t s);
 int occupancy_bvh_batched(uint32_t predicate, TraceKind kind, uint32_t width, int stack);
 void launch_kd(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, hipStream_t s);
+void launch_kd_batched(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, hipStream_t s);
+int occupancy_kd_batched(uint32_t predicate, TraceKind kind);
 void launch_count(const TraceLaunch& L, uint32_t accel, uint32_t predicate, TraceKind kind, uint32_t width, int grid,
                   hipStream_t s);
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);

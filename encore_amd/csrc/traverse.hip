@@ -752,7 +752,12 @@ __device__ __forceinline__ uint32_t fetch_capped(uint32_t cur, uint32_t* counter
 // rays of very different cost (a few to thousands of tests) keep the SIMD
 // full.  Leaf triangles are stored in leaf order (one record per reference),
 // so a leaf's records are contiguous and need no index load.
-template <class Pred, bool ANY, int S, bool COUNT = false>
+//
+// BATCHED: the rays of every item of a descriptor table (kd scenes, any
+// sizes) in one launch.  Lanes claim global ray indices; a lane's item gives
+// its rays, tree, triangle records, root box (L.item_boxes, 8 floats per
+// item) and output, so the scene pointers are per lane.
+template <class Pred, bool ANY, int S, bool COUNT = false, bool BATCHED = false>
 __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
     __shared__ uint32_t lds_stack[256 / kWave][S * 3][kWave];
     const int lane = lane_id();
@@ -763,12 +768,15 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
     stk.spill = L.spill + (size_t)gwave * L.spill_depth * kWave + lane;
     const KdNodeG* nodes = static_cast<const KdNodeG*>(L.nodes);
     const TriRec* tris = L.tris;
+    rq_hit* hits_out = L.hits;
+    uint32_t* occ_out = L.occluded;
     const float kInf = __int_as_float(0x7f800000);
     const uint32_t thresh = L.fetch_threshold;
     const uint32_t nrays = L.n_dev ? *L.n_dev : L.n;
 
     RayState r;
     uint32_t ray = kNone;
+    uint32_t slot = 0;  // output index: the ray, or its index within its item
     bool active = false;
     float rtmax = 0, best = 0, bu = 0, bv = 0, tmin = 0, tmax = 0;
     uint32_t bprim = kNone, cur = 0, lpos = 0, lend = 0;
@@ -786,7 +794,25 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
         ray = fetch_capped(ray, L.counter, nrays, cap, drained);
         if (ray != kNone && ray != kDone && !active) {
             float rtmin;
-            init_ray(r, L.rays, ray, rtmin, rtmax);
+            const rq_ray* rin = L.rays;
+            const float* bpos = L.root_pos;
+            const float* bsize = L.root_size;
+            slot = ray;
+            if (BATCHED) {
+                const uint32_t item = find_item(L.items, L.nitems, ray);
+                const BatchItemDev* it = L.items + item;
+                const uint4 pp = ldgu4(&it->nodes);
+                nodes = reinterpret_cast<const KdNodeG*>(((uint64_t)pp.y << 32) | pp.x);
+                tris = reinterpret_cast<const TriRec*>(((uint64_t)pp.w << 32) | pp.z);
+                const uint4 oo = ldgu4(&it->hits);
+                hits_out = reinterpret_cast<rq_hit*>(((uint64_t)oo.y << 32) | oo.x);
+                occ_out = reinterpret_cast<uint32_t*>(((uint64_t)oo.w << 32) | oo.z);
+                rin = it->rays;
+                slot = ray - it->begin;
+                bpos = L.item_boxes + 8 * (size_t)item;
+                bsize = bpos + 4;
+            }
+            init_ray(r, rin, slot, rtmin, rtmax);
             Pred::prepare(r);
             best = kInf;  // HitInfo.hitTime = INFINITY (Kdtree.cpp:685)
             bu = bv = 0.0f;
@@ -794,12 +820,12 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
             cur = 0;
             lpos = lend = 0;
             stk.reset();
-            active = root_clip(r, L.root_pos, L.root_size, tmin, tmax);
+            active = root_clip(r, bpos, bsize, tmin, tmax);
             if (!active) {  // the ray misses the scene box
                 if (ANY)
-                    L.occluded[ray] = 0u;
+                    stg1(occ_out + slot, 0u);
                 else
-                    *reinterpret_cast<float4*>(&L.hits[ray]) = make_float4(rtmax, 0.0f, 0.0f, __uint_as_float(kNone));
+                    stg4(hits_out + slot, make_float4(rtmax, 0.0f, 0.0f, __uint_as_float(kNone)));
                 ray = kNone;
             }
         }
@@ -854,6 +880,12 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
                 c.sx = __shfl(r.sx, src);
                 c.sy = __shfl(r.sy, src);
                 c.sz = __shfl(r.sz, src);
+                const TriRec* ctris = tris;
+                if (BATCHED) {
+                    const uint64_t tv = reinterpret_cast<uint64_t>(tris);
+                    const uint32_t tlo = __shfl((uint32_t)tv, src), thi = __shfl((uint32_t)(tv >> 32), src);
+                    ctris = reinterpret_cast<const TriRec*>(((uint64_t)thi << 32) | tlo);
+                }
                 float cbest = __shfl(best, src), cbu = __shfl(bu, src), cbv = __shfl(bv, src);
                 uint32_t cprim = __shfl(bprim, src), cpos = __shfl(lpos, src);
                 // every lane takes part in the shuffle (a source lane can lie
@@ -868,7 +900,7 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
                     float t = kInf, u = 0.0f, v = 0.0f;
                     uint32_t prim = kNone;
                     if (j < cnt) {
-                        const TriRec* tp = tris + cpos + j;
+                        const TriRec* tp = ctris + cpos + j;
                         const float4 ta = ldg4(tp->a), tb = ldg4(tp->b), tc = ldg4(tp->c);
                         hit = Pred::test(c, ta, tb, tc, 0.0f, t, u, v);
                         prim = __float_as_uint(ta.w);
@@ -999,10 +1031,10 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
             if (finish) {
                 const bool has = (bprim != kNone) && (best < rtmax);
                 if (ANY) {
-                    L.occluded[ray] = has ? 1u : 0u;
+                    stg1(occ_out + slot, has ? 1u : 0u);
                 } else {
-                    *reinterpret_cast<float4*>(&L.hits[ray]) = make_float4(
-                        has ? best : rtmax, has ? bu : 0.0f, has ? bv : 0.0f, __uint_as_float(has ? bprim : kNone));
+                    stg4(hits_out + slot, make_float4(has ? best : rtmax, has ? bu : 0.0f, has ? bv : 0.0f,
+                                                      __uint_as_float(has ? bprim : kNone)));
                 }
                 if (COUNT) {
                     atomicAdd(&L.counts[0], (unsigned long long)n_nodes);
@@ -1028,7 +1060,7 @@ __global__ void zero_u32(uint32_t* p, uint32_t n) {
 // Launchers
 // ---------------------------------------------------------------------------
 // kd ring: 8 entries x 3 dwords x 64 lanes x 4 waves = 24 KB of LDS per block.
-constexpr int kKdStack = 8;
+constexpr int kKdStack = kKdStackEntries;
 
 template <class Pred, bool ANY, bool COUNT, bool BATCHED, int WIDTH>
 static const void* bvh_fn_s(int stack) {
@@ -1064,9 +1096,12 @@ static const void* bvh_fn(uint32_t predicate, TraceKind kind, bool count, bool b
                : bvh_fn_v<dev::PredEncore, false>(count, batched, width, stack);
 }
 
-static const void* kd_fn(uint32_t predicate, TraceKind kind, bool count) {
+static const void* kd_fn(uint32_t predicate, TraceKind kind, bool count, bool batched = false) {
     const bool any = kind == TraceKind::Any;
-#define RQ_KD(P, A) (count ? (const void*)dev::kd_trace<P, A, kKdStack, true> : (const void*)dev::kd_trace<P, A, kKdStack>)
+#define RQ_KD(P, A)                                                                  \
+    (count     ? (const void*)dev::kd_trace<P, A, kKdStack, true>                    \
+     : batched ? (const void*)dev::kd_trace<P, A, kKdStack, false, true>             \
+               : (const void*)dev::kd_trace<P, A, kKdStack>)
     if (predicate == RQ_PRED_WATERTIGHT)
         return any ? RQ_KD(dev::PredWatertight, true) : RQ_KD(dev::PredWatertight, false);
     return any ? RQ_KD(dev::PredEncore, true) : RQ_KD(dev::PredEncore, false);
@@ -1101,6 +1136,14 @@ int occupancy_bvh_batched(uint32_t predicate, TraceKind kind, uint32_t width, in
 
 void launch_kd(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, hipStream_t s) {
     launch_trace(kd_fn(predicate, kind, false), L, grid, kKdBlock, s);
+}
+
+void launch_kd_batched(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, hipStream_t s) {
+    launch_trace(kd_fn(predicate, kind, false, true), L, grid, kKdBlock, s);
+}
+
+int occupancy_kd_batched(uint32_t predicate, TraceKind kind) {
+    return occupancy_of(kd_fn(predicate, kind, false, true), kKdBlock);
 }
 
 void launch_count(const TraceLaunch& L, uint32_t accel, uint32_t predicate, TraceKind kind, uint32_t width, int grid,

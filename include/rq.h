@@ -72,7 +72,7 @@ typedef struct rq_hit {
 #define RQ_NO_HIT 0xFFFFFFFFu
 
 typedef enum rq_accel_kind {
-    RQ_ACCEL_BVH = 0,  /* binned-SAH BVH2, 64-byte nodes */
+    RQ_ACCEL_BVH = 0,  /* binned-SAH BVH: BVH4, 128-byte nodes (default), or BVH2, 64-byte nodes */
     RQ_ACCEL_KD = 1    /* ENCORE's spatial-median kd-tree, same build rules */
 } rq_accel_kind;
 
@@ -202,15 +202,19 @@ rq_status rq_occluded(rq_scene* scene, const rq_ray* d_rays, uint32_t n,
                       uint32_t* d_occluded, void* stream);
 
 /* ---------------------------------------------------------------------
- * Batched launches.  Many ray batches of any sizes, on any resident BVH
- * scenes sharing one predicate, traced by ONE persistent launch: waves pull
- * global ray indices from one counter and find the owning batch in a device
- * descriptor table.  The launch can be captured once into a hipGraph and
- * replayed.  No counterpart in ENCORE, whose renderers trace one ray per
- * AccelerationStructure::intersect call.
+ * Batched launches.  Many ray batches of any sizes, on any resident scenes
+ * of one kind (all BVH scenes of one node width, or all kd scenes -- the
+ * exact path, ENCORE's own tree traversed bit for bit as Kdtree::intersect,
+ * Kdtree.cpp:682-784) sharing one predicate, traced by ONE persistent
+ * launch: waves pull global ray indices from one counter and find the owning
+ * batch in a device descriptor table.  The launch can be captured once into
+ * a hipGraph and replayed.  A kd scene rebuilt by rq_scene_refit moves its
+ * arrays: call rq_batch_set again before the next launch.  No counterpart in
+ * ENCORE, whose renderers trace one ray per AccelerationStructure::intersect
+ * call.
  * --------------------------------------------------------------------- */
 typedef struct rq_batch_item {
-    rq_scene* scene;       /* RQ_ACCEL_BVH scene */
+    rq_scene* scene;       /* RQ_ACCEL_BVH or RQ_ACCEL_KD scene */
     const rq_ray* d_rays;  /* n rays */
     void* d_out;           /* n rq_hit (closest) or n uint32_t (any hit) */
     uint32_t n;
@@ -218,8 +222,9 @@ typedef struct rq_batch_item {
 } rq_batch_item;
 
 typedef struct rq_kernel_config {
-    uint32_t stack;            /* LDS ring entries per lane: 8, 16 or 32 */
-    uint32_t blocks_per_cu;    /* 256-thread workgroups per CU (0 = occupancy limit) */
+    uint32_t stack;            /* LDS ring entries per lane: 8, 16 or 32 (kd batches: always 8) */
+    uint32_t blocks_per_cu;    /* workgroups per CU (0 = occupancy limit): one wave each for BVH
+                                  scenes, four for kd scenes */
     uint32_t fetch_threshold;  /* refill idle lanes when fewer than this many of 64 are busy (0..64) */
     uint32_t grid;             /* out: workgroups of the last launch */
 } rq_kernel_config;
@@ -236,7 +241,7 @@ rq_status rq_batch_set_config(rq_batch* batch, const rq_kernel_config* cfg);
 /* Time every stack x blocks-per-CU x fetch-threshold candidate once on the
  * batch's current items (outputs are overwritten), time the six fastest
  * again over five launches each, and keep the fastest of those; best_ms is
- * its mean launch time. */
+ * its mean launch time.  kd batches sweep blocks per CU and the threshold. */
 rq_status rq_batch_autotune(rq_batch* batch, void* stream, rq_kernel_config* chosen, double* best_ms);
 void rq_batch_destroy(rq_batch* batch);
 

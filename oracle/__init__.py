@@ -40,6 +40,7 @@ def lib():
             "orc_count_hits_below": (None, [_f32p, _u32, _u32p, _u32, _f32p, _u32, _f32p, ctypes.c_float, _u32p]),
             "orc_fp64_eval": (None, [_f32p, _u32p, _f32p, _u32p, _u32, _f64p]),
             "orc_fp64_closest": (None, [_f32p, _u32, _u32p, _u32, _f32p, _u32, _f64p, _u32p, _int]),
+            "orc_fp64_closest_culled": (None, [_f32p, _u32, _u32p, _u32, _f32p, _u32, _f64p, _u32p, _int]),
             "orc_kd_build": (_vp, [_f32p, _u32, _u32p, _u32, _f32p, _f32p]),
             "orc_kd_free": (None, [_vp]),
             "orc_kd_stats": (None, [_vp, _u32p]),
@@ -159,14 +160,17 @@ def fp64_eval(xyz, idx, rays, prim):
     return out
 
 
-def fp64_closest(xyz, idx, rays, nthreads=None):
+def fp64_closest(xyz, idx, rays, nthreads=None, culled=False):
+    """FP64 closest hit over every triangle (no flush): (t, alpha, beta) and
+    prim per ray.  culled=True skips triangles by conservative boxes and
+    returns exactly the brute-force result, fast enough for large scenes."""
     xyz = _c32(xyz).reshape(-1)
     idx = _cu32(idx).reshape(-1)
     rays = _c32(rays).reshape(-1, 8)
     out = np.zeros((rays.shape[0], 3), np.float64)
     prim = np.zeros(rays.shape[0], np.uint32)
-    lib().orc_fp64_closest(xyz, len(xyz) // 3, idx, len(idx) // 3, rays, rays.shape[0], out, prim,
-                           nthreads or threads())
+    fn = lib().orc_fp64_closest_culled if culled else lib().orc_fp64_closest
+    fn(xyz, len(xyz) // 3, idx, len(idx) // 3, rays, rays.shape[0], out, prim, nthreads or threads())
     return out, prim
 
 

@@ -691,8 +691,39 @@ void orc_fp64_eval(const float* xyz, const uint32_t* idx, const float* rays, con
     }
 }
 
+// FP64 Moller-Trumbore without the flush, with early outs: true and
+// (t, alpha, beta) when triangle t is crossed at t > 0.
+static inline bool fp64_tri(const float* xyz, const uint32_t* idx, uint32_t t, const double o[3], const double d[3],
+                     double& tt, double& a, double& bta) {
+    double v[3][3];
+    for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) v[j][k] = xyz[3 * (size_t)idx[3 * (size_t)t + j] + k];
+    double e1[3], e2[3], s[3], p[3], q[3];
+    for (int k = 0; k < 3; ++k) {
+        e1[k] = v[1][k] - v[0][k];
+        e2[k] = v[2][k] - v[0][k];
+        s[k] = o[k] - v[0][k];
+    }
+    p[0] = d[1] * e2[2] - d[2] * e2[1];
+    p[1] = d[2] * e2[0] - d[0] * e2[2];
+    p[2] = d[0] * e2[1] - d[1] * e2[0];
+    double det = e1[0] * p[0] + e1[1] * p[1] + e1[2] * p[2];
+    if (det == 0.0) return false;
+    double inv = 1.0 / det;
+    a = (s[0] * p[0] + s[1] * p[1] + s[2] * p[2]) * inv;
+    if (a < 0.0 || a > 1.0) return false;
+    q[0] = s[1] * e1[2] - s[2] * e1[1];
+    q[1] = s[2] * e1[0] - s[0] * e1[2];
+    q[2] = s[0] * e1[1] - s[1] * e1[0];
+    bta = (d[0] * q[0] + d[1] * q[1] + d[2] * q[2]) * inv;
+    if (bta < 0.0 || a + bta > 1.0) return false;
+    tt = (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) * inv;
+    return tt > 0.0;
+}
+
 // FP64 brute-force closest hit (no flush): the independent oracle for a ray
-// sample.  hits: (t, alpha, beta) double + prim.
+// sample.  hits: (t, alpha, beta) double + prim.  Among equal t the lowest
+// triangle index wins (list order with a strict <).
 void orc_fp64_closest(const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt, const float* rays,
                       uint32_t n, double* out3, uint32_t* prim_out, int nthreads) {
     (void)nv;
@@ -703,34 +734,145 @@ void orc_fp64_closest(const float* xyz, uint32_t nv, const uint32_t* idx, uint32
             double best = INFINITY, ba = 0, bb = 0;
             uint32_t bp = 0xFFFFFFFFu;
             for (uint32_t t = 0; t < nt; ++t) {
-                double v[3][3];
-                for (int j = 0; j < 3; ++j)
-                    for (int k = 0; k < 3; ++k) v[j][k] = xyz[3 * (size_t)idx[3 * (size_t)t + j] + k];
-                double e1[3], e2[3], s[3], p[3], q[3];
-                for (int k = 0; k < 3; ++k) {
-                    e1[k] = v[1][k] - v[0][k];
-                    e2[k] = v[2][k] - v[0][k];
-                    s[k] = o[k] - v[0][k];
-                }
-                p[0] = d[1] * e2[2] - d[2] * e2[1];
-                p[1] = d[2] * e2[0] - d[0] * e2[2];
-                p[2] = d[0] * e2[1] - d[1] * e2[0];
-                double det = e1[0] * p[0] + e1[1] * p[1] + e1[2] * p[2];
-                if (det == 0.0) continue;
-                double inv = 1.0 / det;
-                double a = (s[0] * p[0] + s[1] * p[1] + s[2] * p[2]) * inv;
-                if (a < 0.0 || a > 1.0) continue;
-                q[0] = s[1] * e1[2] - s[2] * e1[1];
-                q[1] = s[2] * e1[0] - s[0] * e1[2];
-                q[2] = s[0] * e1[1] - s[1] * e1[0];
-                double bta = (d[0] * q[0] + d[1] * q[1] + d[2] * q[2]) * inv;
-                if (bta < 0.0 || a + bta > 1.0) continue;
-                double tt = (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) * inv;
-                if (tt > 0.0 && tt < best) {
+                double tt, a, bta;
+                if (fp64_tri(xyz, idx, t, o, d, tt, a, bta) && tt < best) {
                     best = tt;
                     ba = a;
                     bb = bta;
                     bp = t;
+                }
+            }
+            out3[3 * i] = best;
+            out3[3 * i + 1] = ba;
+            out3[3 * i + 2] = bb;
+            prim_out[i] = bp;
+        }
+    });
+}
+
+// The same FP64 closest hit over every triangle, with the triangles culled by
+// a bounding-volume hierarchy so that 1% samples of million-triangle scenes
+// finish in seconds.  Boxes are the triangles' vertex bounds in double,
+// widened by 1e-6 of their magnitude and extent, so a ray that the FP64 test
+// accepts always enters every box around the triangle; a box is skipped only
+// when its entry distance exceeds the best t.  Ties keep the lowest triangle
+// index, so the output is exactly orc_fp64_closest's (tests check this).
+void orc_fp64_closest_culled(const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt, const float* rays,
+                             uint32_t n, double* out3, uint32_t* prim_out, int nthreads) {
+    (void)nv;
+    struct CNode {
+        double lo[3], hi[3];
+        uint32_t left, first, count;  // count > 0: leaf of tris[first, first + count)
+    };
+    std::vector<uint32_t> tris(nt);
+    std::vector<double> cen(3 * (size_t)nt), blo(3 * (size_t)nt), bhi(3 * (size_t)nt);
+    for (uint32_t t = 0; t < nt; ++t) {
+        tris[t] = t;
+        for (int k = 0; k < 3; ++k) {
+            double lo = INFINITY, hi = -INFINITY;
+            for (int j = 0; j < 3; ++j) {
+                const double c = xyz[3 * (size_t)idx[3 * (size_t)t + j] + k];
+                lo = std::min(lo, c);
+                hi = std::max(hi, c);
+            }
+            const double pad = 1e-6 * (std::max(std::fabs(lo), std::fabs(hi)) + (hi - lo)) + 1e-30;
+            blo[3 * (size_t)t + k] = lo - pad;
+            bhi[3 * (size_t)t + k] = hi + pad;
+            cen[3 * (size_t)t + k] = 0.5 * (lo + hi);
+        }
+    }
+    std::vector<CNode> nodes;
+    nodes.reserve(nt ? 2 * (size_t)nt / 4 + 8 : 1);
+    // Median split on the longest centroid axis, leaves of up to 8.
+    struct Todo {
+        uint32_t node, first, count;
+    };
+    std::vector<Todo> todo;
+    nodes.push_back(CNode{});
+    todo.push_back({0, 0, nt});
+    while (!todo.empty()) {
+        const Todo w = todo.back();
+        todo.pop_back();
+        CNode nd{};
+        for (int k = 0; k < 3; ++k) {
+            nd.lo[k] = INFINITY;
+            nd.hi[k] = -INFINITY;
+        }
+        double clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t j = w.first; j < w.first + w.count; ++j) {
+            const uint32_t t = tris[j];
+            for (int k = 0; k < 3; ++k) {
+                nd.lo[k] = std::min(nd.lo[k], blo[3 * (size_t)t + k]);
+                nd.hi[k] = std::max(nd.hi[k], bhi[3 * (size_t)t + k]);
+                clo[k] = std::min(clo[k], cen[3 * (size_t)t + k]);
+                chi[k] = std::max(chi[k], cen[3 * (size_t)t + k]);
+            }
+        }
+        if (w.count <= 8) {
+            nd.first = w.first;
+            nd.count = w.count;
+            nodes[w.node] = nd;
+            continue;
+        }
+        int ax = 0;
+        for (int k = 1; k < 3; ++k)
+            if (chi[k] - clo[k] > chi[ax] - clo[ax]) ax = k;
+        const uint32_t half = w.count / 2;
+        std::nth_element(tris.begin() + w.first, tris.begin() + w.first + half, tris.begin() + w.first + w.count,
+                         [&](uint32_t a, uint32_t b) {
+                             const double ca = cen[3 * (size_t)a + ax], cb = cen[3 * (size_t)b + ax];
+                             return ca < cb || (ca == cb && a < b);
+                         });
+        nd.left = (uint32_t)nodes.size();
+        nd.count = 0;
+        nodes[w.node] = nd;
+        nodes.push_back(CNode{});
+        nodes.push_back(CNode{});
+        todo.push_back({nd.left, w.first, half});
+        todo.push_back({nd.left + 1, w.first + half, w.count - half});
+    }
+    auto enter = [](const CNode& b, const double o[3], const double d[3], double& tin) {
+        double t0 = 0.0, t1 = INFINITY;
+        for (int k = 0; k < 3; ++k) {
+            if (d[k] == 0.0) {
+                if (o[k] < b.lo[k] || o[k] > b.hi[k]) return false;
+                continue;
+            }
+            double a = (b.lo[k] - o[k]) / d[k], c = (b.hi[k] - o[k]) / d[k];
+            if (a > c) std::swap(a, c);
+            t0 = std::max(t0, a);
+            t1 = std::min(t1, c);
+        }
+        tin = t0;
+        return t0 <= t1;
+    };
+    parallel_chunks(n, nthreads, 64, [&](uint32_t b, uint32_t e) {
+        std::vector<uint32_t> stack;
+        for (uint32_t i = b; i < e; ++i) {
+            const float* r = rays + 8 * (size_t)i;
+            double o[3] = {r[0], r[1], r[2]}, d[3] = {r[4], r[5], r[6]};
+            double best = INFINITY, ba = 0, bb = 0;
+            uint32_t bp = 0xFFFFFFFFu;
+            stack.assign(1, 0u);
+            while (nt && !stack.empty()) {
+                const CNode& nd = nodes[stack.back()];
+                stack.pop_back();
+                double tin;
+                if (!enter(nd, o, d, tin) || tin > best) continue;
+                if (nd.count) {
+                    for (uint32_t j = nd.first; j < nd.first + nd.count; ++j) {
+                        const uint32_t t = tris[j];
+                        double tt, a, bta;
+                        if (fp64_tri(xyz, idx, t, o, d, tt, a, bta) && (tt < best || (tt == best && t < bp))) {
+                            best = tt;
+                            ba = a;
+                            bb = bta;
+                            bp = t;
+                        }
+                    }
+                } else {
+                    stack.push_back(nd.left);
+                    stack.push_back(nd.left + 1);
                 }
             }
             out3[3 * i] = best;
