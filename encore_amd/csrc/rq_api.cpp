@@ -99,16 +99,32 @@ static int grid_for(rq_device* dev, uint32_t accel, uint32_t pred, TraceKind kin
 }
 
 // n_dev: the ray count is read on the device from there (n bounds it).
+// fixed: scratch of fixed_bytes given by the caller (counter, then spill
+// stacks); the grid then shrinks to what it holds and nothing is allocated.
 static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, uint32_t* occ, TraceKind kind,
-                  hipStream_t s, unsigned long long* counts = nullptr, const uint32_t* n_dev = nullptr) {
+                  hipStream_t s, unsigned long long* counts = nullptr, const uint32_t* n_dev = nullptr,
+                  uint32_t* fixed = nullptr, size_t fixed_bytes = 0) {
     if (n == 0) return;
     if (n > kMaxCount) throw Error(RQ_ERR_CAPACITY, "at most 2^30 rays per call");
     rq_device* dev = sc->dev;
     RQ_HIP(hipSetDevice(dev->ordinal));
-    const int grid = grid_for(dev, sc->accel, sc->predicate, kind, sc->width, n);
+    int grid = grid_for(dev, sc->accel, sc->predicate, kind, sc->width, n);
     const uint32_t words = scene_spill_words(sc);
-    const size_t spill_bytes = (size_t)grid * block_threads(sc->accel) * words * sizeof(uint32_t);
-    uint32_t* scratch = stream_scratch(dev, s, spill_bytes);
+    uint32_t* scratch = nullptr;
+    if (fixed) {
+        // A persistent launch serves every ray with fewer workgroups, only
+        // more slowly: keep as many as the fixed spill region holds.
+        const size_t per_block = (size_t)block_threads(sc->accel) * words * sizeof(uint32_t);
+        if (fixed_bytes < 256 + per_block)
+            throw Error(RQ_ERR_CAPACITY, "the scene's traversal stacks need " + std::to_string(256 + per_block) +
+                                             " bytes of scratch per workgroup; the static region holds " +
+                                             std::to_string(fixed_bytes));
+        if (per_block) grid = (int)std::min<size_t>((size_t)grid, (fixed_bytes - 256) / per_block);
+        scratch = fixed;
+    } else {
+        const size_t spill_bytes = (size_t)grid * block_threads(sc->accel) * words * sizeof(uint32_t);
+        scratch = stream_scratch(dev, s, spill_bytes);
+    }
     TraceLaunch L{};
     L.nodes = sc->nodes.ptr;
     L.tris = sc->tris.as<TriRec>();
@@ -144,6 +160,11 @@ static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, ui
 
 void trace_closest(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, hipStream_t s, const uint32_t* n_dev) {
     trace(sc, rays, n, hits, nullptr, TraceKind::Closest, s, nullptr, n_dev);
+}
+
+void trace_closest_in(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, hipStream_t s, uint32_t* scratch,
+                      size_t bytes) {
+    trace(sc, rays, n, hits, nullptr, TraceKind::Closest, s, nullptr, nullptr, scratch, bytes);
 }
 
 static void vertex_bounds(const float* xyz, uint32_t nverts, const rq_scene_desc& dd, float bmin[3], float bmax[3]);
@@ -300,6 +321,7 @@ rq_status rq_device_open(int ordinal, rq_device** out) {
         if (const char* e = std::getenv("RQ_BLOCKS_PER_CU")) d->kc.blocks_per_cu = std::atoi(e);
         if (const char* e = std::getenv("RQ_FETCH_THRESHOLD")) d->kc.fetch_threshold = std::atoi(e);
         if (d->kc.stack != 8 && d->kc.stack != 16 && d->kc.stack != 32) d->kc.stack = 8;
+        bind_static_scratch(d.get());  // with this device current
         *out = d.release();
     });
 }

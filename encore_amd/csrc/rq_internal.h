@@ -454,6 +454,20 @@ const std::vector<float>& ph_phi_table();
 void trace_closest(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, hipStream_t s,
                    const uint32_t* n_dev = nullptr);
 
+// Allocation-free queries (noalloc.hip): the work counter and the spill
+// stacks of rq_intersect_closest_static live in a file-scope device array of
+// kStaticScratchBytes, one copy per GPU.  bind_static_scratch (called by
+// rq_device_open with the device current) stores that GPU's address of the
+// array in dev->static_scratch.
+constexpr size_t kStaticScratchBytes = RQ_STATIC_SCRATCH_BYTES;
+void bind_static_scratch(rq_device* dev);
+// Closest hit with caller-provided scratch (counter in the first 256 bytes,
+// spill stacks after): the grid shrinks to what `bytes` can hold, and
+// RQ_ERR_CAPACITY results when not even one workgroup's spill stacks fit.
+// Allocates nothing.
+void trace_closest_in(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, hipStream_t s, uint32_t* scratch,
+                      size_t bytes);
+
 // Threads per block: the BVH kernels run one wave per block (their LDS use
 // per wave is large, so single-wave blocks pack a CU best); the kd kernel
 // four.
@@ -509,6 +523,8 @@ struct rq_device {
     // Per-stream scratch: the work counter in the first 256 bytes, spill
     // stacks after it.  Launches on one stream are ordered, so they can share.
     std::map<hipStream_t, rq::DevBuf> scratch;
+    // This GPU's address of the allocation-free entry point's static array.
+    uint32_t* static_scratch = nullptr;
     rq::KernelConfig kc;
     // Photon-emission workspace, kept between calls (grown on demand);
     // emissions on one device take turns on it.

@@ -194,6 +194,20 @@ rq_status rq_scene_load(rq_device* dev, const char* path, rq_scene** out);
 rq_status rq_intersect_closest(rq_scene* scene, const rq_ray* d_rays, uint32_t n,
                                rq_hit* d_hits, void* stream);
 
+/* Closest hit exactly as rq_intersect_closest, allocating nothing.  No
+ * counterpart in ENCORE: it is for callers that cannot allocate device memory
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * allocation).  The launch's work counter and traversal spill stacks live in
+ * a static device array of 64 MiB per GPU (RQ_STATIC_SCRATCH_BYTES), held
+ * until the process ends; the grid shrinks to the workgroups whose spill
+ * stacks fit there, and RQ_ERR_CAPACITY is returned when not even one does.
+ * Every call on one GPU shares that array and leaves its contents behind
+ * (the counter is cleared at the start of each call): calls that overlap on
+ * one GPU (other streams, other threads) race and must not be issued. */
+#define RQ_STATIC_SCRATCH_BYTES (64u << 20)
+rq_status rq_intersect_closest_static(rq_scene* scene, const rq_ray* d_rays, uint32_t n, rq_hit* d_hits,
+                                      void* stream);
+
 /* Any hit: d_occluded[i] = 1 iff some triangle is hit with tmin < t < tmax,
  * else 0.  With tmax = |L - p| this is PhotonMapperCPU::IsInShadow
  * (PhotonMapperCPU.cpp:1192-1207), which ENCORE answers with a full
