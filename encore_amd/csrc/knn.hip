@@ -329,8 +329,14 @@ __device__ uint32_t select_kth(const uint32_t* s_key, uint32_t n, uint32_t& need
         prefix |= digit << shift;
         mask |= 255u << shift;
         __builtin_amdgcn_wave_barrier();
+        // The caller reads t as a float bound, so end may be taken only while
+        // it is a float in [0, +inf]: a bucket of keys at or above 2^127
+        // would end at 0x80000000, which reads as -0.  (The callers pass more
+        // than k keys, all finite squared distances, so the bucket of the
+        // k-th key holds keys above it and never passes this test at that
+        // end; the bound keeps it so for any input.)
         const uint64_t end = (uint64_t)prefix + (1ull << shift);
-        if (need == in_bucket && end <= 0xFFFFFFFFull) {
+        if (need == in_bucket && end <= 0x7F800000ull) {
             need = 0;
             return (uint32_t)end;
         }

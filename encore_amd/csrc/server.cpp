@@ -365,7 +365,10 @@ rq_status rq_server_create(rq_device* dev, const rq_server_config* cfg, rq_serve
         if (cfg) s->cfg = *cfg;
         if (s->cfg.slots == 0) s->cfg.slots = 3;
         if (s->cfg.plain) s->cfg.slots = 1;
-        if (s->cfg.max_rays_per_launch == 0) s->cfg.max_rays_per_launch = 1u << 24;
+        // 2^20 rays per launch: larger launches add their copies and
+        // traversal to the latency of every request they hold (at 16M the
+        // served p99 under load was a third higher for the same throughput).
+        if (s->cfg.max_rays_per_launch == 0) s->cfg.max_rays_per_launch = 1u << 20;
         if (s->cfg.max_requests_per_launch == 0) s->cfg.max_requests_per_launch = 1024;
         s->kc = dev->kc;
         if (s->cfg.stack) s->kc.stack = (int)s->cfg.stack;
@@ -401,7 +404,7 @@ rq_status rq_server_submit(rq_server* s, rq_scene* scene, const rq_ray* h_rays, 
         if (scene->accel != RQ_ACCEL_BVH) throw Error(RQ_ERR_UNSUPPORTED, "the server serves BVH scenes");
         const uint64_t bytes = request_bytes(n, any_hit);
         if (bytes > s->arena.size) throw Error(RQ_ERR_CAPACITY, "request larger than the server's HBM budget");
-        if (n > s->cfg.max_rays_per_launch) throw Error(RQ_ERR_CAPACITY, "request larger than max_rays_per_launch");
+        if (n > kMaxCount) throw Error(RQ_ERR_CAPACITY, "at most 2^30 rays per request");
         auto r = std::make_unique<Request>();
         r->scene = scene;
         r->h_rays = h_rays;

@@ -176,13 +176,20 @@ def geodesic_sphere(freq):
     ii, jj = ii[mask], jj[mask]
     lid = -np.ones((freq + 1, freq + 1), np.int64)
     lid[ii, jj] = np.arange(len(ii))
-    tri = []
+    # Per lattice row i: the upward triangles (i, j), (i+1, j), (i, j+1) for
+    # j < freq - i, then the downward (i+1, j), (i+1, j+1), (i, j+1) for
+    # j < freq - i - 1, interleaved up, down, up, ... along the row.
+    rows = []
     for i in range(freq):
-        for j in range(freq - i):
-            tri.append([lid[i, j], lid[i + 1, j], lid[i, j + 1]])
-            if j + i + 1 < freq:
-                tri.append([lid[i + 1, j], lid[i + 1, j + 1], lid[i, j + 1]])
-    tri = np.array(tri, np.int64)
+        j = np.arange(freq - i)
+        up = np.stack([lid[i, j], lid[i + 1, j], lid[i, j + 1]], 1)
+        jd = j[:-1]
+        down = np.stack([lid[i + 1, jd], lid[i + 1, jd + 1], lid[i, jd + 1]], 1)
+        row = np.empty((len(up) + len(down), 3), np.int64)
+        row[0::2] = up
+        row[1::2] = down
+        rows.append(row)
+    tri = np.concatenate(rows)
     base = 0
     for a, b, c in f0:
         p = (v0[a][None] * (freq - ii - jj)[:, None] + v0[b][None] * ii[:, None] + v0[c][None] * jj[:, None]) / freq
@@ -193,15 +200,24 @@ def geodesic_sphere(freq):
     f = np.concatenate(faces)
     v /= np.linalg.norm(v, axis=1, keepdims=True)
     key = np.round(v * 1e9).astype(np.int64)
-    _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
-    inv = inv.reshape(-1)
+    # Unique rows in lexicographic order, each with its first occurrence (what
+    # np.unique(key, axis=0, return_index=True, return_inverse=True) returns),
+    # through a stable lexsort.
+    order = np.lexsort((key[:, 2], key[:, 1], key[:, 0]))
+    ks = key[order]
+    start = np.ones(len(ks), bool)
+    start[1:] = np.any(ks[1:] != ks[:-1], axis=1)
+    first = order[start]
+    inv = np.empty(len(ks), np.int64)
+    inv[order] = np.cumsum(start) - 1
     return v[first], inv[f]
 
 
 def value_noise3(p, seed, freqs=(4.0, 8.0, 16.0), amps=(1.0, 0.5, 0.25)):
     """Sum of value-noise octaves at points p (n,3), normalised to [-1,1].
     Lattice values are hashed from (seed, octave, cell) with SplitMix64 and
-    interpolated trilinearly with a smoothstep fade."""
+    interpolated trilinearly with a smoothstep fade.  The lattice values of
+    an octave are hashed once over the cells the points span and gathered."""
     out = np.zeros(len(p))
     for o, (fr, am) in enumerate(zip(freqs, amps)):
         q = p * fr
@@ -209,14 +225,20 @@ def value_noise3(p, seed, freqs=(4.0, 8.0, 16.0), amps=(1.0, 0.5, 0.25)):
         w = q - c
         w = w * w * (3.0 - 2.0 * w)
         ci = c.astype(np.int64)
+        lo = ci.min(axis=0)
+        ext = ci.max(axis=0) - lo + 2  # lattice points per axis, corners included
+        gx, gy, gz = np.meshgrid(np.arange(ext[0]) + lo[0], np.arange(ext[1]) + lo[1], np.arange(ext[2]) + lo[2],
+                                 indexing="ij")
+        key = ((gx & 0xFFFFF) << 40) | ((gy & 0xFFFFF) << 20) | (gz & 0xFFFFF)
+        hv = splitmix64(seed + 0x1000 * (o + 1), key.reshape(-1).astype(np.uint64))
+        table = (hv >> np.uint64(11)).astype(np.float64) * (2.0 / 9007199254740992.0) - 1.0
+        rel = ci - lo
+        flat = (rel[:, 0] * ext[1] + rel[:, 1]) * ext[2] + rel[:, 2]
         acc = np.zeros(len(p))
         for dx in (0, 1):
             for dy in (0, 1):
                 for dz in (0, 1):
-                    key = (((ci[:, 0] + dx) & 0xFFFFF) << 40) | (((ci[:, 1] + dy) & 0xFFFFF) << 20) | (
-                        (ci[:, 2] + dz) & 0xFFFFF)
-                    hv = splitmix64(seed + 0x1000 * (o + 1), key.astype(np.uint64))
-                    val = (hv >> np.uint64(11)).astype(np.float64) * (2.0 / 9007199254740992.0) - 1.0
+                    val = table[flat + (dx * ext[1] + dy) * ext[2] + dz]
                     wx = w[:, 0] if dx else 1.0 - w[:, 0]
                     wy = w[:, 1] if dy else 1.0 - w[:, 1]
                     wz = w[:, 2] if dz else 1.0 - w[:, 2]

@@ -273,7 +273,8 @@ void rq_batch_destroy(rq_batch* batch);
 typedef struct rq_server_config {
     uint32_t slots;                    /* launches in flight (0 = 3) */
     uint32_t plain;                    /* 1 = one request per launch */
-    uint32_t max_rays_per_launch;      /* 0 = 2^24 */
+    uint32_t max_rays_per_launch;      /* no request joins a launch that would then exceed this many
+                                          rays; a larger request runs alone (0 = 2^20) */
     uint32_t max_requests_per_launch;  /* 0 = 1024 */
     uint64_t hbm_budget_bytes;         /* request arena (0 = a quarter of free HBM) */
     uint64_t hbm_reserve_bytes;        /* HBM the arena must leave free */

@@ -79,3 +79,30 @@ def test_wave_radiance_edge_cases(device):
     assert cnt.tolist() == [0, 0, 0, 0] and np.all(rgb == 0)
     pm.destroy()
     pm0.destroy()
+
+
+def test_wave_radiance_huge_distances(device):
+    """Squared distances near the top of the float range, many at or above
+    2^127 (keys 0x7F......, the last bucket of the radix selection's first
+    pass): photons ~1e19 from the queries, r2 = fl(1.8e19^2) = 3.24e38, some
+    photons beyond float range (d2 = inf, never selected).  Counts and
+    estimates equal the heap kernel's and the oracle's."""
+    rng = np.random.default_rng(11)
+    P = rng.uniform(-1.15e19, 1.15e19, (3000, 3)).astype(np.float32)
+    N = np.tile(np.float32([0, 1, 0]), (len(P), 1))
+    W = rng.uniform(0.1, 1.0, (len(P), 3)).astype(np.float32)
+    Q = rng.uniform(-2e18, 2e18, (512, 3)).astype(np.float32)
+    QN = np.tile(np.float32([0, 1, 0]), (len(Q), 1))
+    pm = rq.PhotonMap(device, P, N, W)
+    r = 1.8e19
+    d2 = ((P[None, :, :].astype(np.float64) - Q[:, None, :]) ** 2).sum(-1)
+    assert (d2 >= 2.0 ** 127).mean() > 0.2  # keys with top byte 0x7F are common
+    for k in (32, 100):
+        log("case: wave radiance at huge distances k=%d" % k)
+        wave_rgb, wave_cnt = _radiance(pm, Q, QN, k, r, False, False)
+        lane_rgb, lane_cnt = _radiance(pm, Q, QN, k, r, True, False)
+        ref = oracle.PhotonMap(P, N, W).radiance(Q, QN, k, r)
+        assert np.array_equal(wave_cnt, lane_cnt) and (wave_cnt == k).mean() > 0.9
+        np.testing.assert_allclose(wave_rgb, lane_rgb, rtol=1e-5)
+        np.testing.assert_allclose(wave_rgb, ref, rtol=1e-5)
+    pm.destroy()

@@ -173,26 +173,49 @@ def test_watertight_bvh(device, cornell, sphere, width):
         assert c["edge_band"] <= 1e-3 * c["n"]
 
 
-def test_fp64_error_report(device, sphere):
-    """FP64 brute-force oracle on a 1% sample: the engine's closest prim is the
-    FP64 closest prim (up to ties) and t, u, v errors stay below 1e-5 rel."""
+@pytest.mark.parametrize("predicate", ["encore", "watertight"])
+def test_fp64_error_report(device, sphere, predicate):
+    """FP64 brute-force oracle (no flush) on a 1% sample of the bounce rays
+    and of the eye rays (every 100th ray).  The engine's closest prim is the
+    FP64 closest prim on at least 99% of the hits.  ENCORE predicate: on
+    those rays the engine's (t, u, v) are the reference's bits, so its FP64
+    error is exactly the reference's own, which is reported (Cross()'s flush
+    and float32 cancellation put it far above 1e-5 on some rays).  Watertight
+    predicate: t within 1e-5 relative of FP64; u and v within 1e-5 relative
+    plus an absolute band for the float32 rounding of the vertex coordinates
+    relative to the ray origin, 64 ulps of (largest |v_i - o| component /
+    shortest edge) per ray (measured on every ray of these sets: at most 48,
+    99th percentile 3; a swapped or misnamed barycentric is off by ~0.1)."""
     s = sphere
-    rays = s["bounce"][::100]
-    log("case: fp64 sample", len(rays))
-    sc = rq.Scene(device, s["xyz"], s["idx"], accel="bvh")
-    got = _trace(sc, rays)
-    gt, gu, gv, gp = split_hits(got)
-    f, fp = oracle.fp64_closest(s["xyz"], s["idx"], rays)
-    hit = (fp != oracle.NO_HIT) & (gp != oracle.NO_HIT)
-    same = hit & (gp == fp)
-    rel_t = np.abs(gt[same] - f[same, 0]) / np.abs(f[same, 0])
-    err_u = np.abs(gu[same] - f[same, 1])
-    err_v = np.abs(gv[same] - f[same, 2])
-    log("fp64: n=%d same=%d maxrel_t=%.3g max|du|=%.3g max|dv|=%.3g" % (len(rays), same.sum(), rel_t.max(),
-                                                                      err_u.max(), err_v.max()))
-    assert same.sum() >= 0.99 * hit.sum()
-    assert rel_t.max() < 1e-5
-    assert max(err_u.max(), err_v.max()) < 1e-4
+    sc = rq.Scene(device, s["xyz"], s["idx"], accel="bvh", predicate=predicate)
+    for name, rays, ref in (("bounce", s["bounce"][::100], s["bounce_ref"][::100]),
+                            ("eye", s["primary"][::100], s["primary_ref"][::100])):
+        log("case: fp64 sample", predicate, name, len(rays))
+        got = _trace(sc, rays)
+        gt, gu, gv, gp = split_hits(got)
+        f, fp = oracle.fp64_closest(s["xyz"], s["idx"], rays)
+        hit = (fp != oracle.NO_HIT) & (gp != oracle.NO_HIT)
+        same = hit & (gp == fp)
+        rel_t = np.abs(gt[same] - f[same, 0]) / np.abs(f[same, 0])
+        du, dv = np.abs(gu[same] - f[same, 1]), np.abs(gv[same] - f[same, 2])
+        tri = s["xyz"][s["idx"][gp[same]]].astype(np.float64)
+        reach = np.abs(tri - rays[same, None, 0:3].astype(np.float64)).max(axis=(1, 2))
+        emin = np.linalg.norm(tri - np.roll(tri, 1, axis=1), axis=2).min(axis=1)
+        band = 64 * 2.0 ** -24 * reach / emin
+        ru = (du - 1e-5 * np.abs(f[same, 1])) / band
+        rv = (dv - 1e-5 * np.abs(f[same, 2])) / band
+        log("fp64 %s %s: n=%d hits=%d same=%d max rel t=%.3g max |du|=%.3g |dv|=%.3g, of the band %.3g %.3g" % (
+            predicate, name, len(rays), hit.sum(), same.sum(), rel_t.max(), du.max(), dv.max(), ru.max(), rv.max()))
+        assert same.sum() >= 0.99 * hit.sum()
+        if predicate == "encore":
+            rp = ref[:, 3].view(np.uint32)
+            m = same & (rp == gp)
+            assert m.sum() >= 0.99 * same.sum()
+            assert np.array_equal(got[m].view(np.uint32), ref[m].view(np.uint32))
+        else:
+            assert rel_t.max() <= 1e-5
+            assert ru.max() <= 1.0 and rv.max() <= 1.0
+    sc.destroy()
 
 
 @pytest.mark.parametrize("width", [2, 4])
