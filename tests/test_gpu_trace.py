@@ -315,3 +315,32 @@ def test_counts_beyond_2_30_are_refused(device):
     assert float(hits[0, 0]) > 0.0  # a hit distance, or tmax on a miss
     pm.destroy()
     sc.destroy()
+
+
+def test_kd_split_snap_on_gpu(device):
+    """Kdtree.cpp:711 on the GPU's kd path: the hand-worked scene of
+    tests/golden/oracle_pins.json, where the ray from x = 0.0003 (between the
+    snapped split 0 and the stored ~0.0005) visits only the right child and
+    misses the wall at x = 0.0004 that brute force finds; plain launch and a
+    batched launch, bit for bit as the oracle."""
+    import json
+    import os
+
+    c = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_pins.json")))["kd_snap"]
+    xyz, idx = np.array(c["xyz"], np.float32), np.array(c["idx"], np.uint32)
+    rays = np.array(c["rays"], np.float32)
+    ref = oracle.KdTree(xyz, idx).intersect(rays)
+    sc = rq.Scene(device, xyz, idx, accel="kd")
+    log("case: kd split snap")
+    got = _trace(sc, rays)
+    assert got[:, 3].view(np.uint32).tolist() == c["kd_prim"]
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    d = torch.from_numpy(rays).cuda()
+    out = torch.full((len(rays), 4), -5.0, device="cuda")
+    b = rq.Batch(device)
+    b.set([(sc, d[:1], out[:1]), (sc, d[1:], out[1:])])
+    b.launch(graph=False)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+    b.destroy()
+    sc.destroy()
