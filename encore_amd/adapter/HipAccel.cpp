@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 
+#include "HitFill.h"
 #include "IModel.h"
 #include "Primitive.h"
 #include "Triangle.h"
@@ -172,12 +173,11 @@ void HipAccel::fillHit(Ray& ray, const rq_hit& h, HitInfo* out) const {
     if (h.prim == RQ_NO_HIT || h.prim >= m_Prims.size()) return;
     const float* f = &m_Xyz[9 * (size_t)h.prim];
     const Point3 v0(f[0], f[1], f[2]), v1(f[3], f[4], f[5]), v2(f[6], f[7], f[8]);
-    const Vector3 edge1(v0, v1), edge2(v0, v2);
+    const EncoreHitFields hf = encore_hit_fields(v0, v1, v2, ray.Direction(), h);
     out->hitTime = h.t;
-    out->hitNormal = Cross(edge1, edge2);
-    out->hitNormal.Normalize();
-    out->hitPoint = v0 + (v1 - v0) * h.u + (v2 - v0) * h.v;
-    out->isEntering = (Dot(ray.Direction(), out->hitNormal) < 0);
+    out->hitNormal = hf.normal;
+    out->hitPoint = hf.point;
+    out->isEntering = hf.entering;
     out->hitObject = m_Prims[h.prim];
     out->bHasInfo = true;
 }

@@ -8,13 +8,31 @@
 
 HipPhotonMap::HipPhotonMap(size_t maxPhotonCount, int ordinal, rq_photon_layout layout)
     : m_max(maxPhotonCount), m_caustic(false), m_built(false), m_ordinal(ordinal), m_layout(layout), m_dev(0),
-      m_map(0) {
+      m_map(0), m_stage(0), m_stageBytes(0) {
     m_photons.reserve(maxPhotonCount);
 }
 
 HipPhotonMap::~HipPhotonMap() {
+    if (m_stage) (void)hipFree(m_stage);
     if (m_map) rq_photons_destroy(m_map);
     if (m_dev) rq_device_close(m_dev);
+}
+
+// One device staging buffer for the gather's queries and results, kept
+// between calls and grown on demand: a one-point GetNearestNPhotons then
+// costs copies and a launch, not six allocations.
+bool HipPhotonMap::ensureStage(size_t bytes) {
+    if (bytes <= m_stageBytes) return true;
+    if (m_stage) (void)hipFree(m_stage);
+    m_stage = 0;
+    m_stageBytes = 0;
+    if (hipMalloc(&m_stage, bytes) != hipSuccess) {
+        m_stage = 0;
+        printf("HipPhotonMap: device staging allocation of %zu bytes failed\n", bytes);
+        return false;
+    }
+    m_stageBytes = bytes;
+    return true;
 }
 
 void HipPhotonMap::AddPhoton(Photon& p) {
@@ -74,10 +92,17 @@ void HipPhotonMap::gather(int N, float maxDistance, const float* loc, const floa
     rgb.assign(3 * nq, 0.0f);
     if (!m_built) BalanceAndConstruct();
     if (!m_map || nq == 0 || N <= 0) return;
-    void *dq = 0, *dn = 0, *di = 0, *dd = 0, *dc = 0, *dr = 0;
-    bool ok = hipMalloc(&dq, nq * 12) == hipSuccess && hipMalloc(&dn, nq * 12) == hipSuccess &&
-              hipMalloc(&di, nq * N * 4) == hipSuccess && hipMalloc(&dd, nq * N * 4) == hipSuccess &&
-              hipMalloc(&dc, nq * 4) == hipSuccess && hipMalloc(&dr, nq * 12) == hipSuccess;
+    // Carve the staging buffer: queries, normals, indices, d2, counts, rgb.
+    const size_t sizes[6] = {nq * 12, nq * 12, nq * (size_t)N * 4, nq * (size_t)N * 4, nq * 4, nq * 12};
+    size_t off[6], total = 0;
+    for (int i = 0; i < 6; ++i) {
+        off[i] = total;
+        total += (sizes[i] + 255) & ~(size_t)255;
+    }
+    bool ok = ensureStage(total);
+    char* base = static_cast<char*>(m_stage);
+    void *dq = base + off[0], *dn = base + off[1], *di = base + off[2], *dd = base + off[3], *dc = base + off[4],
+         *dr = base + off[5];
     if (ok) {
         ok = hipMemcpy(dq, loc, nq * 12, hipMemcpyHostToDevice) == hipSuccess &&
              hipMemcpy(dn, nrm, nq * 12, hipMemcpyHostToDevice) == hipSuccess;
@@ -98,9 +123,6 @@ void HipPhotonMap::gather(int N, float maxDistance, const float* loc, const floa
         (void)hipMemcpy(cnt.data(), dc, nq * 4, hipMemcpyDeviceToHost);
         if (radiance) (void)hipMemcpy(rgb.data(), dr, nq * 12, hipMemcpyDeviceToHost);
     }
-    void* bufs[] = {dq, dn, di, dd, dc, dr};
-    for (void* b : bufs)
-        if (b) (void)hipFree(b);
 }
 
 void HipPhotonMap::GetNearestNPhotonsBatch(const int N, const float maxDistance, const std::vector<float>& locations,

@@ -46,6 +46,7 @@ private:
     void gather(int N, float maxDistance, const float* loc, const float* nrm, size_t nq, bool radiance,
                 std::vector<uint32_t>& idx, std::vector<float>& d2, std::vector<uint32_t>& cnt,
                 std::vector<float>& rgb);
+    bool ensureStage(size_t bytes);
 
     std::vector<Photon> m_photons;
     size_t m_max;
@@ -55,4 +56,6 @@ private:
     rq_photon_layout m_layout;
     rq_device* m_dev;
     rq_photons* m_map;
+    void* m_stage;        // device staging for gather(), kept between calls
+    size_t m_stageBytes;
 };
