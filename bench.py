@@ -52,10 +52,10 @@ def parse():
     ap.add_argument("--no-counts", action="store_true")
     ap.add_argument("--no-exact", action="store_true", help="skip the bit-exact kd-path measurement")
     ap.add_argument("--dump", default="", help="write the adjudicated parity differences to this .npz")
-    ap.add_argument("--workload", choices=["c3", "c1", "c2", "c4", "c5", "fg", "pt", "refit"], default="c3",
+    ap.add_argument("--workload", choices=["c3", "c1", "c2", "c4", "c5", "fg", "ic", "pt", "refit"], default="c3",
                     help="c3 (default): 8M incoherent rays at 1.31M triangles; c1: Cornell box eye rays; c2: "
                          "1080p eye + shadow rays at 1.31M triangles; c4: photon gather; c5: serving; fg: final "
-                         "gather; pt: photon emission; refit: scene update")
+                         "gather; ic: irradiance-cache lookups; pt: photon emission; refit: scene update")
     ap.add_argument("--pt-sample", type=int, default=100000,
                     help="pt: global-map capacity of the CPU baseline and parity sample")
     ap.add_argument("--seed", type=int, default=7, help="pt: seed of the emission draws")
@@ -638,6 +638,92 @@ def _final_gather_workload(args, dev, thr, trace_fn):
     return 0
 
 
+def _irrcache_workload(args, dev, thr, trace_fn):
+    """Irradiance-cache lookups (IrrCache::GetIrrValuesNearPoint and the final
+    gather's weighted average, IrrCacheOctree.h:294-326, PhotonMapperCPU.cpp:
+    1087-1103) at the ~1M hit points of a 1000x1000 view of the Cornell box at
+    1/20 scale (34,816 triangles).  The cache holds what the final gather
+    (rq_final_gather: 8x8 samples, indirect and direct estimates as the fg
+    workload) computes at the 128x128 view's hit points, with ENCORE's
+    FGTolerance a = 0.08 and root box (-75, -75, -75) + 150^3 scaled by 1/20."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from encore_amd import rq, scenes
+
+    xyz, idx = scenes.config4_scene()
+    bvh = rq.Scene(dev, xyz, idx, accel="bvh")
+    P, N, W, B = scenes.trace_photons(xyz, idx, trace_fn(bvh), 2000000, batch=1 << 18, with_bounce=True)
+    ind, dr = B >= 1, B == 0
+    gi = rq.PhotonMap(dev, P[ind], N[ind], W[ind])
+    gd = rq.PhotonMap(dev, P[dr], N[dr], W[dr])
+    Qc, QNc = scenes.photon_queries(xyz, idx, trace_fn(bvh), res=128)
+    nper = 64
+    dirs = scenes.hemisphere_dirs(QNc, 8, scenes.Stream(scenes.SEED_BASE + 6))
+    diffuse = np.random.default_rng(6).uniform(0.2, 0.9, (len(idx), 3)).astype(np.float32)
+    rgb_c = torch.empty((len(Qc), 3), device="cuda")
+    ri_c = torch.empty((len(Qc),), device="cuda")
+    bvh.final_gather(gi, 100, 0.15, gd, 100, 0.10, torch.from_numpy(diffuse).cuda(), torch.from_numpy(Qc).cuda(),
+                     torch.from_numpy(dirs).cuda(), nper, rgb_c, ri_c)
+    torch.cuda.synchronize()
+    val, ri = rgb_c.cpu().numpy(), ri_c.cpu().numpy()
+    a, lo, size = 0.08, (-3.75, -3.75, -3.75), (7.5, 7.5, 7.5)
+    cache = rq.IrrCache(dev, a, lo, size)
+    st = cache.add(val, Qc, QNc, ri)
+    Q, QN = scenes.photon_queries(xyz, idx, trace_fn(bvh), res=1000)
+    dq, dqn = torch.from_numpy(Q).cuda(), torch.from_numpy(QN).cuda()
+    rgb = torch.empty((len(Q), 3), device="cuda")
+    ws = torch.empty((len(Q),), device="cuda")
+    cnt = torch.empty((len(Q),), dtype=torch.int32, device="cuda")
+    t = time.perf_counter()
+    cache.query(dq, dqn, rgb, ws, cnt)
+    torch.cuda.synchronize()
+    first_ms = (time.perf_counter() - t) * 1e3  # includes building the index
+    wall, ev = _time_gpu(lambda: cache.query(dq, dqn, rgb, ws, cnt), args.steps, args.warmup)
+    info = cache.info()
+    n_look, n_exam, n_cnt = cache.counts(dq, dqn, torch.empty_like(rgb))
+    # Algorithmic bytes per lookup: query position and normal (24), outputs
+    # (rgb, weight sum, count: 20), the cell's range (8), and per cached value
+    # examined its index (4) and record (48: position and Ri, normal, value).
+    alg = 52.0 * n_look + 52.0 * n_exam
+    achieved = alg / (ev / 1e3) / 1e9
+    log("irrcache: %d values (%d global), index %d cells; %.2f values examined and %.2f counted per lookup; "
+        "%.3f ms per call (first %.1f ms)" % (info["count"], info["global_count"], info["index_cells"],
+                                             n_exam / n_look, n_cnt / n_look, ev, first_ms))
+    # CPU baseline: the reference's walk of the whole list per lookup, on a
+    # strided sample; bit-exact parity on the same sample.
+    o = oracle.IrrCache(a, lo, size)
+    ost = o.add(val, Qc, QNc, ri)
+    stride = max(1, len(Q) // 20000)
+    sq, sqn = np.ascontiguousarray(Q[::stride]), np.ascontiguousarray(QN[::stride])
+    t = time.perf_counter()
+    orgb, ows, ocnt = o.query(sq, sqn, nthreads=thr)
+    cpu_s = time.perf_counter() - t
+    g = (rgb.cpu().numpy()[::stride], ws.cpu().numpy()[::stride], cnt.cpu().numpy().view(np.uint32)[::stride])
+    eq = np.all(g[0].view(np.uint32) == orgb.view(np.uint32), axis=1) & (g[1].view(np.uint32) == ows.view(np.uint32)) & (
+        g[2] == ocnt)
+    parity = {"sample": int(len(sq)), "bit_equal": int(eq.sum()), "status_equal": bool(np.array_equal(st, ost)),
+              "lookups_with_values": int((g[2] > 0).sum())}
+    log("parity", parity)
+    _line("Mlookups/s irradiance cache (final gather, a = 0.08)", len(Q) / (ev / 1e3) / 1e6, "Mlookups/s", ev,
+          {"workload": "irradiance-cache lookups at %d hit points of the Cornell box at 1/20 scale (%d triangles); "
+                       "cache of %d final-gather values (128x128 view, 8x8 samples), a = 0.08" % (
+                           len(Q), len(idx), info["count"])},
+          roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "alg_bytes_per_lookup": round(alg / n_look, 1),
+                    "values_examined_per_lookup": round(n_exam / n_look, 3),
+                    "values_counted_per_lookup": round(n_cnt / n_look, 3), "kernel": "ic_query"},
+          index=info, first_call_ms_incl_index=round(first_ms, 2), parity=parity,
+          cpu_baseline={"value": round(len(sq) / cpu_s / 1e6, 5), "unit": "Mlookups/s", "cores": thr, "kind": "port",
+                        "sample": "%d lookups (every %dth), the whole list per lookup as IrrCache's root list" % (
+                            len(sq), stride)})
+    for x in (cache, gi, gd, bvh):
+        x.destroy()
+    return 0
+
+
 def _photon_emission_workload(args, dev, thr):
     """Photon emission (PhotonMapperCPU::EmitPhotons / TracePhoton,
     PhotonMapperCPU.cpp:503-775) into the global (1,000,000), direct
@@ -917,6 +1003,8 @@ def run_workload(args):
         return _final_gather_workload(args, dev, thr, trace_fn)
     if args.workload == "pt":
         return _photon_emission_workload(args, dev, thr)
+    if args.workload == "ic":
+        return _irrcache_workload(args, dev, thr, trace_fn)
     if args.workload == "refit":
         return _refit_workload(args, dev, thr)
     if args.workload == "c4":

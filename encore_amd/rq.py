@@ -169,6 +169,17 @@ def load():
         L.rq_emit_photons.argtypes = [_vp, _vp, _u32, ctypes.POINTER(PhotonParams), ctypes.POINTER(PhotonBuffers),
                                       ctypes.POINTER(PhotonResult), _vp]
         L.rq_emit_photons.restype = ctypes.c_int
+    if hasattr(L, "rq_irrcache_create"):
+        L.rq_irrcache_create.argtypes = [_vp, ctypes.c_float, _vp, _vp, ctypes.POINTER(_vp)]
+        L.rq_irrcache_add.argtypes = [_vp, _vp, _vp, _vp, _vp, _u32, _vp]
+        L.rq_irrcache_query.argtypes = [_vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp]
+        L.rq_irrcache_info.argtypes = [_vp, _vp, _vp, _vp, _vp]
+        L.rq_irrcache_counts.argtypes = [_vp, _vp, _vp, _u32, _vp, _vp, _vp]
+        L.rq_irrcache_counts.restype = ctypes.c_int
+        L.rq_irrcache_destroy.argtypes = [_vp]
+        L.rq_irrcache_destroy.restype = None
+        for name in ("rq_irrcache_create", "rq_irrcache_add", "rq_irrcache_query", "rq_irrcache_info"):
+            getattr(L, name).restype = ctypes.c_int
     L.rq_debug_fail_alloc_after.argtypes = [ctypes.c_int64]
     L.rq_debug_fail_alloc_after.restype = None
     L.rq_host_tree_build.argtypes = [_vp, _u32, _vp, _u32, ctypes.POINTER(SceneDesc), ctypes.POINTER(_vp)]
@@ -558,6 +569,62 @@ class PhotonMap:
         check(load().rq_radiance_estimate(self.h, _ptr(qpos), _ptr(qnrm), qpos.shape[0], k, float(max_dist),
                                           int(caustic), _ptr(rgb), _ptr(idx), _ptr(d2), _ptr(count),
                                           _stream(stream)))
+
+
+class IrrCache:
+    """The final gather's irradiance cache (IrrCache<Color>) on the GPU:
+    add() cached values in order, query() the weighted averages."""
+
+    def __init__(self, device, a, min_point=(-75.0, -75.0, -75.0), size=(150.0, 150.0, 150.0)):
+        h = _vp()
+        mp = np.ascontiguousarray(min_point, np.float32)
+        sz = np.ascontiguousarray(size, np.float32)
+        check(load().rq_irrcache_create(device.h, float(a), mp.ctypes.data, sz.ctypes.data, ctypes.byref(h)))
+        self.h = h
+
+    def add(self, value, pos, nrm, ri):
+        """Host arrays (n,3), (n,3), (n,3), (n,); returns the per-value status
+        (0 stored, 1 outside the root box, 2 refused); raises RQError
+        (status 4) when any value is refused, storing none."""
+        value, pos, nrm = (np.ascontiguousarray(x, np.float32).reshape(-1, 3) for x in (value, pos, nrm))
+        ri = np.ascontiguousarray(ri, np.float32).reshape(-1)
+        st = np.zeros(max(1, len(ri)), np.uint32)
+        check(load().rq_irrcache_add(self.h, value.ctypes.data, pos.ctypes.data, nrm.ctypes.data, ri.ctypes.data,
+                                     len(ri), st.ctypes.data))
+        return st[: len(ri)]
+
+    def query(self, qpos, qnrm, rgb, wsum=None, count=None, n=None, stream=None):
+        """Device tensors: qpos, qnrm (n,3) float32 -> rgb (n,3), wsum (n,), count (n,)."""
+        n = qpos.shape[0] if n is None else n
+        check(load().rq_irrcache_query(self.h, _ptr(qpos), _ptr(qnrm), n, _ptr(rgb), _ptr(wsum), _ptr(count),
+                                       _stream(stream)))
+
+    def counts(self, qpos, qnrm, rgb, n=None, stream=None):
+        """(lookups, cached values examined, values counted) of one lookup
+        call, from the instrumented kernel (synchronous)."""
+        n = qpos.shape[0] if n is None else n
+        out = np.zeros(3, np.uint64)
+        check(load().rq_irrcache_counts(self.h, _ptr(qpos), _ptr(qnrm), n, _ptr(rgb), out.ctypes.data,
+                                        _stream(stream)))
+        return tuple(int(x) for x in out)
+
+    def info(self):
+        c, g = ctypes.c_uint32(), ctypes.c_uint32()
+        cells, refs = ctypes.c_uint64(), ctypes.c_uint64()
+        check(load().rq_irrcache_info(self.h, ctypes.byref(c), ctypes.byref(g), ctypes.byref(cells),
+                                      ctypes.byref(refs)))
+        return dict(count=c.value, global_count=g.value, index_cells=cells.value, index_refs=refs.value)
+
+    def destroy(self):
+        if getattr(self, "h", None):
+            load().rq_irrcache_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:  # interpreter shutdown: the library may already be gone
+            pass
 
 
 def photon_balance_host(pos, layout="encore"):

@@ -389,6 +389,50 @@ rq_status rq_final_gather(rq_scene* scene, rq_photons* indirect, uint32_t k_ind,
                           void* stream);
 
 /* ---------------------------------------------------------------------
+ * Irradiance cache of the final gather (IrrCache<Color>,
+ * IrrCacheOctree.h:92-326; PhotonMapperCPU.cpp:175, 1087-1103, 1170-1173).
+ * --------------------------------------------------------------------- */
+typedef struct rq_irrcache rq_irrcache;
+
+/* IrrCache(a, minPoint, xSize, ySize, zSize): a is the final-gather
+ * tolerance (FGTolerance, pmConfig 0.08); the root box is min_point[3],
+ * size[3] (ENCORE: (-75, -75, -75), 150^3). */
+rq_status rq_irrcache_create(rq_device* dev, float a, const float* min_point, const float* size,
+                             rq_irrcache** out);
+/* AddElement for n cached values in order (host arrays): value[3n] (the
+ * illumination Color), pos[3n], normal[3n], ri[n] (the harmonic mean
+ * distance).  status[i] (may be NULL): 0 stored, 1 outside the root box and
+ * dropped (as AddElement drops it), 2 refused.  A value with 2 a Ri above
+ * the root's x size is one the reference's AddElement never returns from
+ * (it descends without end, IrrCacheOctree.h:229-241): the call then returns
+ * RQ_ERR_UNSUPPORTED and stores none of the n values. */
+rq_status rq_irrcache_add(rq_irrcache* cache, const float* value, const float* pos, const float* normal,
+                          const float* ri, uint32_t n, uint32_t* status);
+/* GetIrrValuesNearPoint and the caller's weighted average for nq lookups
+ * (device arrays d_qpos[3nq], d_qnrm[3nq]): d_rgb[3i..] = sum(value_j w_j) /
+ * sum(w_j) over the cached values with GetWeight w_j = 1 / (|p - x_j| / R_j
+ * + sqrt(1 - n.n_j)) above 1 / a that are not SampleInFront, summed in
+ * insertion order (0 when no value counts); d_wsum[i] (may be NULL) = the
+ * weight sum, which ENCORE tests before using the cache; d_count[i] (may be
+ * NULL) = the values counted.  The caller applies the hit's diffuse colour
+ * and the 0.35 scale (PhotonMapperCPU.cpp:1103).  Bit for bit the reference's
+ * result.  The first lookup after rq_irrcache_add builds the cache's index
+ * (host, synchronous). */
+rq_status rq_irrcache_query(rq_irrcache* cache, const float* d_qpos, const float* d_qnrm, uint32_t nq,
+                            float* d_rgb, float* d_wsum, uint32_t* d_count, void* stream);
+/* The lookup work of rq_irrcache_query, counted by an instrumented copy of
+ * its kernel (synchronous): counts[0] = lookups, counts[1] = cached values
+ * examined, counts[2] = values counted; d_rgb receives the results.  No
+ * counterpart in ENCORE (measurement). */
+rq_status rq_irrcache_counts(rq_irrcache* cache, const float* d_qpos, const float* d_qnrm, uint32_t nq,
+                             float* d_rgb, uint64_t* counts, void* stream);
+/* Values stored; of them, the ones every lookup checks (unbounded reach);
+ * cells and cell references of the index (after a lookup).  Any may be NULL. */
+rq_status rq_irrcache_info(const rq_irrcache* cache, uint32_t* count, uint32_t* global_count,
+                           uint64_t* index_cells, uint64_t* index_refs);
+void rq_irrcache_destroy(rq_irrcache* cache);
+
+/* ---------------------------------------------------------------------
  * Photon emission: PhotonMapperCPU::EmitPhotons and TracePhoton
  * (PhotonMapperCPU.cpp:503-775) on the device.
  * --------------------------------------------------------------------- */

@@ -58,6 +58,11 @@ def lib():
                                    _u32, _u32p]),
             "orc_transmit_ray": (_int, [_f32p, _f32p, _f32p, _int, _f32p, _f32p]),
             "orc_kd_textures": (None, [_vp, _f32p, _f32p, _f32p, _f32p, _u32p]),
+            "orc_ic_create": (_vp, [ctypes.c_float, _f32p, _f32p]),
+            "orc_ic_free": (None, [_vp]),
+            "orc_ic_add": (None, [_vp, _f32p, _f32p, _f32p, _f32p, _u32, _u32p]),
+            "orc_ic_size": (_u32, [_vp]),
+            "orc_ic_query": (None, [_vp, _f32p, _f32p, _u32, _f32p, _f32p, _u32p, _int]),
             "orc_emit_photons": (None, [_vp, _f32p, _u32p, _f32p, _u32, _u32, _u32p, _f32p, ctypes.c_uint64,
                                         ctypes.c_uint64, _vp, _u32p, ctypes.POINTER(ctypes.c_uint64)]),
         }
@@ -272,6 +277,42 @@ class PhotonMap:
         lib().orc_pm_radiance(self.h, qpos, qnrm, qpos.shape[0], k, float(max_dist), int(caustic), out,
                               nthreads or threads())
         return out
+
+
+class IrrCache:
+    """ENCORE's irradiance cache (IrrCacheOctree.h) restated: IrrCache(a,
+    minPoint, size) with AddElement and the final gather's weighted lookup
+    (PhotonMapperCPU.cpp:1087-1103)."""
+
+    def __init__(self, a, min_point, size):
+        self.h = lib().orc_ic_create(float(a), _c32(min_point), _c32(size))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_ic_free(self.h)
+            self.h = None
+
+    def add(self, value, pos, nrm, ri):
+        """Returns per element 0 stored, 1 outside the root box, 2 refused
+        (the reference's AddElement would never return)."""
+        value, pos, nrm = (_c32(x).reshape(-1, 3) for x in (value, pos, nrm))
+        ri = _c32(ri).reshape(-1)
+        st = np.zeros(len(ri), np.uint32)
+        lib().orc_ic_add(self.h, value, pos, nrm, ri, len(ri), st)
+        return st
+
+    def __len__(self):
+        return int(lib().orc_ic_size(self.h))
+
+    def query(self, qpos, qnrm, nthreads=None):
+        """(rgb (n,3), weight sum (n,), count (n,)) per query point."""
+        qpos, qnrm = _c32(qpos).reshape(-1, 3), _c32(qnrm).reshape(-1, 3)
+        n = len(qpos)
+        rgb = np.zeros((n, 3), np.float32)
+        ws = np.zeros(n, np.float32)
+        cnt = np.zeros(n, np.uint32)
+        lib().orc_ic_query(self.h, qpos, qnrm, n, rgb, ws, cnt, nthreads or threads())
+        return rgb, ws, cnt
 
 
 def hit_attributes(xyz, idx, hits):

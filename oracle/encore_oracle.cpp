@@ -1438,3 +1438,146 @@ extern "C" void orc_kd_textures(void* h, float* cell, float* v0, float* v1, floa
         }
     }
 }
+
+// ---------------------------------------------------------------------------
+// Irradiance cache (IrrCacheOctree.h:92-326) and the final gather's use of it
+// (PhotonMapperCPU.cpp:1087-1103, 1170-1173).
+//
+// AddElement (IrrCacheOctree.h:212-243) drops an element outside the root
+// box and stores it in the root's list when 2 * (a * Ri) <= the root's x
+// size.  Otherwise it descends into a child of half the size, where the same
+// test fails again, and never stops: in any run of the reference that
+// completes, every element sits in the root's list, in insertion order, and
+// the root stays a leaf.  Such an element is refused here (status 2).
+// GetIrrValuesNearPoint (:294-326) then walks that list: an element counts
+// when GetWeight (:55-77, Jensen eq. 11.5) exceeds 1 / a and SampleInFront
+// (:279-290) is false; the caller sums value * weight and the weights in list
+// order and divides (PhotonMapperCPU.cpp:1093-1101).  Every expression keeps
+// the reference's float/double types and operation order.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct IrrElem {
+    F3 value, pos, nrm;
+    float ri;
+};
+
+struct IrrCacheO {
+    float a;
+    F3 minp, size;
+    std::vector<IrrElem> root;
+};
+
+inline float mag3(const F3& v) {  // Vector3::Magnitude (Vector3.cpp:116-124)
+    const float m2 = (v.c[0] * v.c[0]) + (v.c[1] * v.c[1]) + (v.c[2] * v.c[2]);
+    return std::sqrt(m2);
+}
+
+// TCacheElement::GetWeight (IrrCacheOctree.h:55-77).
+inline float irr_weight(const IrrElem& e, const F3& p, const F3& n) {
+    const float distance = mag3(sub(e.pos, p));
+    const float distCalc = (e.ri == 0) ? 0.0f : distance / e.ri;
+    const bool same = n.c[0] == e.nrm.c[0] && n.c[1] == e.nrm.c[1] && n.c[2] == e.nrm.c[2];
+    const float sqrtCalc = same ? 0.0f : (float)std::sqrt(1.0 - (double)dot3(n, e.nrm));
+    const float denom = sqrtCalc + distCalc;
+    return denom == 0 ? 0.0f : 1 / denom;
+}
+
+// SampleInFront (IrrCacheOctree.h:279-290): navg = sampleN + n, pVec =
+// Normalize(p - sampleP) with the flush, Dot(pVec, navg) < -0.1f.
+inline bool irr_in_front(const F3& p, const F3& sp, const F3& n, const F3& sn) {
+    F3 navg;
+    for (int k = 0; k < 3; ++k) navg.c[k] = sn.c[k] + n.c[k];
+    F3 v = sub(p, sp);
+    const float m = mag3(v);
+    for (int k = 0; k < 3; ++k) {
+        v.c[k] = v.c[k] / m;
+        flush_small(v.c[k]);
+    }
+    return dot3(v, navg) < -0.1f;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* orc_ic_create(float a, const float* minp, const float* size) {
+    IrrCacheO* c = new IrrCacheO();
+    c->a = a;
+    for (int k = 0; k < 3; ++k) {
+        c->minp.c[k] = minp[k];
+        c->size.c[k] = size[k];
+    }
+    return c;
+}
+
+void orc_ic_free(void* h) { delete static_cast<IrrCacheO*>(h); }
+
+// status[i]: 0 stored, 1 outside the root box (dropped), 2 refused (the
+// reference's AddElement would not return).
+void orc_ic_add(void* h, const float* value, const float* pos, const float* nrm, const float* ri, uint32_t n,
+                uint32_t* status) {
+    IrrCacheO* c = static_cast<IrrCacheO*>(h);
+    for (uint32_t i = 0; i < n; ++i) {
+        IrrElem e;
+        for (int k = 0; k < 3; ++k) {
+            e.value.c[k] = value[3 * (size_t)i + k];
+            e.pos.c[k] = pos[3 * (size_t)i + k];
+            e.nrm.c[k] = nrm[3 * (size_t)i + k];
+        }
+        e.ri = ri[i];
+        F3 mx;
+        for (int k = 0; k < 3; ++k) mx.c[k] = c->minp.c[k] + c->size.c[k];
+        bool out = false;
+        for (int k = 0; k < 3; ++k) out = out || e.pos.c[k] < c->minp.c[k] || mx.c[k] < e.pos.c[k];
+        if (out) {
+            status[i] = 1;
+            continue;
+        }
+        const float domain = c->a * e.ri;
+        if (!(2 * domain <= c->size.c[0])) {
+            status[i] = 2;
+            continue;
+        }
+        c->root.push_back(e);
+        status[i] = 0;
+    }
+}
+
+uint32_t orc_ic_size(void* h) { return (uint32_t)static_cast<IrrCacheO*>(h)->root.size(); }
+
+// Per query: rgb = (sum of value * weight) / weightSum over the counted
+// elements in list order (0 when weightSum is 0), wsum = weightSum, count.
+void orc_ic_query(void* h, const float* qpos, const float* qnrm, uint32_t nq, float* rgb, float* wsum,
+                  uint32_t* count, int nthreads) {
+    const IrrCacheO* c = static_cast<const IrrCacheO*>(h);
+    const double tolerance = 1 / c->a;  // IrrCacheOctree.h:300 (float division, widened)
+    parallel_chunks(nq, nthreads, 64, [&](uint32_t b, uint32_t e) {
+        for (uint32_t q = b; q < e; ++q) {
+            F3 p, n;
+            for (int k = 0; k < 3; ++k) {
+                p.c[k] = qpos[3 * (size_t)q + k];
+                n.c[k] = qnrm[3 * (size_t)q + k];
+            }
+            float ws = 0, acc[3] = {0, 0, 0};
+            uint32_t cnt = 0;
+            for (const IrrElem& el : c->root) {
+                const float w = irr_weight(el, p, n);
+                if (w > tolerance && !irr_in_front(p, el.pos, n, el.nrm)) {
+                    ws += w;
+                    for (int k = 0; k < 3; ++k) acc[k] += el.value.c[k] * w;
+                    ++cnt;
+                }
+            }
+            if (ws)
+                for (int k = 0; k < 3; ++k) acc[k] /= ws;
+            else
+                acc[0] = acc[1] = acc[2] = 0;
+            for (int k = 0; k < 3; ++k) rgb[3 * (size_t)q + k] = acc[k];
+            wsum[q] = ws;
+            count[q] = cnt;
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+"""The irradiance cache on the GPU (rq_irrcache_*) gives the oracle's
+lookups bit for bit (GetIrrValuesNearPoint and the final gather's weighted
+average, IrrCacheOctree.h:294-326, PhotonMapperCPU.cpp:1087-1103): closed-form
+cases, a realistic cache on the Cornell box with values of unbounded reach
+(Ri = 0) mixed in, lookups outside the index, values added between lookups,
+and a refused value that leaves the cache unchanged."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from encore_amd import rq, scenes
+from helpers import log
+
+pytestmark = pytest.mark.gpu
+
+A = 0.08
+LO, SIZE = (-75.0, -75.0, -75.0), (150.0, 150.0, 150.0)
+
+
+def _query(c, q, n):
+    dq = torch.from_numpy(np.ascontiguousarray(q, np.float32)).cuda()
+    dn = torch.from_numpy(np.ascontiguousarray(n, np.float32)).cuda()
+    rgb = torch.full((len(q), 3), -1.0, device="cuda")
+    ws = torch.full((len(q),), -1.0, device="cuda")
+    cnt = torch.full((len(q),), -1, dtype=torch.int32, device="cuda")
+    c.query(dq, dn, rgb, ws, cnt)
+    torch.cuda.synchronize()
+    return rgb.cpu().numpy(), ws.cpu().numpy(), cnt.cpu().numpy().view(np.uint32)
+
+
+def _same(got, ref):
+    for g, r in zip(got, ref):
+        assert np.array_equal(np.ascontiguousarray(g).view(np.uint32), np.ascontiguousarray(r).view(np.uint32))
+
+
+def _cornell_points(n, seed):
+    """Hit points and hit normals of random eye rays into the Cornell box
+    (scale 1: walls at +-50)."""
+    xyz, idx, _ = scenes.config1(res=8)
+    rng = np.random.default_rng(seed)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = rng.uniform(-40, 40, (n, 3))
+    d = rng.normal(0, 1, (n, 3))
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 7] = np.inf
+    h = oracle.KdTree(xyz, idx).intersect(rays)
+    ok, p, nrm = scenes.hit_points(xyz, idx, h)
+    return p.astype(np.float32), nrm.astype(np.float32)
+
+
+def test_irrcache_closed_forms(device):
+    up = (0.0, 1.0, 0.0)
+    c = rq.IrrCache(device, A, LO, SIZE)
+    o = oracle.IrrCache(A, LO, SIZE)
+    elems = [((1, 2, 3), (0, 0, 0), up, 16.0), ((1e8,) * 3, (10, 0, 0), up, 16.0), ((-1e8,) * 3, (12, 0, 0), up, 16.0),
+             ((1,) * 3, (11, 0, 1), up, 16.0), ((5, 5, 5), (30, 30, 30), up, 0.0)]
+    args = [np.float32([e[k] for e in elems]) for k in range(4)]
+    assert c.add(*args).tolist() == o.add(*args).tolist() == [0] * 5
+    # the last two lie outside the index (its grid covers the bounded values'
+    # reach only): the Ri = 0 value reaches them through the global list
+    q = np.float32([[1, 0, 0], [2, 0, 0], [1.27, 0, 0], [1.28, 0, 0], [0, -1, 0], [11, 0, 0], [-40, 40, 40],
+                    [200, 60, 30]])
+    n = np.tile(np.float32(up), (len(q), 1))
+    n[6] = (0, 0.99995, 0.01)
+    n[7] = (0, 0.99995, 0.01)
+    got, ref = _query(c, q, n), o.query(q, n)
+    log("closed forms: counts", got[2].tolist())
+    _same(got, ref)
+    assert got[2].tolist() == [1, 0, 1, 0, 0, 3, 1, 1]
+    c.destroy()
+
+
+def test_irrcache_matches_oracle(device):
+    rng = np.random.default_rng(21)
+    p, nrm = _cornell_points(12000, 5)
+    p, nrm = p[:8192], nrm[:8192]
+    ri = np.exp(rng.uniform(np.log(2.0), np.log(60.0), len(p))).astype(np.float32)
+    ri[::97] = 0.0  # unbounded reach: every lookup checks these
+    val = rng.uniform(0, 2, (len(p), 3)).astype(np.float32)
+    c = rq.IrrCache(device, A, LO, SIZE)
+    o = oracle.IrrCache(A, LO, SIZE)
+    half = len(p) // 2
+    # values added in two calls with a lookup in between (the index is rebuilt)
+    for lo, hi in ((0, half), (half, len(p))):
+        st = c.add(val[lo:hi], p[lo:hi], nrm[lo:hi], ri[lo:hi])
+        assert np.array_equal(st, o.add(val[lo:hi], p[lo:hi], nrm[lo:hi], ri[lo:hi]))
+        qp, qn = _cornell_points(45000, 9 + lo)
+        qp, qn = qp[:32768], qn[:32768]
+        # perturb half the lookups off the surface and flip a few normals
+        qp[::2] += rng.normal(0, 0.5, (len(qp[::2]), 3)).astype(np.float32)
+        qn[::11] = -qn[::11]
+        got, ref = _query(c, qp, qn), o.query(qp, qn)
+        info = c.info()
+        log("cache %d values (%d global), index %d cells %d refs; lookups with values %d, mean count %.2f" % (
+            info["count"], info["global_count"], info["index_cells"], info["index_refs"], (got[2] > 0).sum(),
+            got[2].mean()))
+        _same(got, ref)
+        assert (got[2] > 0).mean() > 0.3 and info["global_count"] >= len(p[:hi]) // 97
+    c.destroy()
+
+
+def test_irrcache_refusal_leaves_cache_unchanged(device):
+    c = rq.IrrCache(device, A, LO, SIZE)
+    up = np.float32([[0, 1, 0]])
+    c.add(np.float32([[1, 1, 1]]), np.float32([[0, 0, 0]]), up, np.float32([16.0]))
+    with pytest.raises(rq.RQError) as e:
+        c.add(np.float32([[1, 1, 1], [2, 2, 2]]), np.float32([[1, 0, 0], [2, 0, 0]]), np.tile(up, (2, 1)),
+              np.float32([16.0, 1000.0]))
+    assert e.value.status == 4
+    assert c.info()["count"] == 1
+    got = _query(c, np.float32([[1, 0, 0]]), up)
+    assert got[2].tolist() == [1] and got[0].tolist() == [[1, 1, 1]]
+    # lookups of an empty cache and with no lookups at all
+    c2 = rq.IrrCache(device, A, LO, SIZE)
+    got = _query(c2, np.float32([[1, 0, 0]]), up)
+    assert got[2].tolist() == [0] and got[1].tolist() == [0.0]
+    c2.query(torch.empty((0, 3), device="cuda"), torch.empty((0, 3), device="cuda"), torch.empty((0, 3), device="cuda"))
+    c.destroy()
+    c2.destroy()
