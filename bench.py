@@ -1045,6 +1045,12 @@ def run_workload(args):
                          "build_s": round(build, 2)}
             if not tree:
                 out[name]["first_call_ms_incl_grid_build"] = round(first_ms, 2)
+                # the gather's work, counted by the kernel's instrumented copy
+                # (no lists: the same cells, photons and candidates)
+                rgb_c = torch.empty_like(drgb)
+                work = pm.radiance_counts(dq, dqn, k, r, rgb_c)
+                torch.cuda.synchronize()
+                out[name]["counts_rgb_equal"] = bool(torch.equal(rgb_c.view(torch.int32), drgb.view(torch.int32)))
             log(name, out[name])
             pm.destroy()
         os.environ.pop("RQ_KNN_CELLS", None)
@@ -1074,12 +1080,28 @@ def run_workload(args):
         parity = {"sample": int(len(sq)), "count_equal": same_cnt, "set_equal_tie_aware": same_set,
                   "radiance_max_rel": float(rel.max()), "mean_count": float(gc.mean())}
         log("parity", parity)
+        # Algorithmic bytes of one call: per query 24 B in (position, normal),
+        # 8 B per grid cell visited (its photon range), 20 B per photon read
+        # (position and class 16, slot 4), 28 B per photon gathered (power 16
+        # and insertion index 4 read, index and distance 4 + 4 written) and
+        # 16 B out (radiance, count).
+        nqf = float(len(Q))
+        gathered = float(dc.to(torch.int64).sum().item())
+        alg = 40.0 * nqf + 8.0 * work["cells"] + 20.0 * work["photons"] + 28.0 * gathered
+        per_q = {"cells_visited": round(work["cells"] / nqf, 2), "photons_read": round(work["photons"] / nqf, 1),
+                 "candidates": round(work["candidates"] / nqf, 1), "gathered": round(gathered / nqf, 2),
+                 "bytes": round(alg / nqf, 1)}
+        log("per query", per_q)
+        achieved = alg / (out["cells"]["launch_ms"] / 1e3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "knn_cell_prep + knn_cell_kernel (one call, events around both)"}
         _line("Mqueries/s photon gather (k=50, r=0.05)", out["cells"]["Mqueries_s"], "Mqueries/s",
               out["cells"]["launch_ms"],
               {"workload": "radiance estimate at %d gather points, %d photons (3 bounces, albedo 0.7) on the "
                            "Cornell box at 1/20 scale tessellated to %d triangles" % (len(Q), len(P), len(idx)),
                "k": k, "max_dist": r},
-              paths=out, parity=parity,
+              paths=out, parity=parity, per_query=per_q, roofline=roof,
               cpu_baseline={"value": round(len(sq) / cpu_s / 1e6, 5), "unit": "Mqueries/s", "cores": thr,
                             "kind": "port", "sample": "%d gather points" % len(sq)})
         return 0

@@ -597,6 +597,10 @@ constexpr int kCellCand = 2 * (int)kKnnMaxK + 64;
 // waves with __launch_bounds__ makes the compiler spill to scratch instead.
 constexpr int kCellWaves = 4;
 
+// COUNT: the instrumented copy tallies, per query, the cells visited, the
+// photons read and the candidates (L.tally[0..3]: queries, cells, photons,
+// candidates).
+template <bool COUNT>
 __global__ __launch_bounds__(64 * kCellWaves) void knn_cell_kernel(KnnLaunch L) {
     __shared__ uint32_t s_key_w[kCellWaves][kCellCand];
     __shared__ uint32_t s_slot_w[kCellWaves][kCellCand];
@@ -741,8 +745,10 @@ __global__ __launch_bounds__(64 * kCellWaves) void knn_cell_kernel(KnnLaunch L) 
                 return __shfl(ci, 0, 64);
             };
             uint32_t c0 = total ? fetch(0, p, sl) : 0u;  // the cell of the chunk's first element
+            uint32_t n_read = 0, n_cand = 0;                // COUNT only
             for (uint32_t e0 = 0; e0 < total; e0 += 64) {
                 if (s_near[c0] >= r2) break;  // this cell and all later ones are too far
+                if (COUNT) n_read += min(64u, total - e0);
                 float4 pn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 uint32_t sn = 0;
                 uint32_t c0n = c0;
@@ -761,6 +767,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void knn_cell_kernel(KnnLaunch L) 
                     s_slot[pos] = sl;
                 }
                 cand += (uint32_t)__popcll(cm);
+                if (COUNT) n_cand += (uint32_t)__popcll(cm);
                 __builtin_amdgcn_wave_barrier();
                 if (cand >= compact_at) {
                     uint32_t need = k;
@@ -772,7 +779,13 @@ __global__ __launch_bounds__(64 * kCellWaves) void knn_cell_kernel(KnnLaunch L) 
                 sl = sn;
                 c0 = c0n;
             }
+            if (COUNT && lane == 0) {
+                atomicAdd(&L.tally[1], (unsigned long long)ncell);
+                atomicAdd(&L.tally[2], (unsigned long long)n_read);
+                atomicAdd(&L.tally[3], (unsigned long long)n_cand);
+            }
         }
+        if (COUNT && lane == 0) atomicAdd(&L.tally[0], 1ull);
         wave_finish(L, q, cand, false, s_key, s_slot, s_hist);
         __builtin_amdgcn_wave_barrier();
     }
@@ -853,7 +866,7 @@ void launch_knn_wave(const KnnLaunch& L, int grid, uint32_t stack_cap, hipStream
 
 int knn_cell_grid(uint32_t nq, int num_cus) {
     int per_cu = 0;
-    RQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)dev::knn_cell_kernel,
+    RQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)dev::knn_cell_kernel<false>,
                                                         dev::kKnnWave * dev::kCellWaves, 0));
     // Five waves per SIMD (20 per CU): with one-wave workgroups the 4M-query
     // final gather ran 160 M rays/s at 5 waves per SIMD, 146-149 M when 20
@@ -873,7 +886,10 @@ void launch_knn_cells(const KnnLaunch& L, float4* prep, int grid, hipStream_t s)
     RQ_HIP(hipGetLastError());
     KnnLaunch C = L;
     C.qprep = prep;
-    hipLaunchKernelGGL(dev::knn_cell_kernel, dim3(grid), dim3(dev::kKnnWave * dev::kCellWaves), 0, s, C);
+    if (C.tally)
+        hipLaunchKernelGGL(dev::knn_cell_kernel<true>, dim3(grid), dim3(dev::kKnnWave * dev::kCellWaves), 0, s, C);
+    else
+        hipLaunchKernelGGL(dev::knn_cell_kernel<false>, dim3(grid), dim3(dev::kKnnWave * dev::kCellWaves), 0, s, C);
     RQ_HIP(hipGetLastError());
 }
 

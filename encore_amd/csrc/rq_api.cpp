@@ -670,8 +670,11 @@ static bool knn_cells_sorted() { return !env_is("RQ_KNN_SORT", '0'); }
 // Queries of a cell-grid gather are sorted by cell from this many on.
 constexpr uint32_t kKnnSortMin = 16384;
 
+// tally (device, 4 words, zeroed): counts of the cell-grid gather's
+// instrumented kernel; left zero when another kernel serves the call.
 static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint32_t nq, uint32_t k, float max_dist,
-                       uint32_t* idx, float* d2, uint32_t* cnt, float* rgb, uint32_t caustic, hipStream_t s) {
+                       uint32_t* idx, float* d2, uint32_t* cnt, float* rgb, uint32_t caustic, hipStream_t s,
+                       unsigned long long* tally = nullptr) {
     require(p != nullptr, "null photon map");
     require(k >= 1 && k <= kKnnMaxK, "k must be 1..256");
     require(nq == 0 || (qpos && qnrm), "null query arrays");
@@ -704,6 +707,7 @@ static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint
             L.out_rgb = rgb;
             L.caustic = caustic;
             L.grid = pg->view;
+            L.tally = tally;
             const bool sorted = nq >= kKnnSortMin && knn_cells_sorted();
             const size_t prep_bytes = ((size_t)nq * 16 + 255) & ~(size_t)255;
             const size_t sort_bytes = sorted ? knn_cell_order_bytes(nq) : 0;
@@ -720,6 +724,7 @@ static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint
             return;
         }
     }
+    if (tally) throw Error(RQ_ERR_UNSUPPORTED, "counts come from the cell-grid gather, which does not serve this call");
     const int grid = wave ? knn_wave_grid(nq, dev->num_cus) : knn_grid(nq, dev->num_cus, k);
     const uint32_t depth = knn_spill_words_per_lane(p->n);
     const uint32_t cap = knn_wave_stack_cap(p->n);
@@ -767,6 +772,24 @@ rq_status rq_radiance_estimate(rq_photons* p, const float* d_qpos, const float* 
         require((d_idx == nullptr) == (d_d2 == nullptr), "d_idx and d_d2 go together");
         knn_common(p, d_qpos, d_qnormal, nq, k, max_dist, d_idx, d_d2, d_count, d_rgb, caustic ? 1u : 0u,
                    (hipStream_t)stream);
+    });
+}
+
+rq_status rq_radiance_counts(rq_photons* p, const float* d_qpos, const float* d_qnormal, uint32_t nq, uint32_t k,
+                             float max_dist, uint32_t caustic, float* d_rgb, uint64_t* counts, void* stream) {
+    return guarded([&] {
+        require(counts != nullptr, "null counts");
+        require(nq == 0 || d_rgb, "null radiance output");
+        counts[0] = counts[1] = counts[2] = counts[3] = 0;
+        if (nq == 0) return;
+        require(p != nullptr, "null photon map");
+        RQ_HIP(hipSetDevice(p->dev->ordinal));
+        DevBuf t(4 * sizeof(unsigned long long));
+        launch_zero_u32(t.as<uint32_t>(), 8, (hipStream_t)stream);
+        knn_common(p, d_qpos, d_qnormal, nq, k, max_dist, nullptr, nullptr, nullptr, d_rgb, caustic ? 1u : 0u,
+                   (hipStream_t)stream, t.as<unsigned long long>());
+        RQ_HIP(hipMemcpyAsync(counts, t.ptr, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, (hipStream_t)stream));
+        RQ_HIP(hipStreamSynchronize((hipStream_t)stream));
     });
 }
 

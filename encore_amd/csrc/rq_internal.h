@@ -255,6 +255,7 @@ struct KnnLaunch {
     PhotonGridView grid;    // knn_cell_kernel only
     const uint32_t* order;  // knn_cell_kernel: queries in this order (null: 0..nq-1)
     const float4* qprep;    // knn_cell_kernel: position and class per served query
+    unsigned long long* tally;  // knn_cell_kernel counting copy: queries, cells, photons read, candidates
 };
 
 void launch_knn(const KnnLaunch& L, int grid, hipStream_t s);

@@ -147,6 +147,8 @@ def load():
     L.rq_photons_balance_host.argtypes = [_vp, _u32, _u32, _vp, _vp]
     L.rq_knn.argtypes = [_vp, _vp, _vp, _u32, _u32, ctypes.c_float, _vp, _vp, _vp, _vp]
     L.rq_radiance_estimate.argtypes = [_vp, _vp, _vp, _u32, _u32, ctypes.c_float, _u32, _vp, _vp, _vp, _vp, _vp]
+    L.rq_radiance_counts.argtypes = [_vp, _vp, _vp, _u32, _u32, ctypes.c_float, _u32, _vp, _vp, _vp]
+    L.rq_radiance_counts.restype = ctypes.c_int
     if hasattr(L, "rq_scene_save"):
         L.rq_scene_save.argtypes = [_vp, ctypes.c_char_p]
         L.rq_scene_save.restype = ctypes.c_int
@@ -569,6 +571,14 @@ class PhotonMap:
         check(load().rq_radiance_estimate(self.h, _ptr(qpos), _ptr(qnrm), qpos.shape[0], k, float(max_dist),
                                           int(caustic), _ptr(rgb), _ptr(idx), _ptr(d2), _ptr(count),
                                           _stream(stream)))
+
+    def radiance_counts(self, qpos, qnrm, k, max_dist, rgb, caustic=False, stream=None):
+        """Radiance estimates into rgb, with the gather's work counted:
+        {"queries", "cells", "photons", "candidates"} (cell-grid gather only)."""
+        out = np.zeros(4, np.uint64)
+        check(load().rq_radiance_counts(self.h, _ptr(qpos), _ptr(qnrm), qpos.shape[0], k, float(max_dist),
+                                        int(caustic), _ptr(rgb), out.ctypes.data, _stream(stream)))
+        return dict(zip(("queries", "cells", "photons", "candidates"), (int(v) for v in out)))
 
 
 class IrrCache:

@@ -98,7 +98,7 @@ def cornell(seed=SEED_BASE + 1, scale=1.0):
     is U(20,35); the 1x2x1 tall box is U(50,70) high; rotations about +y are
     U(-45,45) degrees; centres are U(-30,30) in x and z.  Geometry follows
     environment/scenes/diffuseCornell.txt:5-30.  `scale` multiplies every
-    coordinate (1/100 gives the unit-size box used for photon mapping)."""
+    coordinate (config4_scene's photon-mapping box uses 1/20: 5 units wide)."""
     st = Stream(seed)
     h = 50.0
     walls = [
@@ -552,9 +552,12 @@ def tessellate(xyz, idx, f):
 def config4_scene(scale=1.0 / 20.0, tess=32, seed=SEED_BASE + 1):
     """The Cornell box of config 1 at world scale `scale` with each triangle
     split tess x tess (34 * 1024 = 34,816 triangles).  At 1/20 the box is
-    5 units wide and a 0.05 gather radius holds about a hundred photons,
-    while tessellated edges (0.04-0.3) mostly stay clear of Cross()'s 1e-3
-    flush, so faces keep well-defined normals."""
+    5 units wide; with 2,000,000 photons a 0.05 gather radius reaches tens
+    to a hundred of them, and the tessellated edges (0.04-0.3) keep the
+    products of two edges mostly clear of Cross()'s 1e-3 flush
+    (Vector3.h:148-160), so faces keep well-defined normals.  At 1/100 the walls
+    edges would be about 0.03 long, their products about 1e-3: normals
+    would flush to zero or to one axis."""
     xyz, idx = cornell(seed, scale=scale)
     return tessellate(xyz, idx, tess)
 

@@ -364,6 +364,16 @@ rq_status rq_knn(rq_photons* photons, const float* d_qpos, const float* d_qnorma
 rq_status rq_radiance_estimate(rq_photons* photons, const float* d_qpos, const float* d_qnormal, uint32_t nq,
                                uint32_t k, float max_dist, uint32_t caustic, float* d_rgb, uint32_t* d_idx,
                                float* d_d2, uint32_t* d_count, void* stream);
+/* The gather work of rq_radiance_estimate (no lists), counted by an
+ * instrumented copy of the photon cell-grid kernel (synchronous):
+ * counts[0] = queries, counts[1] = grid cells visited, counts[2] = photons
+ * read, counts[3] = photons within the radius at the time they were read
+ * (candidates); d_rgb receives the estimates.  RQ_ERR_UNSUPPORTED when k or
+ * the radius sends the call to another kernel.  No counterpart in ENCORE
+ * (measurement). */
+rq_status rq_radiance_counts(rq_photons* photons, const float* d_qpos, const float* d_qnormal, uint32_t nq,
+                             uint32_t k, float max_dist, uint32_t caustic, float* d_rgb, uint64_t* counts,
+                             void* stream);
 
 /* Final gather: PhotonMapperCPU::CalculateIndirectIllumination's
  * final-gather branch (PhotonMapperCPU.cpp:1110-1167) for npoints shading

@@ -196,3 +196,31 @@ def test_cells_degenerate_query_positions(device):
     gi, gd, gc = _gpu_knn(pm, Qm, QNm, 64, 0.04)
     assert np.array_equal(gc, cnt) and np.all(gi[100:106] == 0xFFFFFFFF)
     pm.destroy()
+
+
+def test_cells_counts(device):
+    """rq_radiance_counts: the instrumented kernel returns the same
+    estimates, one query each, and -- with fewer than k photons in range, so
+    the candidate list is never cut -- exactly the brute-force number of
+    photons in range as candidates, out of at least as many photons read
+    from at most 64 cells per query.  A k or radius the grid does not serve
+    is refused."""
+    rng = np.random.default_rng(13)
+    P, N, W, Q, QN = _surface_photons(rng, 40000, 0.0)
+    pm = rq.PhotonMap(device, P, N, W)
+    k, r = 256, 0.03
+    rgb, cnt = _radiance(pm, Q, QN, k, r, False, False)
+    q = torch.from_numpy(Q).cuda()
+    qn = torch.from_numpy(QN).cuda()
+    rgb_c = torch.full((len(Q), 3), -1.0, device="cuda")
+    work = pm.radiance_counts(q, qn, k, r, rgb_c)
+    want = _brute_count(P, N, Q, QN, r)
+    log("case: cell counts %s, in range %d" % (work, want.sum()))
+    assert np.array_equal(rgb_c.cpu().numpy().view(np.uint32), rgb.view(np.uint32))
+    assert work["queries"] == len(Q)
+    assert work["candidates"] == int(want.sum()) == int(cnt.sum())
+    assert work["photons"] >= work["candidates"] and 0 < work["cells"] <= 64 * len(Q)
+    with pytest.raises(rq.RQError) as e:
+        pm.radiance_counts(q, qn, 8, r, rgb_c)
+    assert e.value.status == 4
+    pm.destroy()
