@@ -683,11 +683,17 @@ def _irrcache_workload(args, dev, thr, trace_fn):
     wall, ev = _time_gpu(lambda: cache.query(dq, dqn, rgb, ws, cnt), args.steps, args.warmup)
     info = cache.info()
     n_look, n_exam, n_cnt = cache.counts(dq, dqn, torch.empty_like(rgb))
-    # Algorithmic bytes per lookup: query position and normal (24), outputs
-    # (rgb, weight sum, count: 20), the cell's range (8), and per cached value
-    # examined its index (4) and record (48: position and Ri, normal, value).
-    alg = 52.0 * n_look + 52.0 * n_exam
+    # HBM bytes of one call: per lookup its position and normal (24), outputs
+    # (rgb, weight sum, count: 20) and its cell's range (8); the cache and its
+    # index once (48 per value, 4 per cell reference, 4 per cell start and
+    # per global value).  The cache (16K values, 1.4 MB with its index) stays
+    # in L2, so the per-lookup reads of cached records -- 4 (index) + 32
+    # (position and Ri, normal) per value examined, 16 (value) per value
+    # counted -- are reported beside it as L2-served bytes.
+    alg = 52.0 * n_look + 48.0 * info["count"] + 4.0 * (info["index_refs"] + info["index_cells"] + 1 +
+                                                         info["global_count"])
     achieved = alg / (ev / 1e3) / 1e9
+    rec = 36.0 * n_exam + 16.0 * n_cnt
     log("irrcache: %d values (%d global), index %d cells; %.2f values examined and %.2f counted per lookup; "
         "%.3f ms per call (first %.1f ms)" % (info["count"], info["global_count"], info["index_cells"],
                                              n_exam / n_look, n_cnt / n_look, ev, first_ms))
@@ -712,7 +718,10 @@ def _irrcache_workload(args, dev, thr, trace_fn):
                            len(Q), len(idx), info["count"])},
           roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "alg_bytes_per_lookup": round(alg / n_look, 1),
+                    "hbm_bytes_per_lookup": round(alg / n_look, 1),
+                    "l2_record_bytes_per_lookup": round(rec / n_look, 1),
+                    "l2_record_GBps": round(rec / (ev / 1e3) / 1e9, 1),
+                    "value_tests_per_s": float("%.4g" % (n_exam / (ev / 1e3))),
                     "values_examined_per_lookup": round(n_exam / n_look, 3),
                     "values_counted_per_lookup": round(n_cnt / n_look, 3), "kernel": "ic_query"},
           index=info, first_call_ms_incl_index=round(first_ms, 2), parity=parity,
@@ -1008,26 +1017,55 @@ def run_workload(args):
     if args.workload == "refit":
         return _refit_workload(args, dev, thr)
     if args.workload == "c4":
-        xyz, idx = scenes.config4_scene()
-        sc = rq.Scene(dev, xyz, idx, accel="bvh")
-        t = time.time()
-        P, N, W = scenes.trace_photons(xyz, idx, trace_fn(sc), 2000000, batch=1 << 20)
-        log("photons: %d traced in %.1fs" % (len(P), time.time() - t))
-        Q, QN = scenes.photon_queries(xyz, idx, trace_fn(sc), res=1000)
-        log("queries: %d" % len(Q))
-        k, r = 50, 0.05
+        return _photon_gather_workload(args, dev, thr, trace_fn)
+
+    # ---- c5: serving ---------------------------------------------------
+    res = _serving(args, dev, trace_fn)
+    _line("served Mrays/s at equal p99 (serving, 16 resident scenes)", res["served_at_equal_p99"]["Mrays_s"]
+          if res["served_at_equal_p99"] else 0.0, "Mrays/s", 0.0, res.pop("config"), **res)
+    return 0
+
+
+def _photon_gather_workload(args, dev, thr, trace_fn):
+    """Config 4: radiance estimates with neighbour lists (GetNearestNPhotons
+    and GetRadianceEstimate, PhotonMap.cpp:76-164, 224-297) at the ~1M hit
+    points of a 1000x1000 view, k = 50, over 2,000,000 photons traced into
+    the Cornell box tessellated to 34,816 triangles.  The box is built at
+    1/20 scale (at 1/100 its 0.03-unit edges would flush Cross()'s normals);
+    the headline radius 0.25 is the 1/100 box's r = 0.05 scaled by 5 -- the
+    same photons per gather disk, ~10^3 candidates per query -- and r = 0.05
+    at 1/20 is reported beside it with the balanced-tree walks."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from encore_amd import rq, scenes
+
+    xyz, idx = scenes.config4_scene()
+    sc = rq.Scene(dev, xyz, idx, accel="bvh")
+    t = time.time()
+    P, N, W = scenes.trace_photons(xyz, idx, trace_fn(sc), 2000000, batch=1 << 20)
+    log("photons: %d traced in %.1fs" % (len(P), time.time() - t))
+    Q, QN = scenes.photon_queries(xyz, idx, trace_fn(sc), res=1000)
+    log("queries: %d" % len(Q))
+    k = 50
+    dq = torch.from_numpy(Q).cuda()
+    dqn = torch.from_numpy(QN).cuda()
+    di = torch.empty((len(Q), k), dtype=torch.int32, device="cuda")
+    dd = torch.empty((len(Q), k), dtype=torch.float32, device="cuda")
+    dc = torch.empty((len(Q),), dtype=torch.int32, device="cuda")
+    drgb = torch.empty((len(Q), 3), dtype=torch.float32, device="cuda")
+    ref = oracle.PhotonMap(P, N, W)
+    perm, _ = ref.order()
+    radii = {}
+    for r, trees, cpu_queries in ((0.05, True, 65536), (0.25, False, 8192)):
         out = {}
-        dq = torch.from_numpy(Q).cuda()
-        dqn = torch.from_numpy(QN).cuda()
-        di = torch.empty((len(Q), k), dtype=torch.int32, device="cuda")
-        dd = torch.empty((len(Q), k), dtype=torch.float32, device="cuda")
-        dc = torch.empty((len(Q),), dtype=torch.int32, device="cuda")
-        drgb = torch.empty((len(Q), 3), dtype=torch.float32, device="cuda")
-        # The engine's default (photon cell grid), then the balanced-tree
-        # walks (RQ_KNN_CELLS=0) over the spatial and ENCORE's own layout.
-        # The parity sample below checks the default path's output.
-        for name, layout, tree in (("tree_encore_layout", "encore", True), ("tree_spatial_layout", "spatial", True),
-                                   ("cells", "spatial", False)):
+        # The engine's default (photon cell grid), then, at the small radius,
+        # the balanced-tree walks (RQ_KNN_CELLS=0) over the spatial and
+        # ENCORE's own layout.  The parity sample checks the default path.
+        paths = ((("tree_encore_layout", "encore", True), ("tree_spatial_layout", "spatial", True)) if trees
+                 else ()) + (("cells", "spatial", False),)
+        for name, layout, tree in paths:
             if tree:
                 os.environ["RQ_KNN_CELLS"] = "0"
             else:
@@ -1051,16 +1089,14 @@ def run_workload(args):
                 work = pm.radiance_counts(dq, dqn, k, r, rgb_c)
                 torch.cuda.synchronize()
                 out[name]["counts_rgb_equal"] = bool(torch.equal(rgb_c.view(torch.int32), drgb.view(torch.int32)))
-            log(name, out[name])
+            log("r=%g" % r, name, out[name])
             pm.destroy()
         os.environ.pop("RQ_KNN_CELLS", None)
         cnt = dc.cpu().numpy()
         got_i = di.cpu().numpy().view(np.uint32)
         got_d = dd.cpu().numpy()
         rgb = drgb.cpu().numpy()
-        ref = oracle.PhotonMap(P, N, W)
-        perm, _ = ref.order()
-        stride = max(1, len(Q) // min(args.cpu_sample, 65536))
+        stride = max(1, len(Q) // min(args.cpu_sample, cpu_queries))
         sq, sqn = np.ascontiguousarray(Q[::stride]), np.ascontiguousarray(QN[::stride])
         t = time.perf_counter()
         rrgb = ref.radiance(sq, sqn, k, r, nthreads=thr)
@@ -1079,37 +1115,35 @@ def run_workload(args):
         rel = np.abs(rgb[::stride] - rrgb) / np.maximum(np.abs(rrgb), 1e-30)
         parity = {"sample": int(len(sq)), "count_equal": same_cnt, "set_equal_tie_aware": same_set,
                   "radiance_max_rel": float(rel.max()), "mean_count": float(gc.mean())}
-        log("parity", parity)
+        log("r=%g parity" % r, parity)
         # Algorithmic bytes of one call: per query 24 B in (position, normal),
         # 8 B per grid cell visited (its photon range), 20 B per photon read
         # (position and class 16, slot 4), 28 B per photon gathered (power 16
         # and insertion index 4 read, index and distance 4 + 4 written) and
         # 16 B out (radiance, count).
         nqf = float(len(Q))
-        gathered = float(dc.to(torch.int64).sum().item())
+        gathered = float(cnt.astype(np.int64).sum())
         alg = 40.0 * nqf + 8.0 * work["cells"] + 20.0 * work["photons"] + 28.0 * gathered
         per_q = {"cells_visited": round(work["cells"] / nqf, 2), "photons_read": round(work["photons"] / nqf, 1),
                  "candidates": round(work["candidates"] / nqf, 1), "gathered": round(gathered / nqf, 2),
                  "bytes": round(alg / nqf, 1)}
-        log("per query", per_q)
+        log("r=%g per query" % r, per_q)
         achieved = alg / (out["cells"]["launch_ms"] / 1e3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "knn_cell_prep + knn_cell_kernel (one call, events around both)"}
-        _line("Mqueries/s photon gather (k=50, r=0.05)", out["cells"]["Mqueries_s"], "Mqueries/s",
-              out["cells"]["launch_ms"],
-              {"workload": "radiance estimate at %d gather points, %d photons (3 bounces, albedo 0.7) on the "
-                           "Cornell box at 1/20 scale tessellated to %d triangles" % (len(Q), len(P), len(idx)),
-               "k": k, "max_dist": r},
-              paths=out, parity=parity, per_query=per_q, roofline=roof,
-              cpu_baseline={"value": round(len(sq) / cpu_s / 1e6, 5), "unit": "Mqueries/s", "cores": thr,
-                            "kind": "port", "sample": "%d gather points" % len(sq)})
-        return 0
-
-    # ---- c5: serving ---------------------------------------------------
-    res = _serving(args, dev, trace_fn)
-    _line("served Mrays/s at equal p99 (serving, 16 resident scenes)", res["served_at_equal_p99"]["Mrays_s"]
-          if res["served_at_equal_p99"] else 0.0, "Mrays/s", 0.0, res.pop("config"), **res)
+        radii[r] = {"paths": out, "parity": parity, "per_query": per_q,
+                    "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                                 "kernel": "knn_cell_prep + knn_cell_kernel (one call, events around both)"},
+                    "cpu_baseline": {"value": round(len(sq) / cpu_s / 1e6, 5), "unit": "Mqueries/s", "cores": thr,
+                                     "kind": "port", "sample": "%d gather points (every %dth)" % (len(sq), stride)}}
+    main, side = radii[0.25], radii[0.05]
+    side["Mqueries_s"] = side["paths"]["cells"]["Mqueries_s"]
+    _line("Mqueries/s photon gather (k=50, r=0.05 at 1/100 scale = 0.25 at 1/20)",
+          main["paths"]["cells"]["Mqueries_s"], "Mqueries/s", main["paths"]["cells"]["launch_ms"],
+          {"workload": "radiance estimate with lists at %d gather points, %d photons (3 bounces, albedo 0.7) on "
+                       "the Cornell box at 1/20 scale tessellated to %d triangles; radius 0.25 = the 1/100 box's 0.05 "
+                       "scaled by 5" % (len(Q), len(P), len(idx)), "k": k, "max_dist": 0.25},
+          paths=main["paths"], parity=main["parity"], per_query=main["per_query"], roofline=main["roofline"],
+          cpu_baseline=main["cpu_baseline"], r_0_05=side)
     return 0
 
 

@@ -555,7 +555,7 @@ def config4_scene(scale=1.0 / 20.0, tess=32, seed=SEED_BASE + 1):
     5 units wide; with 2,000,000 photons a 0.05 gather radius reaches tens
     to a hundred of them, and the tessellated edges (0.04-0.3) keep the
     products of two edges mostly clear of Cross()'s 1e-3 flush
-    (Vector3.h:148-160), so faces keep well-defined normals.  At 1/100 the walls
+    (Vector3.h:148-160), so faces keep well-defined normals.  At 1/100 the walls'
     edges would be about 0.03 long, their products about 1e-3: normals
     would flush to zero or to one axis."""
     xyz, idx = cornell(seed, scale=scale)
