@@ -88,6 +88,7 @@ static TraceLaunch launch_params(const rq_batch* b) {
         L.coop_min = kKdCoopMinDefault;
     } else {
         L.fetch_threshold = (uint32_t)b->kc.fetch_threshold;
+        L.leaf_batch = bvh_leaf_batch();
     }
     return L;
 }

@@ -147,6 +147,7 @@ static void trace(rq_scene* sc, const rq_ray* rays, uint32_t n, rq_hit* hits, ui
     // tests for sweeps (0 lanes turns them off); results do not depend on them.
     L.coop_lanes = kKdCoopLanesDefault;
     L.coop_min = kKdCoopMinDefault;
+    L.leaf_batch = bvh_leaf_batch();
     if (const char* e = std::getenv("RQ_KD_COOP_LANES")) L.coop_lanes = (uint32_t)std::min(64, std::max(0, std::atoi(e)));
     if (const char* e = std::getenv("RQ_KD_COOP_MIN")) L.coop_min = (uint32_t)std::max(1, std::atoi(e));
     launch_zero_u32(scratch, 2, s);

@@ -319,6 +319,7 @@ struct TraceLaunch {
     uint32_t fetch_threshold;    // refill idle lanes when fewer than this many are busy
     float root_pos[3], root_size[3];  // kd only
     uint32_t coop_lanes, coop_min;    // kd: cooperative leaf tests (see kd_trace)
+    uint32_t leaf_batch;              // BVH: run the wave's leaf phase once this many lanes wait at a leaf
     const float* item_boxes;          // batched kd launch: root pos[3], pad, size[3], pad per item
 };
 
@@ -485,6 +486,9 @@ void launch_bvh_batched(const TraceLaunch& L, uint32_t predicate, TraceKind kind
 int occupancy_bvh_batched(uint32_t predicate, TraceKind kind, uint32_t width, int stack);
 void launch_kd(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, hipStream_t s);
 void launch_kd_batched(const TraceLaunch& L, uint32_t predicate, TraceKind kind, int grid, hipStream_t s);
+// BVH launches: lanes waiting at a leaf before the wave tests leaves
+// (RQ_BVH_LEAF_BATCH overrides the default for sweeps).
+uint32_t bvh_leaf_batch();
 int occupancy_kd_batched(uint32_t predicate, TraceKind kind);
 void launch_count(const TraceLaunch& L, uint32_t accel, uint32_t predicate, TraceKind kind, uint32_t width, int grid,
                   hipStream_t s);

@@ -196,6 +196,7 @@ void issue(rq_server* s, Slot& sl) {
         L.spill = sl.scratch.as<uint32_t>() + 64;
         L.spill_depth = words;
         L.fetch_threshold = (uint32_t)s->dev->kc.fetch_threshold;
+        L.leaf_batch = bvh_leaf_batch();
         const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(grid_max, ((int64_t)r->n + kBvhBlock - 1) / kBvhBlock));
         launch_zero_u32(L.counter, 2, st);
         launch_bvh(L, pred, kind, width, (int)grid, s->dev->kc, st);
@@ -225,6 +226,7 @@ void issue(rq_server* s, Slot& sl) {
         L.items = sl.table.as<BatchItemDev>();
         L.nitems = i;
         L.fetch_threshold = (uint32_t)s->kc.fetch_threshold;
+        L.leaf_batch = bvh_leaf_batch();
         const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(grid_max, ((int64_t)total + kBvhBlock - 1) / kBvhBlock));
         launch_zero_u32(L.counter, 2, st);
         launch_bvh_batched(L, pred, kind, width, (int)grid, s->kc, st);

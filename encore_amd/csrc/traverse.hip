@@ -22,6 +22,9 @@
 //    traversal must reproduce the reference bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <cstring>
+
 #include "rq_internal.h"
 
 namespace rq {
@@ -161,6 +164,7 @@ __device__ __forceinline__ float flush_small(float x) { return (x > -0.001f && x
 // ENCORE's Moller-Trumbore variant (TrianglePrim.cpp:204-249): edges given,
 // Cross() with the component flush, no determinant test, inclusive edges.
 struct PredEncore {
+    static constexpr bool kShear = false;  // reads only the ray's origin and direction
     __device__ __forceinline__ static void prepare(RayState&) {}
     __device__ __forceinline__ static bool test(const RayState& r, const float4 a, const float4 b, const float4 c,
                                                 float tmin, float& t, float& u, float& v) {
@@ -190,6 +194,7 @@ __device__ __forceinline__ float comp(float x, float y, float z, int k) { return
 
 // Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2(1) 2013).
 struct PredWatertight {
+    static constexpr bool kShear = true;  // also reads the shear (kx, ky, kz, sx, sy, sz)
     __device__ __forceinline__ static void prepare(RayState& r) {
         const float ax = fabsf(r.dx), ay = fabsf(r.dy), az = fabsf(r.dz);
         int kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
@@ -359,6 +364,59 @@ __device__ __forceinline__ uint32_t chunk_item(const BatchItemDev* items, uint32
     return lo;
 }
 
+// BVH4 node step on the node's sign-selected near/far rows (near x, far x,
+// near y, far y, near z, far z) and its refs: the four slab tests,
+// the hit children sorted by entry distance with a 5-comparator network,
+// the nearest visited next and the others pushed far to near with their
+// entry distance.
+template <int S, class Pop>
+__device__ __forceinline__ void node4_step(const RayState& r, const float4 (&d)[7], float tmin, float tbest,
+                                           RingStack<S, 2>& stk, uint32_t& cur, Pop&& pop_next) {
+    const float kInf = __int_as_float(0x7f800000);
+    const float4 nx = d[0], fx = d[1], ny = d[2], fy = d[3], nz = d[4], fz = d[5], rf = d[6];
+    float k[4];
+    uint32_t c[4] = {__float_as_uint(rf.x), __float_as_uint(rf.y), __float_as_uint(rf.z), __float_as_uint(rf.w)};
+    const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
+    const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
+    const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float tnx = __builtin_fmaf(nxa[j], r.ix, r.cnx), tfx = __builtin_fmaf(fxa[j], r.ix, r.cfx);
+        const float tny = __builtin_fmaf(nya[j], r.iy, r.cny), tfy = __builtin_fmaf(fya[j], r.iy, r.cfy);
+        const float tnz = __builtin_fmaf(nza[j], r.iz, r.cnz), tfz = __builtin_fmaf(fza[j], r.iz, r.cfz);
+        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tbest)) * 1.00000048f;
+        k[j] = (tn <= tf && c[j] != kRefEmpty) ? tn : kInf;
+    }
+    cswap(k[0], c[0], k[1], c[1]);
+    cswap(k[2], c[2], k[3], c[3]);
+    cswap(k[0], c[0], k[2], c[2]);
+    cswap(k[1], c[1], k[3], c[3]);
+    cswap(k[1], c[1], k[2], c[2]);
+    if (k[0] < kInf) {
+        // Push children 1..nh-1 far to near: child j lands at offset
+        // npush - j from the current top.
+        const int npush = (k[1] < kInf) + (k[2] < kInf) + (k[3] < kInf);
+        stk.reserve(npush);
+        if (npush >= 3) {
+            uint32_t v[2] = {c[3], __float_as_uint(k[3])};
+            stk.put(0, v);
+        }
+        if (npush >= 2) {
+            uint32_t v[2] = {c[2], __float_as_uint(k[2])};
+            stk.put(npush - 2, v);
+        }
+        if (npush >= 1) {
+            uint32_t v[2] = {c[1], __float_as_uint(k[1])};
+            stk.put(npush - 1, v);
+        }
+        stk.advance(npush);
+        cur = c[0];
+    } else {
+        cur = pop_next();
+    }
+}
+
 // One traversal step per lane per iteration, through shared loads.  Every
 // busy lane fetches one record with the same NL 16-byte load instructions --
 // a node (BVH4: the sign-selected near/far rows and the refs, 7 loads; BVH2:
@@ -377,13 +435,19 @@ __device__ __forceinline__ uint32_t chunk_item(const BatchItemDev* items, uint32
 //
 // One wave per block.  Idle lanes are refilled from the staged chunk
 // whenever fewer than fetch_threshold lanes are busy.
-template <class Pred, bool ANY, int S, bool COUNT = false, bool BATCHED = false, int WIDTH = 2>
+//
+// COOP (BVH4, RQ_BVH_LEAF_MODE=wave): leaves are tested by the whole wave
+// instead (see below).  It cuts VALU instructions by about a third and raises
+// the active lanes per VALU instruction, but at 116 VGPRs it holds fewer waves,
+// and the config-3 launch, bound by node-fetch latency, runs no faster.
+template <class Pred, bool ANY, int S, bool COUNT = false, bool BATCHED = false, int WIDTH = 2, bool COOP = false>
 __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
     constexpr int SW = WIDTH == 4 ? 2 : 1;      // dwords per stack entry
     constexpr int NL = WIDTH == 4 ? 7 : 4;      // 16-byte rows a lane reads from its record
     constexpr int TPS = WIDTH == 4 ? 2 : 1;     // triangles per leaf step
     __shared__ uint32_t lds_stack[S * SW][kWave];
     __shared__ float lds_stage[kStageFields][kWave];
+    __shared__ unsigned long long lds_leafkey[COOP ? kWave : 1];
     const int lane = lane_id();
     const uint32_t gwave = blockIdx.x;
     RingStack<S, SW> stk;
@@ -540,7 +604,165 @@ __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
 
         bool did_leaf = false;  // COUNT only
 
-        if (local != kNone) {
+        if constexpr (COOP) {
+            // Inner nodes and leaves in separate phases.  A lane at an inner
+            // node takes its node step every iteration; a lane at a leaf
+            // waits until leaf_batch lanes wait (or no lane is at a node),
+            // and then the wave tests the waiting leaves' triangles,
+            // concatenated in lane order, one per lane, 64 per round.  Lane
+            // j of a round reads from LDS which leaf (owner lane) and
+            // triangle hold position base + j, takes the owner's ray
+            // through cross-lane reads and tests the triangle against the
+            // owner's closest hit so far.  Each owner keeps the least
+            // (t, position) of its hits -- the hit its own strict-< loop over
+            // the leaf in order would keep -- through a 64-bit LDS minimum,
+            // and pops its next entry with that hit.
+            const bool at_node = local != kNone && (int)cur >= 0 && cur != kRefSentinel;
+            const bool at_leaf = local != kNone && (int)cur < 0 && cur != kRefSentinel;
+            // node rows first, so their loads overlap the leaf phase
+            const float* np = WIDTH == 4 ? reinterpret_cast<const float*>(static_cast<const BvhNode4*>(nodes) + cur)
+                                         : reinterpret_cast<const float*>(static_cast<const BvhNode*>(nodes) + cur);
+            float4 d[NL];
+            if (at_node) {
+                if (WIDTH == 4) {
+                    d[0] = ldg4(np + offx);
+                    d[1] = ldg4(np + (offx ^ 4u));
+                    d[2] = ldg4(np + offy);
+                    d[3] = ldg4(np + (offy ^ 4u));
+                    d[4] = ldg4(np + offz);
+                    d[5] = ldg4(np + (offz ^ 4u));
+                    d[6] = ldg4(np + 24);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NL; ++q) d[q] = ldg4(np + 4 * q);
+                }
+            }
+            const uint64_t lm = __ballot(at_leaf);
+            const uint32_t nwait = (uint32_t)__popcll(lm);
+            if (nwait != 0 && (nwait >= L.leaf_batch || __ballot(at_node) == 0)) {
+                did_leaf = at_leaf;
+                const uint32_t cnt = at_leaf ? ((cur >> 27) & 15u) + 1u : 0u;  // 1..16
+                const uint32_t first = cur & 0x07FFFFFFu;
+                // exclusive prefix of cnt over the lanes, one ballot per bit
+                uint32_t start = 0, total = 0;
+#pragma unroll
+                for (int b = 0; b < 5; ++b) {
+                    const uint64_t mb = __ballot((cnt >> b) & 1u);
+                    start += lanes_below(mb) << b;
+                    total += (uint32_t)__popcll(mb) << b;
+                }
+                if (COUNT) n_tris += cnt;
+                bool hit_any = false;
+                for (uint32_t base = 0; base < total; base += kWave) {
+                    // (owner, triangle) of positions base .. base + 63
+                    if (at_leaf) {
+                        const int j0 = max(0, (int)base - (int)start);
+                        const int j1 = min((int)cnt, (int)(base + kWave) - (int)start);
+                        for (int j = j0; j < j1; ++j)
+                            lds_leafkey[start + j - base] = ((unsigned long long)lane << 32) | (first + (uint32_t)j);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    const uint32_t p = base + (uint32_t)lane;
+                    const unsigned long long ot = p < total ? lds_leafkey[lane] : 0ull;
+                    __builtin_amdgcn_wave_barrier();
+                    const int o = (int)(ot >> 32);
+                    const uint32_t tri = (uint32_t)ot;
+                    RayState orr;
+                    orr.ox = __shfl(r.ox, o, kWave);
+                    orr.oy = __shfl(r.oy, o, kWave);
+                    orr.oz = __shfl(r.oz, o, kWave);
+                    orr.dx = __shfl(r.dx, o, kWave);
+                    orr.dy = __shfl(r.dy, o, kWave);
+                    orr.dz = __shfl(r.dz, o, kWave);
+                    if constexpr (Pred::kShear) {
+                        orr.kx = __shfl(r.kx, o, kWave);
+                        orr.ky = __shfl(r.ky, o, kWave);
+                        orr.kz = __shfl(r.kz, o, kWave);
+                        orr.sx = __shfl(r.sx, o, kWave);
+                        orr.sy = __shfl(r.sy, o, kWave);
+                        orr.sz = __shfl(r.sz, o, kWave);
+                    }
+                    const float otmin = __shfl(tmin, o, kWave);
+                    const float otbest = __shfl(tbest, o, kWave);
+                    const TriRec* otris = tris;
+                    if (BATCHED) {
+                        const uint64_t tp = reinterpret_cast<uint64_t>(tris);
+                        const uint32_t lo32 = (uint32_t)__shfl((int)(uint32_t)tp, o, kWave);
+                        const uint32_t hi32 = (uint32_t)__shfl((int)(uint32_t)(tp >> 32), o, kWave);
+                        otris = reinterpret_cast<const TriRec*>(((uint64_t)hi32 << 32) | lo32);
+                    }
+                    unsigned long long key = ~0ull;
+                    float tu = 0.0f, tv = 0.0f;
+                    uint32_t tprim = 0;
+                    if (p < total) {
+                        const float* tp = reinterpret_cast<const float*>(otris + tri);
+                        const float4 a = ldg4(tp), b = ldg4(tp + 4), c = ldg4(tp + 8);
+                        float t;
+                        if (Pred::test(orr, a, b, c, otmin, t, tu, tv) && t < otbest)
+                            key = ((unsigned long long)__float_as_uint(t) << 32) | p;
+                        tprim = __float_as_uint(a.w);
+                    }
+                    if (at_leaf) lds_leafkey[lane] = ~0ull;
+                    __builtin_amdgcn_wave_barrier();
+                    if (key != ~0ull) atomicMin(&lds_leafkey[o], key);
+                    __builtin_amdgcn_wave_barrier();
+                    const unsigned long long best = at_leaf ? lds_leafkey[lane] : ~0ull;
+                    const int wl = (int)((uint32_t)best - base);  // the winning lane, when best is a hit
+                    const float wu = __shfl(tu, wl, kWave), wv = __shfl(tv, wl, kWave);
+                    const uint32_t wp = (uint32_t)__shfl((int)tprim, wl, kWave);
+                    if (best != ~0ull) {
+                        tbest = __uint_as_float((uint32_t)(best >> 32));
+                        bu = wu;
+                        bv = wv;
+                        bprim = wp;
+                        hit_any = true;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+                if (at_leaf) cur = (ANY && hit_any) ? kRefSentinel : pop_next();
+            }
+            // Node step (the loads above).
+            if (at_node) {
+                if (COUNT) ++n_nodes;
+                if constexpr (WIDTH == 2) {
+                    const uint4 meta = make_uint4(__float_as_uint(d[3].x), __float_as_uint(d[3].y),
+                                                  __float_as_uint(d[3].z), __float_as_uint(d[3].w));
+                    float tn0, tn1;
+                    bool h1;
+                    const bool h0 = box2(r, d[0], d[1], d[2], tmin, tbest, tn0, tn1, h1);
+                    if (h0 && h1) {
+                        const bool first0 = tn0 <= tn1;
+                        uint32_t v[SW] = {first0 ? meta.y : meta.x};
+                        stk.push(v);
+                        cur = first0 ? meta.x : meta.y;
+                    } else if (h0) {
+                        cur = meta.x;
+                    } else if (h1) {
+                        cur = meta.y;
+                    } else {
+                        cur = pop_next();
+                    }
+                } else {
+                    node4_step(r, d, tmin, tbest, stk, cur, pop_next);
+                }
+            }
+            // Retire a finished ray.
+            if (local != kNone && cur == kRefSentinel) {
+                if (ANY) {
+                    stg1(static_cast<uint32_t*>(out) + local, (bprim != kNone) ? 1u : 0u);
+                } else {
+                    const bool has = bprim != kNone;
+                    stg4(static_cast<rq_hit*>(out) + local,
+                         make_float4(tbest, has ? bu : 0.0f, has ? bv : 0.0f, __uint_as_float(bprim)));
+                }
+                if (COUNT) {
+                    atomicAdd(&L.counts[0], (unsigned long long)n_nodes);
+                    atomicAdd(&L.counts[1], (unsigned long long)n_tris);
+                    n_nodes = n_tris = 0;
+                }
+                local = kNone;
+            }
+        } else if (local != kNone) {
             // A leaf ref in cur becomes the leaf in progress.
             if (lpos >= lend && (int)cur < 0 && cur != kRefSentinel) {
                 lpos = cur & 0x07FFFFFFu;
@@ -1062,9 +1284,19 @@ __global__ void zero_u32(uint32_t* p, uint32_t n) {
 // kd ring: 8 entries x 3 dwords x 64 lanes x 4 waves = 24 KB of LDS per block.
 constexpr int kKdStack = kKdStackEntries;
 
+// RQ_BVH_LEAF_MODE=wave selects the wave-shared leaf tests for BVH4 launches
+// (read at every launch; results do not depend on it).
+static bool bvh_leaf_wave() {
+    const char* e = std::getenv("RQ_BVH_LEAF_MODE");
+    return e != nullptr && std::strcmp(e, "wave") == 0;
+}
+
 template <class Pred, bool ANY, bool COUNT, bool BATCHED, int WIDTH>
 static const void* bvh_fn_s(int stack) {
     if (COUNT) return (const void*)dev::bvh_trace<Pred, ANY, 16, COUNT, BATCHED, WIDTH>;
+    if constexpr (WIDTH == 4) {
+        if (bvh_leaf_wave()) return (const void*)dev::bvh_trace<Pred, ANY, 8, false, BATCHED, 4, true>;
+    }
     switch (stack) {
         case 8:
             return (const void*)dev::bvh_trace<Pred, ANY, 8, COUNT, BATCHED, WIDTH>;
@@ -1118,6 +1350,15 @@ static int occupancy_of(const void* fn, int block) {
     int nb = 0;
     RQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0));
     return nb > 0 ? nb : 1;
+}
+
+uint32_t bvh_leaf_batch() {
+    static const uint32_t v = [] {
+        uint32_t b = 16;
+        if (const char* e = std::getenv("RQ_BVH_LEAF_BATCH")) b = (uint32_t)std::min(64, std::max(1, std::atoi(e)));
+        return b;
+    }();
+    return v;
 }
 
 void launch_bvh(const TraceLaunch& L, uint32_t predicate, TraceKind kind, uint32_t width, int grid,
