@@ -1117,14 +1117,15 @@ def _photon_gather_workload(args, dev, thr, trace_fn):
                   "radiance_max_rel": float(rel.max()), "mean_count": float(gc.mean())}
         log("r=%g parity" % r, parity)
         # Algorithmic bytes of one call: per query 24 B in (position, normal),
-        # 8 B per grid cell visited (its photon range), 20 B per photon read
-        # (position and class 16, slot 4), 28 B per photon gathered (power 16
-        # and insertion index 4 read, index and distance 4 + 4 written) and
-        # 16 B out (radiance, count).
+        # 8 B per range ranked (a cell's photon range, or a block's 32-byte
+        # box counted at the cell rate), 20 B per photon read (position and
+        # class 16, slot 4), 28 B per photon gathered (power 16 and insertion
+        # index 4 read, index and distance 4 + 4 written) and 16 B out
+        # (radiance, count).
         nqf = float(len(Q))
         gathered = float(cnt.astype(np.int64).sum())
-        alg = 40.0 * nqf + 8.0 * work["cells"] + 20.0 * work["photons"] + 28.0 * gathered
-        per_q = {"cells_visited": round(work["cells"] / nqf, 2), "photons_read": round(work["photons"] / nqf, 1),
+        alg = 40.0 * nqf + 8.0 * work["ranges"] + 20.0 * work["photons"] + 28.0 * gathered
+        per_q = {"ranges_ranked": round(work["ranges"] / nqf, 2), "photons_read": round(work["photons"] / nqf, 1),
                  "candidates": round(work["candidates"] / nqf, 1), "gathered": round(gathered / nqf, 2),
                  "bytes": round(alg / nqf, 1)}
         log("r=%g per query" % r, per_q)

@@ -369,6 +369,17 @@ __device__ uint32_t compact_kth(uint32_t* s_key, uint32_t* s_slot, uint32_t n, u
     return out;
 }
 
+// The largest of the n keys left after a compaction: the k-th smallest
+// squared distance so far, the strict bound for later candidates (ENCORE
+// takes the heap's maximum once the heap is full, PhotonMap.cpp:255, 268).
+// The selection's threshold can be the end of a radix bucket, up to 4x that.
+__device__ __forceinline__ uint32_t kept_max(const uint32_t* s_key, uint32_t n) {
+    uint32_t m = 0;
+    for (uint32_t i = __lane_id(); i < n; i += 64) m = max(m, s_key[i]);
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
+    return m;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
@@ -549,7 +560,7 @@ __global__ __launch_bounds__(64) void knn_wave_kernel(KnnLaunch L, uint32_t stac
                     uint32_t need = k;
                     const uint32_t t = select_kth(s_key, cand, need, s_hist);
                     cand = compact_kth(s_key, s_slot, cand, t, need);
-                    r2 = __uint_as_float(t);
+                    r2 = __uint_as_float(kept_max(s_key, cand));
                 }
             }
         }
@@ -596,12 +607,14 @@ constexpr int kCellCand = 2 * (int)kKnnMaxK + 64;
 // a wave per SIMD (a variant at 98 VGPRs ran 8% slower), and asking for 5
 // waves with __launch_bounds__ makes the compiler spill to scratch instead.
 constexpr int kCellWaves = 4;
+// Blocks a query may examine before it keeps to whole cells.
+constexpr uint32_t kCellMaxBlocks = 256;
 
-// COUNT: the instrumented copy tallies, per query, the cells visited, the
-// photons read and the candidates (L.tally[0..3]: queries, cells, photons,
-// candidates).
+// COUNT: the instrumented copy tallies, per query, the ranges ranked (cells,
+// or blocks), the photons read and the candidates (L.tally[0..3]: queries,
+// ranges, photons, candidates).
 template <bool COUNT>
-__global__ __launch_bounds__(64 * kCellWaves) void knn_cell_kernel(KnnLaunch L) {
+__global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch L) {
     __shared__ uint32_t s_key_w[kCellWaves][kCellCand];
     __shared__ uint32_t s_slot_w[kCellWaves][kCellCand];
     __shared__ uint32_t s_hist_w[kCellWaves][256];
@@ -668,12 +681,88 @@ __global__ __launch_bounds__(64 * kCellWaves) void knn_cell_kernel(KnnLaunch L) 
             near2 *= 1.0 - 0x1p-20;
             v = v && near2 < (double)r2 && !far;
             uint2 range = make_uint2(0u, 0u);
+            uint32_t fb = 0;  // the cell's first block (grids with blocks)
             if (v) {
-                if (G.dense)
-                    range = G.dense[c[0] + (uint64_t)G.dims[0] * (c[1] + (uint64_t)G.dims[1] * c[2])];
-                else
+                if (G.dense) {
+                    const uint64_t lin = c[0] + (uint64_t)G.dims[0] * (c[1] + (uint64_t)G.dims[1] * c[2]);
+                    range = G.dense[lin];
+                    if (G.blocks) fb = G.dense_block[lin];
+                } else {
                     range = cell_lookup(G, cell_key(c[0], c[1], c[2]));
+                }
                 v = range.y > 0;
+            }
+            if (G.blocks) {
+                // Blocks instead of cells: when the kept cells hold at most
+                // kCellMaxBlocks blocks and some cell holds more than one,
+                // the wave examines them 64 at a time and keeps those whose
+                // box lies within the radius; if at most 64 remain, they
+                // replace the cells.  Block j's owner is the last lane whose
+                // first block number is at or below j (a lane with no block
+                // shares its start with the next lane that has one).
+                const uint32_t nb = v ? (range.y + 63u) >> 6 : 0u;
+                uint32_t inc = nb;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const uint32_t t = __shfl_up(inc, o, 64);
+                    if (lane >= (uint32_t)o) inc += t;
+                }
+                const uint32_t nblk = __builtin_amdgcn_readlane(inc, 63);
+                if (nblk <= kCellMaxBlocks && nblk > (uint32_t)__popcll(__ballot(v))) {
+                    const uint32_t ex = inc - nb;
+                    uint32_t nsurv = 0;
+                    bool fits = true;
+                    for (uint32_t base = 0; base < nblk; base += 64) {
+                        const uint32_t j = base + lane;
+                        uint32_t o = 0;
+                        for (uint32_t step = 32; step; step >>= 1) {
+                            const uint32_t e = (uint32_t)__shfl((int)ex, (int)(o + step), 64);
+                            if (e <= j) o += step;
+                        }
+                        const uint32_t b =
+                            (uint32_t)__shfl((int)fb, (int)o, 64) + (j - (uint32_t)__shfl((int)ex, (int)o, 64));
+                        bool keep = false;
+                        float nf = 0.0f;
+                        uint32_t bf = 0, bc = 0;
+                        if (j < nblk) {
+                            const float4 lo = G.blocks[2 * (size_t)b], hi = G.blocks[2 * (size_t)b + 1];
+                            const double l[3] = {(double)lo.x, (double)lo.y, (double)lo.z};
+                            const double h[3] = {(double)hi.x, (double)hi.y, (double)hi.z};
+                            double nr = 0.0;
+                            for (int a = 0; a < 3; ++a) {
+                                const double d = fmax(fmax(l[a] - qd[a], qd[a] - h[a]), 0.0);
+                                nr += d * d;
+                            }
+                            // a photon's float d2 is within 2^-21 of its distance
+                            nr *= 1.0 - 0x1p-20;
+                            keep = nr < (double)r2;
+                            nf = __double2float_rd(nr);
+                            bf = __float_as_uint(lo.w);
+                            bc = __float_as_uint(hi.w);
+                        }
+                        const uint64_t km = __ballot(keep);
+                        const uint32_t cnt = (uint32_t)__popcll(km);
+                        if (nsurv + cnt > 64u) {
+                            fits = false;
+                            break;
+                        }
+                        if (keep) {
+                            const uint32_t at = nsurv + lanes_below(km);
+                            s_first[at] = bf;
+                            s_pre[at] = bc;
+                            s_near[at] = nf;
+                        }
+                        nsurv += cnt;
+                    }
+                    if (fits) {
+                        __builtin_amdgcn_wave_barrier();
+                        v = lane < nsurv;
+                        if (v) {
+                            range = make_uint2(s_first[lane], s_pre[lane]);
+                            near2 = (double)s_near[lane];
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                }
             }
             // Nearest first.  The bound is kept as a float rounded down with
             // its 6 low bits cleared (still a lower bound), and the lane in
@@ -774,7 +863,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void knn_cell_kernel(KnnLaunch L) 
                     uint32_t need = k;
                     const uint32_t t = select_kth(s_key, cand, need, s_hist);
                     cand = compact_kth(s_key, s_slot, cand, t, need);
-                    r2 = __uint_as_float(t);
+                    r2 = __uint_as_float(kept_max(s_key, cand));
                 }
                 p = pn;
                 sl = sn;

@@ -9,15 +9,25 @@
 // class and distance tests); the selecting kernel then keeps the k nearest.
 //
 // Build, on the device, for one squared radius:
+//   pg_subkeys Morton key of each photon's sub-cell (16 per axis) within its
+//              cell, and a stable radix sort of the slots by it;
 //   pg_keys    Morton key of each photon's cell (~0 for photons no query can
 //              select: a position that is not finite or no normal class);
-//   sort       (key, slot) pairs by key, rocPRIM radix sort;
+//   sort       (key, slot) pairs by key, stable, rocPRIM radix sort: a cell's
+//              photons keep their sub-cell order;
 //   pg_starts  the first photon of each cell inserts the cell into the table;
 //   pg_ends    the last photon of each cell writes its count;
-//   pg_gather  positions and classes in grid order.
+//   pg_gather  positions and classes in grid order;
+//   blocks     (dense table only) each cell's photons cut into blocks of up
+//              to 64 from its first photon, numbered by a scan, with their
+//              bounding boxes (pg_bflags, scan, pg_blocks).
 // The table is a dense array over the photons' bounding box when that takes
 // at most kGridDenseMaxBytes (one load per cell, no hashing or probing),
 // otherwise an open-addressing hash table keyed by the cell's Morton code.
+//
+// A query whose cells hold at most 64 blocks ranks the blocks instead of the
+// cells: along the sub-cell curve a block covers a small part of its cell, so
+// once the k-th distance has shrunk below a block's box the block is skipped.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -33,10 +43,16 @@ namespace dev {
 struct PgBuild {
     const PhotonHot* hot;
     uint32_t n, nvalid;
+    uint32_t* subkeys;
+    const uint32_t* ids1;  // slots in sub-cell order
     unsigned long long* keys;
     uint32_t* ids;
     const unsigned long long* skeys;
     const uint32_t* sids;
+    uint32_t* flags;       // 1 where a block starts (grid order)
+    const uint32_t* bidx;  // exclusive scan of flags: the block of a start
+    uint32_t* dense_block;
+    float4* blocks;
     float4* pos;
     uint32_t* slot;
     uint4* cells;
@@ -71,9 +87,37 @@ __global__ void pg_clear(uint4* cells, uint32_t n) {
     if (i < n) cells[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
 }
 
-__global__ void pg_keys(PgBuild B) {
+__device__ __forceinline__ uint32_t spread4(uint32_t v) {
+    uint32_t r = 0;
+    for (int b = 0; b < 4; ++b) r |= ((v >> b) & 1u) << (3 * b);
+    return r;
+}
+
+// Sub-cell (16 per axis) of each photon within its cell, as a 12-bit Morton
+// key; the slots are then sorted by it.
+__global__ void pg_subkeys(PgBuild B) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B.n) return;
+    const PhotonHot p = B.hot[i];
+    uint32_t key = 0;
+    if (isfinite(p.x) && isfinite(p.y) && isfinite(p.z)) {
+        const float x[3] = {p.x, p.y, p.z};
+        uint32_t f[3];
+        for (int a = 0; a < 3; ++a) {
+            const double t = ((double)x[a] - B.origin[a]) * B.inv_cell;
+            const double c = floor(t);
+            f[a] = (uint32_t)min(max((int64_t)floor((t - c) * 16.0), (int64_t)0), (int64_t)15);
+        }
+        key = (spread4(f[0]) << 2) | (spread4(f[1]) << 1) | spread4(f[2]);
+    }
+    B.subkeys[i] = key;
+    B.ids[i] = i;
+}
+
+__global__ void pg_keys(PgBuild B) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= B.n) return;
+    const uint32_t i = B.ids1[j];
     const PhotonHot p = B.hot[i];
     unsigned long long key = kEmptyKey;
     if (isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && (p.meta >> 2) != kClassNever) {
@@ -83,8 +127,8 @@ __global__ void pg_keys(PgBuild B) {
         const int64_t cz = grid_coord((double)p.z, B.origin[2], B.inv_cell);
         key = cell_key((uint64_t)cx, (uint64_t)cy, (uint64_t)cz);
     }
-    B.keys[i] = key;
-    B.ids[i] = i;
+    B.keys[j] = key;
+    B.ids[j] = i;
 }
 
 __global__ void pg_starts(PgBuild B) {
@@ -140,6 +184,39 @@ __global__ void pg_gather(PgBuild B) {
     B.slot[i] = s;
 }
 
+// Block starts: a cell's first photon and every 64th after it.
+__global__ void pg_bflags(PgBuild B) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.nvalid) return;
+    const uint2 e = B.dense[pg_dense_index(B, B.hot[B.sids[i]])];
+    B.flags[i] = ((i - e.x) & 63u) == 0u ? 1u : 0u;
+}
+
+// One thread per block start: the block's bounding box and range, and the
+// cell's first block.
+__global__ void pg_blocks(PgBuild B) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.nvalid || B.flags[i] == 0u) return;
+    const uint64_t c = pg_dense_index(B, B.hot[B.sids[i]]);
+    const uint2 e = B.dense[c];
+    const uint32_t off = i - e.x;
+    const uint32_t len = min(64u, e.y - off);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t k = 0; k < len; ++k) {
+        const float4 q = B.pos[i + k];
+        lo[0] = fminf(lo[0], q.x);
+        lo[1] = fminf(lo[1], q.y);
+        lo[2] = fminf(lo[2], q.z);
+        hi[0] = fmaxf(hi[0], q.x);
+        hi[1] = fmaxf(hi[1], q.y);
+        hi[2] = fmaxf(hi[2], q.z);
+    }
+    const uint32_t b = B.bidx[i];
+    B.blocks[2 * (size_t)b] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(i));
+    B.blocks[2 * (size_t)b + 1] = make_float4(hi[0], hi[1], hi[2], __uint_as_float(len));
+    if (off == 0) B.dense_block[c] = b;
+}
+
 }  // namespace dev
 
 static unsigned blocks_for(uint64_t n, unsigned block = 256) { return (unsigned)((n + block - 1) / block); }
@@ -181,10 +258,13 @@ const PhotonGrid* photon_grid_for(rq_photons* p, float r2, hipStream_t s) {
     grid->cells = DevBuf((size_t)cap * sizeof(uint4));
     if (dense) grid->dense = DevBuf(ncells * sizeof(uint2));
     DevBuf keys((size_t)n * 8), skeys((size_t)n * 8), ids((size_t)n * 4), sids((size_t)n * 4);
+    DevBuf subkeys((size_t)n * 4), ssub((size_t)n * 4), ids1((size_t)n * 4);
     dev::PgBuild B{};
     B.hot = p->hot.as<PhotonHot>();
     B.n = n;
     B.nvalid = nv;
+    B.subkeys = subkeys.as<uint32_t>();
+    B.ids1 = ids1.as<uint32_t>();
     B.keys = keys.as<unsigned long long>();
     B.ids = ids.as<uint32_t>();
     B.skeys = skeys.as<unsigned long long>();
@@ -205,13 +285,20 @@ const PhotonGrid* photon_grid_for(rq_photons* p, float r2, hipStream_t s) {
         hipLaunchKernelGGL(dev::pg_clear_dense, dim3(blocks_for(ncells)), dim3(256), 0, s, B.dense, ncells);
         RQ_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(dev::pg_keys, dim3(blocks_for(n)), dim3(256), 0, s, B);
+    // Slots by sub-cell, then (stable) by cell: a cell's photons follow the
+    // sub-cell curve.
+    hipLaunchKernelGGL(dev::pg_subkeys, dim3(blocks_for(n)), dim3(256), 0, s, B);
     RQ_HIP(hipGetLastError());
-    // Stable sort: photons of one cell keep their slot order.
-    size_t tmp_bytes = 0;
+    size_t tmp_sub = 0, tmp_bytes = 0;
+    RQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_sub, B.subkeys, ssub.as<uint32_t>(), B.ids,
+                                              ids1.as<uint32_t>(), (int)n, 0, 12, s));
     RQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, B.keys, skeys.as<unsigned long long>(), B.ids,
                                               sids.as<uint32_t>(), (int)n, 0, 64, s));
-    DevBuf tmp(std::max<size_t>(tmp_bytes, 16));
+    DevBuf tmp(std::max<size_t>(std::max(tmp_bytes, tmp_sub), 16));
+    RQ_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tmp_sub, B.subkeys, ssub.as<uint32_t>(), B.ids,
+                                              ids1.as<uint32_t>(), (int)n, 0, 12, s));
+    hipLaunchKernelGGL(dev::pg_keys, dim3(blocks_for(n)), dim3(256), 0, s, B);
+    RQ_HIP(hipGetLastError());
     RQ_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tmp_bytes, B.keys, skeys.as<unsigned long long>(), B.ids,
                                               sids.as<uint32_t>(), (int)n, 0, 64, s));
     hipLaunchKernelGGL(dev::pg_starts, dim3(blocks_for(nv)), dim3(256), 0, s, B);
@@ -220,6 +307,32 @@ const PhotonGrid* photon_grid_for(rq_photons* p, float r2, hipStream_t s) {
     RQ_HIP(hipGetLastError());
     hipLaunchKernelGGL(dev::pg_gather, dim3(blocks_for(nv)), dim3(256), 0, s, B);
     RQ_HIP(hipGetLastError());
+    if (dense) {
+        DevBuf flags((size_t)nv * 4), bidx((size_t)nv * 4);
+        B.flags = flags.as<uint32_t>();
+        B.bidx = bidx.as<uint32_t>();
+        hipLaunchKernelGGL(dev::pg_bflags, dim3(blocks_for(nv)), dim3(256), 0, s, B);
+        RQ_HIP(hipGetLastError());
+        size_t scan_bytes = 0;
+        RQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, B.flags, bidx.as<uint32_t>(), (int)nv, s));
+        DevBuf scan_tmp(std::max<size_t>(scan_bytes, 16));
+        RQ_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.ptr, scan_bytes, B.flags, bidx.as<uint32_t>(), (int)nv, s));
+        uint32_t last[2] = {0, 0};
+        RQ_HIP(hipMemcpyAsync(&last[0], bidx.as<uint32_t>() + (nv - 1), 4, hipMemcpyDeviceToHost, s));
+        RQ_HIP(hipMemcpyAsync(&last[1], B.flags + (nv - 1), 4, hipMemcpyDeviceToHost, s));
+        RQ_HIP(hipStreamSynchronize(s));
+        const uint64_t nblocks = (uint64_t)last[0] + last[1];
+        // every block holds a photon: a count above nvalid is a device fault
+        if (nblocks == 0 || nblocks > nv) throw Error(RQ_ERR_DEVICE, "photon grid: inconsistent block count");
+        grid->nblocks = (uint32_t)nblocks;
+        grid->blocks = DevBuf((size_t)nblocks * 2 * sizeof(float4));
+        grid->dense_block = DevBuf(ncells * sizeof(uint32_t));
+        B.blocks = grid->blocks.as<float4>();
+        B.dense_block = grid->dense_block.as<uint32_t>();
+        hipLaunchKernelGGL(dev::pg_blocks, dim3(blocks_for(nv)), dim3(256), 0, s, B);
+        RQ_HIP(hipGetLastError());
+        RQ_HIP(hipStreamSynchronize(s));
+    }
     // The temporaries are freed on return, and other streams may query the
     // grid next: finish the build here.
     RQ_HIP(hipStreamSynchronize(s));
@@ -236,6 +349,8 @@ const PhotonGrid* photon_grid_for(rq_photons* p, float r2, hipStream_t s) {
     v.cell = cell;
     v.inv_cell = inv_cell;
     v.reach = reach;
+    v.dense_block = dense ? grid->dense_block.as<uint32_t>() : nullptr;
+    v.blocks = dense ? grid->blocks.as<float4>() : nullptr;
     // A grid for another radius may still be read by queued queries.
     if (p->grid) RQ_HIP(hipDeviceSynchronize());
     p->grid = std::move(grid);

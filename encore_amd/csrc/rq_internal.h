@@ -209,6 +209,13 @@ struct PhotonGridView {
     double origin[3];      // the photons' lower bounds
     double cell, inv_cell;
     double reach;          // every photon with float d2 < r2 lies within it
+    // Dense grids only (else null): a cell's photons run along a Morton
+    // curve of 16^3 sub-cells and are cut into blocks of up to 64 from the
+    // cell's first photon.  dense_block: a cell's first block; blocks[2b],
+    // blocks[2b + 1]: block b's bounding box, (lo.xyz, first photon) and
+    // (hi.xyz, photon count), the integers as float bits.
+    const uint32_t* dense_block;
+    const float4* blocks;
 };
 
 __host__ __device__ inline int64_t grid_coord(double x, double origin, double inv_cell) {
@@ -499,7 +506,8 @@ void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
 constexpr uint64_t kGridDenseMaxBytes = 256ull << 20;
 struct PhotonGrid {
     uint32_t r2_bits = 0;
-    DevBuf pos, slot, cells, dense;
+    DevBuf pos, slot, cells, dense, dense_block, blocks;
+    uint32_t nblocks = 0;
     PhotonGridView view{};
 };
 // Makes (or keeps) the grid for r2 on stream s and returns it, or null when

@@ -574,11 +574,12 @@ class PhotonMap:
 
     def radiance_counts(self, qpos, qnrm, k, max_dist, rgb, caustic=False, stream=None):
         """Radiance estimates into rgb, with the gather's work counted:
-        {"queries", "cells", "photons", "candidates"} (cell-grid gather only)."""
+        {"queries", "ranges" (cells or blocks ranked), "photons" (read),
+        "candidates"} (cell-grid gather only)."""
         out = np.zeros(4, np.uint64)
         check(load().rq_radiance_counts(self.h, _ptr(qpos), _ptr(qnrm), qpos.shape[0], k, float(max_dist),
                                         int(caustic), _ptr(rgb), out.ctypes.data, _stream(stream)))
-        return dict(zip(("queries", "cells", "photons", "candidates"), (int(v) for v in out)))
+        return dict(zip(("queries", "ranges", "photons", "candidates"), (int(v) for v in out)))
 
 
 class IrrCache:

@@ -366,9 +366,10 @@ rq_status rq_radiance_estimate(rq_photons* photons, const float* d_qpos, const f
                                float* d_d2, uint32_t* d_count, void* stream);
 /* The gather work of rq_radiance_estimate (no lists), counted by an
  * instrumented copy of the photon cell-grid kernel (synchronous):
- * counts[0] = queries, counts[1] = grid cells visited, counts[2] = photons
- * read, counts[3] = photons within the radius at the time they were read
- * (candidates); d_rgb receives the estimates.  RQ_ERR_UNSUPPORTED when k or
+ * counts[0] = queries, counts[1] = photon ranges ranked (grid cells, or the
+ * 64-photon blocks of those cells when a query ranks blocks), counts[2] =
+ * photons read, counts[3] = photons within the radius at the time they were
+ * read (candidates); d_rgb receives the estimates.  RQ_ERR_UNSUPPORTED when k or
  * the radius sends the call to another kernel.  No counterpart in ENCORE
  * (measurement). */
 rq_status rq_radiance_counts(rq_photons* photons, const float* d_qpos, const float* d_qnormal, uint32_t nq,

@@ -203,7 +203,7 @@ def test_cells_counts(device):
     estimates, one query each, and -- with fewer than k photons in range, so
     the candidate list is never cut -- exactly the brute-force number of
     photons in range as candidates, out of at least as many photons read
-    from at most 64 cells per query.  A k or radius the grid does not serve
+    from at most 64 ranges (cells or blocks) per query.  A k or radius the grid does not serve
     is refused."""
     rng = np.random.default_rng(13)
     P, N, W, Q, QN = _surface_photons(rng, 40000, 0.0)
@@ -219,7 +219,7 @@ def test_cells_counts(device):
     assert np.array_equal(rgb_c.cpu().numpy().view(np.uint32), rgb.view(np.uint32))
     assert work["queries"] == len(Q)
     assert work["candidates"] == int(want.sum()) == int(cnt.sum())
-    assert work["photons"] >= work["candidates"] and 0 < work["cells"] <= 64 * len(Q)
+    assert work["photons"] >= work["candidates"] and 0 < work["ranges"] <= 64 * len(Q)
     with pytest.raises(rq.RQError) as e:
         pm.radiance_counts(q, qn, 8, r, rgb_c)
     assert e.value.status == 4
