@@ -279,7 +279,7 @@ rq_status rq_batch_autotune(rq_batch* b, void* stream, rq_kernel_config* chosen,
         // kd lanes hold rays of very different cost, and refills claim rays
         // with an atomic per round: lower thresholds.
         const std::vector<int> thresholds =
-            kd ? std::vector<int>{8, 16, 24, 32} : std::vector<int>{32, 44, 48, 56, 60};
+            kd ? std::vector<int>{16, 32, 48, 56} : std::vector<int>{32, 44, 48, 56, 60};
         // Mean time of `reps` back-to-back launches with configuration kc.
         auto time_config = [&](const KernelConfig& kc, int reps) {
             b->kc = kc;

@@ -633,7 +633,7 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
     const uint32_t lane = __lane_id();
     const uint32_t k = L.k;
     const uint32_t compact_at =
-        min((uint32_t)kCellCand - 64u, L.compact_slack ? k + L.compact_slack : max(2u * k, k + 64u));
+        min((uint32_t)kCellCand - 64u, L.compact_slack ? k + L.compact_slack : max(2u * k, k + 128u));
     // Grid-stride over the prepared queries (cell order when sorted): the
     // waves running at one time hold a window of consecutive queries, which
     // share cells.  The next query's record is loaded a query ahead.
@@ -977,7 +977,7 @@ void launch_knn_cells(const KnnLaunch& L, float4* prep, int grid, hipStream_t s)
     KnnLaunch C = L;
     C.qprep = prep;
     // RQ_KNN_COMPACT_SLACK: compact the candidates once they exceed k by this
-    // many (default max(k, 64)), for sweeps; results do not depend on it.
+    // many (default max(k, 128)), for sweeps; results do not depend on it.
     if (const char* e = std::getenv("RQ_KNN_COMPACT_SLACK")) C.compact_slack = (uint32_t)std::max(1, std::atoi(e));
     if (C.tally)
         hipLaunchKernelGGL(dev::knn_cell_kernel<true>, dim3(grid), dim3(dev::kKnnWave * dev::kCellWaves), 0, s, C);
