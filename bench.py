@@ -1260,7 +1260,7 @@ def _serving_measure(ctx, served_cfg):
     for f in (0.5, 0.8, 1.0):
         sweep["plain"].append(run_trace(plain, f * cap_rate))
         log("plain", sweep["plain"][-1])
-    for f in (0.5, 0.8, 1.0, 1.25, 1.5, 1.75, 2.0, 2.5, 3.0, 4.0):
+    for f in (0.5, 0.8, 1.0, 1.25, 1.5, 1.75, 2.0, 2.25, 2.5, 2.75, 3.0, 3.25, 3.5, 4.0):
         sweep["served"].append(run_trace(served, f * cap_rate))
         log("served", sweep["served"][-1])
     served_cap = run_trace(served, float("inf"))
