@@ -224,3 +224,49 @@ def test_cells_counts(device):
         pm.radiance_counts(q, qn, 8, r, rgb_c)
     assert e.value.status == 4
     pm.destroy()
+
+
+@pytest.mark.parametrize("shape", ["plane", "patch", "volume"])
+def test_cell_blocks_and_their_fallbacks(device, shape):
+    """The 64-photon blocks of a dense grid and the two ways a query keeps its
+    cells instead: plane -- cells of hundreds of photons, a query's cells hold
+    tens of blocks (blocks ranked); patch -- 400,000 photons in a 0.2-wide
+    square, cells of thousands (more than 256 blocks: cells ranked); volume --
+    photons filling a cube, where more than 64 of a query's blocks often lie
+    within the radius (cells ranked for those queries).  Counts, radiance and
+    tie-aware lists equal the per-lane heap kernel's."""
+    rng = np.random.default_rng(17)
+    n, nq, r, k = 200000, 1200, 0.05, 64
+    if shape == "plane":
+        P = np.stack([rng.uniform(0, 1, n), rng.uniform(0, 1, n), np.zeros(n)], 1)
+        Q = np.stack([rng.uniform(0.05, 0.95, nq), rng.uniform(0.05, 0.95, nq), np.zeros(nq)], 1)
+    elif shape == "patch":
+        n = 400000
+        P = np.stack([rng.uniform(0.4, 0.6, n), rng.uniform(0.4, 0.6, n), np.zeros(n)], 1)
+        # a sparse background keeps the grid's box wide enough for 4 cells
+        P[:2000, 0:2] = rng.uniform(0, 1, (2000, 2))
+        Q = np.stack([rng.uniform(0.38, 0.62, nq), rng.uniform(0.38, 0.62, nq), np.zeros(nq)], 1)
+    else:
+        r = 0.1  # cells of about 200 photons; about 56 blocks reach a query's sphere
+        P = rng.uniform(0, 1, (n, 3))
+        Q = rng.uniform(0.1, 0.9, (nq, 3))
+    P, Q = P.astype(np.float32), Q.astype(np.float32)
+    N = np.tile(np.float32([0, 0, 1]), (len(P), 1))
+    QN = np.tile(np.float32([0, 0, 1]), (nq, 1))
+    W = rng.uniform(0.1, 1.0, (len(P), 3)).astype(np.float32)
+    pm = rq.PhotonMap(device, P, N, W)
+    rgb, cnt = _radiance(pm, Q, QN, k, r, False, False)
+    work = pm.radiance_counts(torch.from_numpy(Q).cuda(), torch.from_numpy(QN).cuda(), k, r,
+                              torch.empty((nq, 3), device="cuda"))
+    heap_rgb, heap_cnt = _heap(_radiance, pm, Q, QN, k, r, True, False)
+    log("case: blocks %s: %.1f ranges, %.0f photons read per query, mean count %.1f" % (
+        shape, work["ranges"] / nq, work["photons"] / nq, cnt.mean()))
+    assert np.array_equal(cnt, heap_cnt) and cnt.min() > 0
+    np.testing.assert_allclose(rgb, heap_rgb, rtol=1e-5, atol=1e-12)
+    gi, gd, gc = _gpu_knn(pm, Q[:300], QN[:300], k, r)
+    hi, hd, hc = _heap(_gpu_knn, pm, Q[:300], QN[:300], k, r)
+    assert np.array_equal(gc, hc)
+    for i in range(300):
+        c = int(gc[i])
+        assert same_knn_set(gi[i, :c], gd[i, :c], hi[i, :c], hd[i, :c]), i
+    pm.destroy()
