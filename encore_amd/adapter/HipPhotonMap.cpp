@@ -83,6 +83,45 @@ void HipPhotonMap::BalanceAndConstruct() {
     m_built = true;
 }
 
+// Every increment-th photon of ENCORE's balanced order gets the radiance
+// estimate at its own position and normal as its power (the engine picks
+// them and estimates on the device); the new map holds copies of those
+// photons with that power, added in that order and balanced, as
+// PhotonMap::PrecomputeIrradiance builds it.
+HipPhotonMap* HipPhotonMap::PrecomputeIrradiance(int increment, const int N, const float maxDistance) {
+    HipPhotonMap* out = new HipPhotonMap(m_max, m_ordinal, m_layout);
+    if (!m_built) BalanceAndConstruct();
+    if (!m_map || increment <= 0 || N <= 0) return out;
+    const size_t n = m_photons.size();
+    std::vector<float> pos(3 * n), nrm(3 * n);
+    for (size_t i = 0; i < n; ++i) {
+        const Point3& x = m_photons[i].Position();
+        const Vector3& s = m_photons[i].SurfNormal();
+        for (int k = 0; k < 3; ++k) {
+            pos[3 * i + k] = x[k];
+            nrm[3 * i + k] = s[k];
+        }
+    }
+    const size_t m = n ? (n - 1) / (size_t)increment + 1 : 0;
+    std::vector<uint32_t> chosen(std::max<size_t>(1, m));
+    std::vector<float> pw(3 * std::max<size_t>(1, m));
+    rq_photons* made = 0;
+    if (rq_photons_precompute_irradiance(m_map, pos.data(), nrm.data(), (uint32_t)n, (uint32_t)increment,
+                                         (uint32_t)N, maxDistance, m_caustic ? 1u : 0u, m_layout, chosen.data(),
+                                         pw.data(), &made) != RQ_OK) {
+        printf("HipPhotonMap: %s\n", rq_last_error());
+        return out;
+    }
+    rq_photons_destroy(made);  // the new map builds its own from its photons
+    for (size_t j = 0; j < m; ++j) {
+        Photon p = m_photons[chosen[j]];
+        p.Power(Color(pw[3 * j], pw[3 * j + 1], pw[3 * j + 2]));
+        out->AddPhoton(p);
+    }
+    out->BalanceAndConstruct();
+    return out;
+}
+
 void HipPhotonMap::gather(int N, float maxDistance, const float* loc, const float* nrm, size_t nq, bool radiance,
                           std::vector<uint32_t>& idx, std::vector<float>& d2, std::vector<uint32_t>& cnt,
                           std::vector<float>& rgb) {

@@ -9,6 +9,7 @@
 //   GetRadianceEstimate(...)  PhotonMap.cpp:121-164, caustic weighting when
 //                             SetCausticsMap(true)
 //   ScalePhotons(f)           scales stored power (before or after building)
+//   PrecomputeIrradiance(...) PhotonMap.cpp:57-74: a new map (caller owns it)
 // plus batched forms that take many gather points per call.
 #pragma once
 
@@ -31,6 +32,7 @@ public:
                                                    Vector3 normal);
     Color GetRadianceEstimate(const int N, const float maxDistance, Point3 location, Vector3 normal);
     void ScalePhotons(const float scaleFactor);
+    HipPhotonMap* PrecomputeIrradiance(int increment, const int N, const float maxDistance);
     bool Full() { return m_photons.size() == m_max; }
     int GetCountStored() { return (int)m_photons.size(); }
     size_t GetMaxPhotons() { return m_max; }

@@ -794,6 +794,53 @@ rq_status rq_radiance_counts(rq_photons* p, const float* d_qpos, const float* d_
     });
 }
 
+rq_status rq_photons_precompute_irradiance(rq_photons* src, const float* pos, const float* normal, uint32_t n,
+                                           uint32_t increment, uint32_t k, float max_dist, uint32_t caustic,
+                                           uint32_t layout, uint32_t* chosen, float* power_out, rq_photons** out) {
+    return guarded([&] {
+        require(src != nullptr && out != nullptr, "null argument");
+        require(n == src->n, "n must be the map's photon count");
+        require(n == 0 || (pos && normal), "null photon arrays");
+        require(increment >= 1, "increment must be at least 1");
+        require(layout == RQ_PHOTONS_ENCORE_ORDER || layout == RQ_PHOTONS_SPATIAL, "unknown photon layout");
+        require(k >= 1 && k <= kKnnMaxK, "k must be 1..256");
+        // m_photons after BalanceAndConstruct: ENCORE's balanced order, which
+        // depends on the positions only (whatever layout the map uses).
+        PhotonHost h;
+        if (n) {
+            const std::vector<float> zero(3 * (size_t)n, 0.0f);
+            build_photons(pos, zero.data(), nullptr, n, RQ_PHOTONS_ENCORE_ORDER, h);
+        }
+        const uint32_t m = n == 0 ? 0u : (n - 1) / increment + 1;
+        std::vector<uint32_t> pick(m);
+        std::vector<float> qp(3 * (size_t)m), qn(3 * (size_t)m), rgb(3 * (size_t)m, 0.0f);
+        for (uint32_t j = 0; j < m; ++j) {
+            const uint32_t i = h.perm[(size_t)j * increment];
+            pick[j] = i;
+            for (int a = 0; a < 3; ++a) {
+                qp[3 * (size_t)j + a] = pos[3 * (size_t)i + a];
+                qn[3 * (size_t)j + a] = normal[3 * (size_t)i + a];
+            }
+        }
+        if (m) {
+            RQ_HIP(hipSetDevice(src->dev->ordinal));
+            const size_t bytes = 3 * (size_t)m * sizeof(float);
+            DevBuf dq(bytes), dn(bytes), drgb(bytes);
+            dq.upload(qp.data(), bytes);
+            dn.upload(qn.data(), bytes);
+            knn_common(src, dq.as<float>(), dn.as<float>(), m, k, max_dist, nullptr, nullptr, nullptr, drgb.as<float>(),
+                       caustic ? 1u : 0u, nullptr);
+            RQ_HIP(hipMemcpy(rgb.data(), drgb.ptr, bytes, hipMemcpyDeviceToHost));
+        }
+        rq_photons* made = nullptr;
+        const rq_status st = rq_photons_create(src->dev, qp.data(), qn.data(), rgb.data(), m, layout, &made);
+        if (st != RQ_OK) throw Error(st, rq_last_error());
+        if (chosen && m) std::memcpy(chosen, pick.data(), m * sizeof(uint32_t));
+        if (power_out && m) std::memcpy(power_out, rgb.data(), 3 * (size_t)m * sizeof(float));
+        *out = made;
+    });
+}
+
 rq_status rq_final_gather(rq_scene* scene, rq_photons* indirect, uint32_t k_ind, float r_ind, rq_photons* direct,
                           uint32_t k_dir, float r_dir, const float* d_diffuse, const float* d_origin,
                           uint32_t npoints, const float* d_dirs, uint32_t nper, float* d_rgb, float* d_ri,

@@ -149,6 +149,9 @@ def load():
     L.rq_radiance_estimate.argtypes = [_vp, _vp, _vp, _u32, _u32, ctypes.c_float, _u32, _vp, _vp, _vp, _vp, _vp]
     L.rq_radiance_counts.argtypes = [_vp, _vp, _vp, _u32, _u32, ctypes.c_float, _u32, _vp, _vp, _vp]
     L.rq_radiance_counts.restype = ctypes.c_int
+    L.rq_photons_precompute_irradiance.argtypes = [_vp, _vp, _vp, _u32, _u32, _u32, ctypes.c_float, _u32, _u32, _vp,
+                                                   _vp, _vp]
+    L.rq_photons_precompute_irradiance.restype = ctypes.c_int
     if hasattr(L, "rq_scene_save"):
         L.rq_scene_save.argtypes = [_vp, ctypes.c_char_p]
         L.rq_scene_save.restype = ctypes.c_int
@@ -571,6 +574,25 @@ class PhotonMap:
         check(load().rq_radiance_estimate(self.h, _ptr(qpos), _ptr(qnrm), qpos.shape[0], k, float(max_dist),
                                           int(caustic), _ptr(rgb), _ptr(idx), _ptr(d2), _ptr(count),
                                           _stream(stream)))
+
+    def precompute_irradiance(self, pos, normal, increment, k, max_dist, caustic=False, layout="spatial"):
+        """PhotonMap::PrecomputeIrradiance: (new PhotonMap, insertion indices
+        of its photons, their powers).  pos, normal: this map's photons in
+        insertion order (host)."""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        normal = np.ascontiguousarray(normal, np.float32).reshape(-1, 3)
+        m = 0 if self.n == 0 or int(increment) < 1 else (self.n - 1) // int(increment) + 1  # the library checks
+        chosen = np.zeros(max(1, m), np.uint32)
+        power = np.zeros((max(1, m), 3), np.float32)
+        h = _vp()
+        check(load().rq_photons_precompute_irradiance(self.h, pos.ctypes.data, normal.ctypes.data, len(pos),
+                                                      int(increment), int(k), float(max_dist), int(caustic),
+                                                      PHOTON_LAYOUT[layout], chosen.ctypes.data, power.ctypes.data,
+                                                      ctypes.byref(h)))
+        made = PhotonMap.__new__(PhotonMap)
+        made.h = h
+        made.n = m
+        return made, chosen[:m], power[:m]
 
     def radiance_counts(self, qpos, qnrm, k, max_dist, rgb, caustic=False, stream=None):
         """Radiance estimates into rgb, with the gather's work counted:

@@ -376,6 +376,21 @@ rq_status rq_radiance_counts(rq_photons* photons, const float* d_qpos, const flo
                              uint32_t k, float max_dist, uint32_t caustic, float* d_rgb, uint64_t* counts,
                              void* stream);
 
+/* PhotonMap::PrecomputeIrradiance (PhotonMap.cpp:57-74): of the map's
+ * photons in ENCORE's balanced order (m_photons after BalanceAndConstruct,
+ * which depends on the positions only), every increment-th from the first
+ * gets the radiance estimate of the map at its own position and normal
+ * (GetRadianceEstimate with k, max_dist and the caustic weighting when
+ * caustic != 0) as its power, and these m = ceil(n / increment) photons, in
+ * that order, make a new map *out with the given layout.  pos and normal are
+ * the map's photons in insertion order (host, 3*n floats each, as given to
+ * rq_photons_create).  chosen[m] (host, may be NULL) receives the insertion
+ * index of each new photon, power_out[3*m] (host, may be NULL) its power.
+ * Synchronous. */
+rq_status rq_photons_precompute_irradiance(rq_photons* photons, const float* pos, const float* normal, uint32_t n,
+                                           uint32_t increment, uint32_t k, float max_dist, uint32_t caustic,
+                                           uint32_t layout, uint32_t* chosen, float* power_out, rq_photons** out);
+
 /* Final gather: PhotonMapperCPU::CalculateIndirectIllumination's
  * final-gather branch (PhotonMapperCPU.cpp:1110-1167) for npoints shading
  * points at once.  For point i (origin d_origin[3i..3i+2]) and each of its
