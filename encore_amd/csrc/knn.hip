@@ -609,6 +609,10 @@ constexpr int kCellCand = 2 * (int)kKnnMaxK + 64;
 constexpr int kCellWaves = 4;
 // Blocks a query may examine before it keeps to whole cells.
 constexpr uint32_t kCellMaxBlocks = 256;
+// Ranges a query may rank: two per lane (slots lane and lane + 64).  Whole
+// cells need at most 64 (4 x 4 x 4); blocks within the radius often number
+// 64-100 where the photons are dense, and a query with more keeps to cells.
+constexpr uint32_t kCellMaxRanges = 128;
 
 // COUNT: the instrumented copy tallies, per query, the ranges ranked (cells,
 // or blocks), the photons read and the candidates (L.tally[0..3]: queries,
@@ -618,8 +622,8 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
     __shared__ uint32_t s_key_w[kCellWaves][kCellCand];
     __shared__ uint32_t s_slot_w[kCellWaves][kCellCand];
     __shared__ uint32_t s_hist_w[kCellWaves][256];
-    __shared__ uint32_t s_first_w[kCellWaves][64], s_pre_w[kCellWaves][64];
-    __shared__ float s_near_w[kCellWaves][64];
+    __shared__ uint32_t s_first_w[kCellWaves][kCellMaxRanges], s_pre_w[kCellWaves][kCellMaxRanges];
+    __shared__ float s_near_w[kCellWaves][kCellMaxRanges];
     __shared__ uint32_t s_mark_w[kCellWaves][64];
     const uint32_t wv = threadIdx.x / kKnnWave;
     uint32_t* s_key = s_key_w[wv];
@@ -680,7 +684,9 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
             }
             near2 *= 1.0 - 0x1p-20;
             v = v && near2 < (double)r2 && !far;
-            uint2 range = make_uint2(0u, 0u);
+            uint2 range = make_uint2(0u, 0u), range1 = make_uint2(0u, 0u);  // slots lane, lane + 64
+            double near1 = 0.0;
+            bool v1 = false;
             uint32_t fb = 0;  // the cell's first block (grids with blocks)
             if (v) {
                 if (G.dense) {
@@ -696,8 +702,8 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
                 // Blocks instead of cells: when the kept cells hold at most
                 // kCellMaxBlocks blocks and some cell holds more than one,
                 // the wave examines them 64 at a time and keeps those whose
-                // box lies within the radius; if at most 64 remain, they
-                // replace the cells.  Block j's owner is the last lane whose
+                // box lies within the radius; if at most kCellMaxRanges
+                // remain, they replace the cells.  Block j's owner is the last lane whose
                 // first block number is at or below j (a lane with no block
                 // shares its start with the next lane that has one).
                 const uint32_t nb = v ? (range.y + 63u) >> 6 : 0u;
@@ -741,7 +747,7 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
                         }
                         const uint64_t km = __ballot(keep);
                         const uint32_t cnt = (uint32_t)__popcll(km);
-                        if (nsurv + cnt > 64u) {
+                        if (nsurv + cnt > kCellMaxRanges) {
                             fits = false;
                             break;
                         }
@@ -756,42 +762,66 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
                     if (fits) {
                         __builtin_amdgcn_wave_barrier();
                         v = lane < nsurv;
+                        v1 = lane + 64u < nsurv;
                         if (v) {
                             range = make_uint2(s_first[lane], s_pre[lane]);
                             near2 = (double)s_near[lane];
+                        }
+                        if (v1) {
+                            range1 = make_uint2(s_first[lane + 64u], s_pre[lane + 64u]);
+                            near1 = (double)s_near[lane + 64u];
                         }
                         __builtin_amdgcn_wave_barrier();
                     }
                 }
             }
             // Nearest first.  The bound is kept as a float rounded down with
-            // its 6 low bits cleared (still a lower bound), and the lane in
+            // its 7 low bits cleared (still a lower bound), and the slot in
             // those bits makes the 32-bit sort keys distinct.
-            const uint32_t nb = __float_as_uint(__double2float_rd(near2)) & ~63u;
-            const uint32_t key = nb | lane;
-            const uint64_t vm = __ballot(v);
-            uint32_t rank = 0;
+            const uint32_t nb = __float_as_uint(__double2float_rd(near2)) & ~127u;
+            const uint32_t nb1 = __float_as_uint(__double2float_rd(near1)) & ~127u;
+            const uint32_t key = nb | lane, key1 = nb1 | (lane + 64u);
+            const uint64_t vm = __ballot(v), vm1 = __ballot(v1);
+            uint32_t rank = 0, rank1 = 0;
             for (uint64_t rest = vm; rest; rest &= rest - 1) {
-                const int j = __ffsll((unsigned long long)rest) - 1;
-                rank += (uint32_t)__shfl((int)key, j, 64) < key ? 1u : 0u;
+                const uint32_t kj = __builtin_amdgcn_readlane(key, __ffsll((unsigned long long)rest) - 1);
+                rank += kj < key ? 1u : 0u;
+                rank1 += kj < key1 ? 1u : 0u;
             }
-            const uint32_t ncell = (uint32_t)__popcll(vm);
+            for (uint64_t rest = vm1; rest; rest &= rest - 1) {
+                const uint32_t kj = __builtin_amdgcn_readlane(key1, __ffsll((unsigned long long)rest) - 1);
+                rank += kj < key ? 1u : 0u;
+                rank1 += kj < key1 ? 1u : 0u;
+            }
+            const uint32_t ncell = (uint32_t)__popcll(vm) + (uint32_t)__popcll(vm1);
             if (v) {
                 s_first[rank] = range.x;
                 s_pre[rank] = range.y;
                 s_near[rank] = __uint_as_float(nb);
             }
-            __builtin_amdgcn_wave_barrier();
-            // photon counts -> exclusive prefix, in rank order
-            const uint32_t cnt = lane < ncell ? s_pre[lane] : 0u;
-            uint32_t incl = cnt;
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t t = __shfl_up(incl, o, 64);
-                if (lane >= (uint32_t)o) incl += t;
+            if (v1) {
+                s_first[rank1] = range1.x;
+                s_pre[rank1] = range1.y;
+                s_near[rank1] = __uint_as_float(nb1);
             }
-            const uint32_t total = __shfl(incl, 63, 64);
+            __builtin_amdgcn_wave_barrier();
+            // photon counts -> exclusive prefix, in rank order (slots lane,
+            // then lane + 64 after the first 64 in total)
+            const uint32_t cnt = lane < ncell ? s_pre[lane] : 0u;
+            const uint32_t cnt1 = lane + 64u < ncell ? s_pre[lane + 64u] : 0u;
+            uint32_t incl = cnt, incl1 = cnt1;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t t = __shfl_up(incl, o, 64), t1 = __shfl_up(incl1, o, 64);
+                if (lane >= (uint32_t)o) {
+                    incl += t;
+                    incl1 += t1;
+                }
+            }
+            const uint32_t total0 = __shfl(incl, 63, 64);
+            const uint32_t total = total0 + __shfl(incl1, 63, 64);
             __builtin_amdgcn_wave_barrier();
             if (lane < ncell) s_pre[lane] = incl - cnt;
+            if (lane + 64u < ncell) s_pre[lane + 64u] = total0 + incl1 - cnt1;
             __builtin_amdgcn_wave_barrier();
             // Chunk e0 covers elements e0 .. e0 + 63 of the concatenated
             // ranges; its photons (position, class, slot) are loaded one
@@ -804,6 +834,7 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
             // marks their starts in LDS and takes a running maximum over the
             // lanes, carried from the previous chunk's last element.
             const uint32_t my_start = lane < ncell ? s_pre[lane] : 0xFFFFFFFFu;
+            const uint32_t my_start1 = lane + 64u < ncell ? s_pre[lane + 64u] : 0xFFFFFFFFu;
             uint32_t ci = 0, carry = 0;  // this lane's cell; the cell of the chunk's previous element
             float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             uint32_t sl = 0;
@@ -811,13 +842,16 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
             auto fetch = [&](uint32_t e0, float4& fp, uint32_t& fs) -> uint32_t {
                 const uint32_t e = e0 + lane;
                 const bool starts_here = lane < ncell && my_start - e0 < 64u;
-                if (__popcll(__ballot(starts_here)) <= 3) {
+                const bool starts_here1 = lane + 64u < ncell && my_start1 - e0 < 64u;
+                if (__popcll(__ballot(starts_here)) + __popcll(__ballot(starts_here1)) <= 3) {
                     if (e < total)
                         while (ci + 1 < ncell && s_pre[ci + 1] <= e) ++ci;
                 } else {
                     s_mark[lane] = 0u;
                     __builtin_amdgcn_wave_barrier();
+                    // ranges are not empty: no two start at one element
                     if (starts_here) s_mark[my_start - e0] = lane;
+                    if (starts_here1) s_mark[my_start1 - e0] = lane + 64u;
                     __builtin_amdgcn_wave_barrier();
                     uint32_t c = max(s_mark[lane], carry);
                     for (int o = 1; o < 64; o <<= 1) {

@@ -25,9 +25,10 @@
 // at most kGridDenseMaxBytes (one load per cell, no hashing or probing),
 // otherwise an open-addressing hash table keyed by the cell's Morton code.
 //
-// A query whose cells hold at most 64 blocks ranks the blocks instead of the
-// cells: along the sub-cell curve a block covers a small part of its cell, so
-// once the k-th distance has shrunk below a block's box the block is skipped.
+// A query whose cells hold at most 256 blocks, at most 128 of them within the
+// radius, ranks those blocks instead of the cells: along the sub-cell curve a
+// block covers a small part of its cell, so once the k-th distance has shrunk
+// below a block's box the block is skipped.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
