@@ -596,6 +596,50 @@ __device__ __forceinline__ uint2 cell_lookup(const PhotonGridView& G, uint64_t k
     return make_uint2(0u, 0u);
 }
 
+// The value held by lane (lane ^ J) of the wave: DPP quad permutes for
+// J = 1, 2, LDS swizzles within 32 lanes for J = 4 .. 16, a permute for 32.
+template <uint32_t J>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t x) {
+    if constexpr (J == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
+    else if constexpr (J == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);
+    else if constexpr (J <= 16) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x1F | (J << 10));
+    else return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((__lane_id() ^ J) << 2), (int)x);
+}
+
+// One compare-exchange step of a bitonic sort: element e (lane e, or lane
+// e - 64 of the second register) and element e ^ J, within a run of SIZE
+// elements sorted ascending when e & SIZE is 0 (always, for SIZE = 128).
+template <uint32_t SIZE, uint32_t J>
+__device__ __forceinline__ uint32_t bitonic_step(uint32_t x, uint32_t e) {
+    const uint32_t p = lane_xor<J>(x);
+    const bool up = SIZE >= 128 || (e & SIZE) == 0;
+    return ((e & J) == 0) == up ? min(x, p) : max(x, p);
+}
+
+template <uint32_t SIZE, uint32_t J>
+__device__ __forceinline__ void bitonic_merge(uint32_t& x0, uint32_t& x1, uint32_t lane, bool two) {
+    if constexpr (J == 64) {
+        const uint32_t lo = min(x0, x1), hi = max(x0, x1);
+        x0 = lo;
+        x1 = hi;
+    } else {
+        x0 = bitonic_step<SIZE, J>(x0, lane);
+        if (two) x1 = bitonic_step<SIZE, J>(x1, lane + 64u);
+    }
+    if constexpr (J > 1) bitonic_merge<SIZE, J / 2>(x0, x1, lane, two);
+}
+
+// Sorts 64 keys (one per lane) ascending, or 128 (x0 then x1) when `two`
+// (wave-uniform).
+template <uint32_t SIZE = 2>
+__device__ __forceinline__ void bitonic_sort(uint32_t& x0, uint32_t& x1, uint32_t lane, bool two) {
+    if constexpr (SIZE <= 128) {
+        if (SIZE == 128 && !two) return;
+        bitonic_merge<SIZE, SIZE / 2>(x0, x1, lane, two);
+        bitonic_sort<SIZE * 2>(x0, x1, lane, two);
+    }
+}
+
 // Candidates: compaction at max(2k, k + 64) <= 512 for k <= 256, plus one
 // chunk of 64.
 constexpr int kCellCand = 2 * (int)kKnnMaxK + 64;
@@ -613,6 +657,9 @@ constexpr uint32_t kCellMaxBlocks = 256;
 // cells need at most 64 (4 x 4 x 4); blocks within the radius often number
 // 64-100 where the photons are dense, and a query with more keeps to cells.
 constexpr uint32_t kCellMaxRanges = 128;
+// Up to this many ranges are ranked by counting the smaller keys (a few
+// instructions per range); more are sorted (about 27 exchange steps).
+constexpr uint32_t kCellRankByCount = 16;
 
 // COUNT: the instrumented copy tallies, per query, the ranges ranked (cells,
 // or blocks), the photons read and the candidates (L.tally[0..3]: queries,
@@ -782,18 +829,28 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
             const uint32_t nb1 = __float_as_uint(__double2float_rd(near1)) & ~127u;
             const uint32_t key = nb | lane, key1 = nb1 | (lane + 64u);
             const uint64_t vm = __ballot(v), vm1 = __ballot(v1);
-            uint32_t rank = 0, rank1 = 0;
-            for (uint64_t rest = vm; rest; rest &= rest - 1) {
-                const uint32_t kj = __builtin_amdgcn_readlane(key, __ffsll((unsigned long long)rest) - 1);
-                rank += kj < key ? 1u : 0u;
-                rank1 += kj < key1 ? 1u : 0u;
-            }
-            for (uint64_t rest = vm1; rest; rest &= rest - 1) {
-                const uint32_t kj = __builtin_amdgcn_readlane(key1, __ffsll((unsigned long long)rest) - 1);
-                rank += kj < key ? 1u : 0u;
-                rank1 += kj < key1 ? 1u : 0u;
-            }
             const uint32_t ncell = (uint32_t)__popcll(vm) + (uint32_t)__popcll(vm1);
+            uint32_t rank = 0, rank1 = 0;
+            if (ncell <= kCellRankByCount) {
+                // few ranges (slots of the first register only): count the
+                // smaller keys
+                for (uint64_t rest = vm; rest; rest &= rest - 1) {
+                    const uint32_t kj = __builtin_amdgcn_readlane(key, __ffsll((unsigned long long)rest) - 1);
+                    rank += kj < key ? 1u : 0u;
+                }
+            } else {
+                // Sort the keys (unused slots as ~0, last); the lane holding
+                // sorted position e tells slot (key & 127) its rank e through
+                // s_hist, which is free until the first compaction.
+                uint32_t x0 = v ? key : 0xFFFFFFFFu, x1 = v1 ? key1 : 0xFFFFFFFFu;
+                bitonic_sort(x0, x1, lane, vm1 != 0);
+                if (lane < ncell) s_hist[x0 & 127u] = lane;
+                if (lane + 64u < ncell) s_hist[x1 & 127u] = lane + 64u;
+                __builtin_amdgcn_wave_barrier();
+                rank = s_hist[lane];
+                rank1 = s_hist[lane + 64u];
+                __builtin_amdgcn_wave_barrier();
+            }
             if (v) {
                 s_first[rank] = range.x;
                 s_pre[rank] = range.y;

@@ -203,7 +203,7 @@ def test_cells_counts(device):
     estimates, one query each, and -- with fewer than k photons in range, so
     the candidate list is never cut -- exactly the brute-force number of
     photons in range as candidates, out of at least as many photons read
-    from at most 64 ranges (cells or blocks) per query.  A k or radius the grid does not serve
+    from at most 128 ranges (cells or blocks) per query.  A k or radius the grid does not serve
     is refused."""
     rng = np.random.default_rng(13)
     P, N, W, Q, QN = _surface_photons(rng, 40000, 0.0)
@@ -219,25 +219,30 @@ def test_cells_counts(device):
     assert np.array_equal(rgb_c.cpu().numpy().view(np.uint32), rgb.view(np.uint32))
     assert work["queries"] == len(Q)
     assert work["candidates"] == int(want.sum()) == int(cnt.sum())
-    assert work["photons"] >= work["candidates"] and 0 < work["ranges"] <= 64 * len(Q)
+    assert work["photons"] >= work["candidates"] and 0 < work["ranges"] <= 128 * len(Q)
     with pytest.raises(rq.RQError) as e:
         pm.radiance_counts(q, qn, 8, r, rgb_c)
     assert e.value.status == 4
     pm.destroy()
 
 
-@pytest.mark.parametrize("shape", ["plane", "patch", "volume"])
+@pytest.mark.parametrize("shape", ["plane", "dense_plane", "patch", "volume", "dense_volume"])
 def test_cell_blocks_and_their_fallbacks(device, shape):
     """The 64-photon blocks of a dense grid and the two ways a query keeps its
     cells instead: plane -- cells of hundreds of photons, a query's cells hold
-    tens of blocks (blocks ranked); patch -- 400,000 photons in a 0.2-wide
-    square, cells of thousands (more than 256 blocks: cells ranked); volume --
-    photons filling a cube, where more than 64 of a query's blocks often lie
-    within the radius (cells ranked for those queries).  Counts, radiance and
+    tens of blocks (blocks ranked, sorted in one register); dense plane --
+    cells of about 1,000, so that most queries keep 65 to 128 blocks (sorted
+    in two registers); patch -- 400,000 photons in a 0.2-wide square, cells of
+    thousands (more than 256 blocks: cells ranked); volume -- photons filling
+    a cube, about 56 blocks within a query's sphere; dense volume -- three
+    times as many: about 70 % of the queries rank 65 to 128 blocks, most of
+    the others keep more than 128 within the radius and a few hold more than
+    256 in their cells (cells ranked for those).  Counts, radiance and
     tie-aware lists equal the per-lane heap kernel's."""
     rng = np.random.default_rng(17)
     n, nq, r, k = 200000, 1200, 0.05, 64
-    if shape == "plane":
+    if shape in ("plane", "dense_plane"):
+        n = 400000 if shape == "dense_plane" else n
         P = np.stack([rng.uniform(0, 1, n), rng.uniform(0, 1, n), np.zeros(n)], 1)
         Q = np.stack([rng.uniform(0.05, 0.95, nq), rng.uniform(0.05, 0.95, nq), np.zeros(nq)], 1)
     elif shape == "patch":
@@ -248,6 +253,7 @@ def test_cell_blocks_and_their_fallbacks(device, shape):
         Q = np.stack([rng.uniform(0.38, 0.62, nq), rng.uniform(0.38, 0.62, nq), np.zeros(nq)], 1)
     else:
         r = 0.1  # cells of about 200 photons; about 56 blocks reach a query's sphere
+        n = 600000 if shape == "dense_volume" else n
         P = rng.uniform(0, 1, (n, 3))
         Q = rng.uniform(0.1, 0.9, (nq, 3))
     P, Q = P.astype(np.float32), Q.astype(np.float32)
@@ -263,6 +269,8 @@ def test_cell_blocks_and_their_fallbacks(device, shape):
         shape, work["ranges"] / nq, work["photons"] / nq, cnt.mean()))
     assert np.array_equal(cnt, heap_cnt) and cnt.min() > 0
     np.testing.assert_allclose(rgb, heap_rgb, rtol=1e-5, atol=1e-12)
+    if shape == "dense_plane":
+        assert work["ranges"] > 64 * nq  # most queries rank more than 64 blocks
     gi, gd, gc = _gpu_knn(pm, Q[:300], QN[:300], k, r)
     hi, hd, hc = _heap(_gpu_knn, pm, Q[:300], QN[:300], k, r)
     assert np.array_equal(gc, hc)
