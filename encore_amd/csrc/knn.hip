@@ -380,9 +380,95 @@ __device__ __forceinline__ uint32_t kept_max(const uint32_t* s_key, uint32_t n) 
     return m;
 }
 
+// The value held by lane (lane ^ J) of the wave: DPP quad permutes for
+// J = 1, 2, LDS swizzles within 32 lanes for J = 4 .. 16, a permute for 32.
+template <uint32_t J>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t x) {
+    if constexpr (J == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
+    else if constexpr (J == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);
+    else if constexpr (J <= 16) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x1F | (J << 10));
+    else return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((__lane_id() ^ J) << 2), (int)x);
+}
+
+template <uint32_t J>
+__device__ __forceinline__ uint64_t lane_xor(uint64_t x) {
+    return (uint64_t)lane_xor<J>((uint32_t)x) | ((uint64_t)lane_xor<J>((uint32_t)(x >> 32)) << 32);
+}
+
+template <uint32_t J>
+__device__ __forceinline__ double lane_xor(double x) {
+    return __longlong_as_double((long long)lane_xor<J>((uint64_t)__double_as_longlong(x)));
+}
+
+// One compare-exchange step of a bitonic sort: element e (lane e, or lane
+// e - 64 of the second register) and element e ^ J, within a run of SIZE
+// elements sorted ascending when e & SIZE is 0 (always, for SIZE = 128).
+template <uint32_t SIZE, uint32_t J, typename T>
+__device__ __forceinline__ T bitonic_step(T x, uint32_t e) {
+    const T p = lane_xor<J>(x);
+    const bool up = SIZE >= 128 || (e & SIZE) == 0;
+    return ((e & J) == 0) == up ? min(x, p) : max(x, p);
+}
+
+template <uint32_t SIZE, uint32_t J, typename T>
+__device__ __forceinline__ void bitonic_merge(T& x0, T& x1, uint32_t lane, bool two) {
+    if constexpr (J == 64) {
+        const T lo = min(x0, x1), hi = max(x0, x1);
+        x0 = lo;
+        x1 = hi;
+    } else {
+        x0 = bitonic_step<SIZE, J>(x0, lane);
+        if (two) x1 = bitonic_step<SIZE, J>(x1, lane + 64u);
+    }
+    if constexpr (J > 1) bitonic_merge<SIZE, J / 2>(x0, x1, lane, two);
+}
+
+// Sorts 64 keys (one per lane) ascending, or 128 (x0 then x1) when `two`
+// (wave-uniform).
+template <uint32_t SIZE = 2, typename T>
+__device__ __forceinline__ void bitonic_sort(T& x0, T& x1, uint32_t lane, bool two) {
+    if constexpr (SIZE <= 128) {
+        if (SIZE == 128 && !two) return;
+        bitonic_merge<SIZE, SIZE / 2>(x0, x1, lane, two);
+        bitonic_sort<SIZE * 2>(x0, x1, lane, two);
+    }
+}
+
+// Butterfly sum over the wave (lane ^ 32 first): every lane gets the same
+// total, in the same order of additions.
 __device__ __forceinline__ double wave_sum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    v += lane_xor<32>(v);
+    v += lane_xor<16>(v);
+    v += lane_xor<8>(v);
+    v += lane_xor<4>(v);
+    v += lane_xor<2>(v);
+    v += lane_xor<1>(v);
     return v;
+}
+
+// Lane 0 writes the count and the radiance of the wave's query from the
+// wave's sums: the caustic filter's density factor applied
+// (PhotonMap.cpp:156-161), NaN when the walk overflowed.
+__device__ __forceinline__ void wave_radiance_out(const KnnLaunch& L, uint32_t q, uint32_t cand, bool overflow,
+                                                  double rmax, double R, double G, double B) {
+    if (__lane_id() != 0) return;
+    float r = 0.0f, g = 0.0f, b = 0.0f;
+    if (cand) {
+        r = (float)R;
+        g = (float)G;
+        b = (float)B;
+        if (L.caustic) {
+            const float dens = (float)((double)L.k / ((double)3.14159f * rmax * rmax));
+            r *= dens;
+            g *= dens;
+            b *= dens;
+        }
+    }
+    if (overflow) r = g = b = __int_as_float(0x7fc00000);
+    if (L.out_cnt) L.out_cnt[q] = cand;
+    L.out_rgb[3 * (size_t)q] = r;
+    L.out_rgb[3 * (size_t)q + 1] = g;
+    L.out_rgb[3 * (size_t)q + 2] = b;
 }
 
 // The end of a one-query-per-wave gather: keeps the k nearest of the `cand`
@@ -394,10 +480,49 @@ __device__ void wave_finish(const KnnLaunch& L, uint32_t q, uint32_t cand, bool 
                             uint32_t* s_slot, uint32_t* s_hist) {
     const uint32_t lane = __lane_id();
     const uint32_t k = L.k;
-    if (cand > k) {
+    if (cand > (k <= 64u ? 128u : k)) {
         uint32_t need = k;
         const uint32_t t = select_kth(s_key, cand, need, s_hist);
         cand = compact_kth(s_key, s_slot, cand, t, need);
+    }
+    if (k <= 64u) {
+        // At most 128 candidates: sort (squared distance, buffer position)
+        // and keep the first k -- the same photons as the selection above
+        // (ties at the k-th distance: those found first), already in list
+        // order.  Sorted position `lane` holds the lane's photon.
+        uint64_t x0 = lane < cand ? ((uint64_t)s_key[lane] << 7) | lane : ~0ull;
+        uint64_t x1 = lane + 64u < cand ? ((uint64_t)s_key[lane + 64u] << 7) | (lane + 64u) : ~0ull;
+        bitonic_sort(x0, x1, lane, cand > 64u);
+        const uint32_t kept = min(cand, k);
+        const uint32_t key = (uint32_t)(x0 >> 7);
+        const uint32_t slot = lane < kept ? s_slot[(uint32_t)x0 & 127u] : 0u;
+        if (L.out_idx) {
+            const size_t at = (size_t)q * k + lane;
+            if (lane < kept) {
+                L.out_idx[at] = L.perm[slot];
+                L.out_d2[at] = __uint_as_float(key);
+            } else if (lane < k) {
+                L.out_idx[at] = 0xFFFFFFFFu;
+                L.out_d2[at] = __int_as_float(0x7f800000);
+            }
+        }
+        if (!L.out_rgb) {
+            if (lane == 0 && L.out_cnt) L.out_cnt[q] = kept;
+            return;
+        }
+        // radiance over the kept set (GetRadianceEstimate, PhotonMap.cpp:121-164)
+        const double rmax = kept ? (double)__uint_as_float(__builtin_amdgcn_readlane(key, kept - 1u)) : 0.0;
+        double R = 0.0, G = 0.0, B = 0.0;
+        if (lane < kept) {
+            const float4 pw = *reinterpret_cast<const float4*>(L.power + slot);
+            const double dd = (double)__uint_as_float(key);
+            const double wgt = L.caustic ? (rmax - dd) / dd : 1.0;
+            R = (double)pw.x * wgt;
+            G = (double)pw.y * wgt;
+            B = (double)pw.z * wgt;
+        }
+        wave_radiance_out(L, q, kept, overflow, rmax, wave_sum(R), wave_sum(G), wave_sum(B));
+        return;
     }
     if (L.out_idx) {
         // the list, nearest first: the rank of each kept photon by (d2, buffer position)
@@ -435,28 +560,7 @@ __device__ void wave_finish(const KnnLaunch& L, uint32_t q, uint32_t cand, bool 
         G += (double)pw.y * wgt;
         B += (double)pw.z * wgt;
     }
-    R = wave_sum(R);
-    G = wave_sum(G);
-    B = wave_sum(B);
-    if (lane == 0) {
-        float r = 0.0f, g = 0.0f, b = 0.0f;
-        if (cand) {
-            r = (float)R;
-            g = (float)G;
-            b = (float)B;
-            if (L.caustic) {
-                const float dens = (float)((double)k / ((double)3.14159f * rmax * rmax));
-                r *= dens;
-                g *= dens;
-                b *= dens;
-            }
-        }
-        if (overflow) r = g = b = __int_as_float(0x7fc00000);
-        if (L.out_cnt) L.out_cnt[q] = cand;
-        L.out_rgb[3 * (size_t)q] = r;
-        L.out_rgb[3 * (size_t)q + 1] = g;
-        L.out_rgb[3 * (size_t)q + 2] = b;
-    }
+    wave_radiance_out(L, q, cand, overflow, rmax, wave_sum(R), wave_sum(G), wave_sum(B));
 }
 
 __global__ __launch_bounds__(64) void knn_wave_kernel(KnnLaunch L, uint32_t stack_cap) {
@@ -594,50 +698,6 @@ __device__ __forceinline__ uint2 cell_lookup(const PhotonGridView& G, uint64_t k
         h = (h + 1) & G.cell_mask;
     }
     return make_uint2(0u, 0u);
-}
-
-// The value held by lane (lane ^ J) of the wave: DPP quad permutes for
-// J = 1, 2, LDS swizzles within 32 lanes for J = 4 .. 16, a permute for 32.
-template <uint32_t J>
-__device__ __forceinline__ uint32_t lane_xor(uint32_t x) {
-    if constexpr (J == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
-    else if constexpr (J == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);
-    else if constexpr (J <= 16) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x1F | (J << 10));
-    else return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((__lane_id() ^ J) << 2), (int)x);
-}
-
-// One compare-exchange step of a bitonic sort: element e (lane e, or lane
-// e - 64 of the second register) and element e ^ J, within a run of SIZE
-// elements sorted ascending when e & SIZE is 0 (always, for SIZE = 128).
-template <uint32_t SIZE, uint32_t J>
-__device__ __forceinline__ uint32_t bitonic_step(uint32_t x, uint32_t e) {
-    const uint32_t p = lane_xor<J>(x);
-    const bool up = SIZE >= 128 || (e & SIZE) == 0;
-    return ((e & J) == 0) == up ? min(x, p) : max(x, p);
-}
-
-template <uint32_t SIZE, uint32_t J>
-__device__ __forceinline__ void bitonic_merge(uint32_t& x0, uint32_t& x1, uint32_t lane, bool two) {
-    if constexpr (J == 64) {
-        const uint32_t lo = min(x0, x1), hi = max(x0, x1);
-        x0 = lo;
-        x1 = hi;
-    } else {
-        x0 = bitonic_step<SIZE, J>(x0, lane);
-        if (two) x1 = bitonic_step<SIZE, J>(x1, lane + 64u);
-    }
-    if constexpr (J > 1) bitonic_merge<SIZE, J / 2>(x0, x1, lane, two);
-}
-
-// Sorts 64 keys (one per lane) ascending, or 128 (x0 then x1) when `two`
-// (wave-uniform).
-template <uint32_t SIZE = 2>
-__device__ __forceinline__ void bitonic_sort(uint32_t& x0, uint32_t& x1, uint32_t lane, bool two) {
-    if constexpr (SIZE <= 128) {
-        if (SIZE == 128 && !two) return;
-        bitonic_merge<SIZE, SIZE / 2>(x0, x1, lane, two);
-        bitonic_sort<SIZE * 2>(x0, x1, lane, two);
-    }
 }
 
 // Candidates: compaction at max(2k, k + 64) <= 512 for k <= 256, plus one
