@@ -673,18 +673,22 @@ __global__ __launch_bounds__(64) void knn_wave_kernel(KnnLaunch L, uint32_t stac
     }
 }
 
-// ----- large k, radiance only, from the photon cell grid -------------------
+// ----- k >= 32, lists and/or radiance, from the photon cell grid ----------
 // One query per wave, as knn_wave_kernel, but the candidates come from the
 // cells within reach of the query instead of a tree walk:
 //  - lane j takes cell (j & 3, (j >> 2) & 3, j >> 4) of the 4 x 4 x 4 block
 //    that covers [q - reach, q + reach] (the cell edge is the reach, so no
 //    axis spans more than 4 cells), drops it when it is outside the grid or
 //    farther than the radius, and looks up its photon range;
-//  - the cells are ordered nearest first, and their photons are read 64 at a
-//    time as one concatenated range;
-//  - candidates, compaction at 2k and the final selection are those of
-//    knn_wave_kernel; once the k-th distance has shrunk below the distance of
-//    the next cell, that cell and every later one are skipped.
+//  - where the kept cells hold several 64-photon blocks, the blocks whose
+//    boxes lie within the radius replace the cells (at most 128 ranges);
+//  - the ranges are ordered nearest first, and their photons are read 64 at
+//    a time as one concatenated range;
+//  - the candidates are cut to the k nearest whenever k + 64 have gathered,
+//    and the k-th distance then bounds later candidates; once it has shrunk
+//    below the distance of the next range, that range and every later one
+//    are skipped;
+//  - the final selection and the outputs are wave_finish's.
 // `near` is a lower bound on the float d2 of every photon of a cell: the
 // distance to the cell's box, widened for the rounding of cell assignment,
 // squared and lowered by 2^-20 for the rounding of the float d2.
@@ -700,8 +704,8 @@ __device__ __forceinline__ uint2 cell_lookup(const PhotonGridView& G, uint64_t k
     return make_uint2(0u, 0u);
 }
 
-// Candidates: compaction at max(2k, k + 64) <= 512 for k <= 256, plus one
-// chunk of 64.
+// Candidates: compaction at k + 64 (a sweep may ask for more, up to 512),
+// plus one chunk of 64.
 constexpr int kCellCand = 2 * (int)kKnnMaxK + 64;
 
 // Workgroups of kCellWaves waves, one query per wave: a workgroup's waves
@@ -744,7 +748,7 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
     const uint32_t lane = __lane_id();
     const uint32_t k = L.k;
     const uint32_t compact_at =
-        min((uint32_t)kCellCand - 64u, L.compact_slack ? k + L.compact_slack : max(2u * k, k + 128u));
+        min((uint32_t)kCellCand - 64u, k + (L.compact_slack ? L.compact_slack : 64u));
     // Grid-stride over the prepared queries (cell order when sorted): the
     // waves running at one time hold a window of consecutive queries, which
     // share cells.  The next query's record is loaded a query ahead.
@@ -1128,7 +1132,7 @@ void launch_knn_cells(const KnnLaunch& L, float4* prep, int grid, hipStream_t s)
     KnnLaunch C = L;
     C.qprep = prep;
     // RQ_KNN_COMPACT_SLACK: compact the candidates once they exceed k by this
-    // many (default max(k, 128)), for sweeps; results do not depend on it.
+    // many (default 64), for sweeps; results do not depend on it.
     if (const char* e = std::getenv("RQ_KNN_COMPACT_SLACK")) C.compact_slack = (uint32_t)std::max(1, std::atoi(e));
     if (C.tally)
         hipLaunchKernelGGL(dev::knn_cell_kernel<true>, dim3(grid), dim3(dev::kKnnWave * dev::kCellWaves), 0, s, C);
