@@ -682,8 +682,8 @@ __global__ __launch_bounds__(64) void knn_wave_kernel(KnnLaunch L, uint32_t stac
 //    farther than the radius, and looks up its photon range;
 //  - where the kept cells hold several 64-photon blocks, the blocks whose
 //    boxes lie within the radius replace the cells (at most 128 ranges);
-//  - the ranges are ordered nearest first, and their photons are read 64 at
-//    a time as one concatenated range;
+//  - the ranges are ordered nearest first; blocks are read one per 64-lane
+//    chunk, cells 64 photons at a time as one concatenated range;
 //  - the candidates are cut to the k nearest whenever k + 64 have gathered,
 //    and the k-th distance then bounds later candidates; once it has shrunk
 //    below the distance of the next range, that range and every later one
@@ -798,6 +798,7 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
             uint2 range = make_uint2(0u, 0u), range1 = make_uint2(0u, 0u);  // slots lane, lane + 64
             double near1 = 0.0;
             bool v1 = false;
+            bool blk = false;  // the ranges are blocks (wave-uniform)
             uint32_t fb = 0;  // the cell's first block (grids with blocks)
             if (v) {
                 if (G.dense) {
@@ -871,6 +872,7 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
                         nsurv += cnt;
                     }
                     if (fits) {
+                        blk = true;
                         __builtin_amdgcn_wave_barrier();
                         v = lane < nsurv;
                         v1 = lane + 64u < nsurv;
@@ -926,90 +928,23 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
                 s_near[rank1] = __uint_as_float(nb1);
             }
             __builtin_amdgcn_wave_barrier();
-            // photon counts -> exclusive prefix, in rank order (slots lane,
-            // then lane + 64 after the first 64 in total)
-            const uint32_t cnt = lane < ncell ? s_pre[lane] : 0u;
-            const uint32_t cnt1 = lane + 64u < ncell ? s_pre[lane + 64u] : 0u;
-            uint32_t incl = cnt, incl1 = cnt1;
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t t = __shfl_up(incl, o, 64), t1 = __shfl_up(incl1, o, 64);
-                if (lane >= (uint32_t)o) {
-                    incl += t;
-                    incl1 += t1;
-                }
-            }
-            const uint32_t total0 = __shfl(incl, 63, 64);
-            const uint32_t total = total0 + __shfl(incl1, 63, 64);
-            __builtin_amdgcn_wave_barrier();
-            if (lane < ncell) s_pre[lane] = incl - cnt;
-            if (lane + 64u < ncell) s_pre[lane + 64u] = total0 + incl1 - cnt1;
-            __builtin_amdgcn_wave_barrier();
-            // Chunk e0 covers elements e0 .. e0 + 63 of the concatenated
-            // ranges; its photons (position, class, slot) are loaded one
-            // chunk ahead, so their latency overlaps the previous chunk.
-            //
-            // Every kept cell holds a photon, so the starts increase with the
-            // cell and an element's cell is the last one starting at or
-            // before it.  When few cells start inside a chunk, each lane
-            // steps its cell forward; when many do (sparse cells), the chunk
-            // marks their starts in LDS and takes a running maximum over the
-            // lanes, carried from the previous chunk's last element.
-            const uint32_t my_start = lane < ncell ? s_pre[lane] : 0xFFFFFFFFu;
-            const uint32_t my_start1 = lane + 64u < ncell ? s_pre[lane + 64u] : 0xFFFFFFFFu;
-            uint32_t ci = 0, carry = 0;  // this lane's cell; the cell of the chunk's previous element
-            float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            uint32_t sl = 0;
-            // loads chunk e0; returns the cell of its first element
-            auto fetch = [&](uint32_t e0, float4& fp, uint32_t& fs) -> uint32_t {
-                const uint32_t e = e0 + lane;
-                const bool starts_here = lane < ncell && my_start - e0 < 64u;
-                const bool starts_here1 = lane + 64u < ncell && my_start1 - e0 < 64u;
-                if (__popcll(__ballot(starts_here)) + __popcll(__ballot(starts_here1)) <= 3) {
-                    if (e < total)
-                        while (ci + 1 < ncell && s_pre[ci + 1] <= e) ++ci;
-                } else {
-                    s_mark[lane] = 0u;
-                    __builtin_amdgcn_wave_barrier();
-                    // ranges are not empty: no two start at one element
-                    if (starts_here) s_mark[my_start - e0] = lane;
-                    if (starts_here1) s_mark[my_start1 - e0] = lane + 64u;
-                    __builtin_amdgcn_wave_barrier();
-                    uint32_t c = max(s_mark[lane], carry);
-                    for (int o = 1; o < 64; o <<= 1) {
-                        const uint32_t t = __shfl_up(c, o, 64);
-                        if (lane >= (uint32_t)o) c = max(c, t);
-                    }
-                    ci = c;
-                }
-                carry = __shfl(ci, 63, 64);
-                if (e < total) {
-                    const uint32_t gi = s_first[ci] + (e - s_pre[ci]);
-                    fp = G.pos[gi];
-                    fs = G.slot[gi];
-                }
-                return __shfl(ci, 0, 64);
-            };
-            uint32_t c0 = total ? fetch(0, p, sl) : 0u;  // the cell of the chunk's first element
-            uint32_t n_read = 0, n_cand = 0;                // COUNT only
-            for (uint32_t e0 = 0; e0 < total; e0 += 64) {
-                if (s_near[c0] >= r2) break;  // this cell and all later ones are too far
-                if (COUNT) n_read += min(64u, total - e0);
-                float4 pn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                uint32_t sn = 0;
-                uint32_t c0n = c0;
-                if (e0 + 64 < total) c0n = fetch(e0 + 64, pn, sn);
+            uint32_t n_read = 0, n_cand = 0;  // COUNT only
+            // Takes the photons of one chunk (position and class, slot) as
+            // candidates, and cuts the candidates to the k nearest once k +
+            // 64 have gathered (the k-th distance then bounds later ones).
+            auto take = [&](const float4& pp, uint32_t ss, bool valid) {
                 bool is_c = false;
                 float d2 = 0.0f;
-                if (e0 + lane < total) {
-                    const float x = qx - p.x, y = qy - p.y, z = qz - p.z;
+                if (valid) {
+                    const float x = qx - pp.x, y = qy - pp.y, z = qz - pp.z;
                     d2 = (x * x + y * y) + z * z;
-                    is_c = d2 < r2 && __float_as_uint(p.w) == cls;
+                    is_c = d2 < r2 && __float_as_uint(pp.w) == cls;
                 }
                 const uint64_t cm = __ballot(is_c);
                 if (is_c) {
                     const uint32_t pos = cand + lanes_below(cm);
                     s_key[pos] = __float_as_uint(d2);
-                    s_slot[pos] = sl;
+                    s_slot[pos] = ss;
                 }
                 cand += (uint32_t)__popcll(cm);
                 if (COUNT) n_cand += (uint32_t)__popcll(cm);
@@ -1020,9 +955,107 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
                     cand = compact_kth(s_key, s_slot, cand, t, need);
                     r2 = __uint_as_float(kept_max(s_key, cand));
                 }
-                p = pn;
-                sl = sn;
-                c0 = c0n;
+            };
+            float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            uint32_t sl = 0;
+            if (blk) {
+                // Blocks hold at most 64 photons: one block per chunk, lane j
+                // taking its photon j; the next block is loaded one ahead.
+                auto fetch_block = [&](uint32_t i, float4& fp, uint32_t& fs) -> uint32_t {
+                    const uint32_t first = s_first[i], n = s_pre[i];
+                    if (lane < n) {
+                        fp = G.pos[first + lane];
+                        fs = G.slot[first + lane];
+                    }
+                    return n;
+                };
+                uint32_t n0 = ncell ? fetch_block(0, p, sl) : 0u;  // no block may lie within the radius
+                for (uint32_t i = 0; i < ncell; ++i) {
+                    if (s_near[i] >= r2) break;  // this block and all later ones are too far
+                    if (COUNT) n_read += n0;
+                    float4 pn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    uint32_t sn = 0, nn = 0;
+                    if (i + 1 < ncell) nn = fetch_block(i + 1, pn, sn);
+                    take(p, sl, lane < n0);
+                    p = pn;
+                    sl = sn;
+                    n0 = nn;
+                }
+            } else {
+                // photon counts -> exclusive prefix, in rank order (slots lane,
+                // then lane + 64 after the first 64 in total)
+                const uint32_t cnt = lane < ncell ? s_pre[lane] : 0u;
+                const uint32_t cnt1 = lane + 64u < ncell ? s_pre[lane + 64u] : 0u;
+                uint32_t incl = cnt, incl1 = cnt1;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const uint32_t t = __shfl_up(incl, o, 64), t1 = __shfl_up(incl1, o, 64);
+                    if (lane >= (uint32_t)o) {
+                        incl += t;
+                        incl1 += t1;
+                    }
+                }
+                const uint32_t total0 = __shfl(incl, 63, 64);
+                const uint32_t total = total0 + __shfl(incl1, 63, 64);
+                __builtin_amdgcn_wave_barrier();
+                if (lane < ncell) s_pre[lane] = incl - cnt;
+                if (lane + 64u < ncell) s_pre[lane + 64u] = total0 + incl1 - cnt1;
+                __builtin_amdgcn_wave_barrier();
+                // Chunk e0 covers elements e0 .. e0 + 63 of the concatenated
+                // ranges; its photons (position, class, slot) are loaded one
+                // chunk ahead, so their latency overlaps the previous chunk.
+                //
+                // Every kept cell holds a photon, so the starts increase with the
+                // cell and an element's cell is the last one starting at or
+                // before it.  When few cells start inside a chunk, each lane
+                // steps its cell forward; when many do (sparse cells), the chunk
+                // marks their starts in LDS and takes a running maximum over the
+                // lanes, carried from the previous chunk's last element.
+                const uint32_t my_start = lane < ncell ? s_pre[lane] : 0xFFFFFFFFu;
+                const uint32_t my_start1 = lane + 64u < ncell ? s_pre[lane + 64u] : 0xFFFFFFFFu;
+                uint32_t ci = 0, carry = 0;  // this lane's cell; the cell of the chunk's previous element
+                // loads chunk e0; returns the cell of its first element
+                auto fetch = [&](uint32_t e0, float4& fp, uint32_t& fs) -> uint32_t {
+                    const uint32_t e = e0 + lane;
+                    const bool starts_here = lane < ncell && my_start - e0 < 64u;
+                    const bool starts_here1 = lane + 64u < ncell && my_start1 - e0 < 64u;
+                    if (__popcll(__ballot(starts_here)) + __popcll(__ballot(starts_here1)) <= 3) {
+                        if (e < total)
+                            while (ci + 1 < ncell && s_pre[ci + 1] <= e) ++ci;
+                    } else {
+                        s_mark[lane] = 0u;
+                        __builtin_amdgcn_wave_barrier();
+                        // ranges are not empty: no two start at one element
+                        if (starts_here) s_mark[my_start - e0] = lane;
+                        if (starts_here1) s_mark[my_start1 - e0] = lane + 64u;
+                        __builtin_amdgcn_wave_barrier();
+                        uint32_t c = max(s_mark[lane], carry);
+                        for (int o = 1; o < 64; o <<= 1) {
+                            const uint32_t t = __shfl_up(c, o, 64);
+                            if (lane >= (uint32_t)o) c = max(c, t);
+                        }
+                        ci = c;
+                    }
+                    carry = __shfl(ci, 63, 64);
+                    if (e < total) {
+                        const uint32_t gi = s_first[ci] + (e - s_pre[ci]);
+                        fp = G.pos[gi];
+                        fs = G.slot[gi];
+                    }
+                    return __shfl(ci, 0, 64);
+                };
+                uint32_t c0 = total ? fetch(0, p, sl) : 0u;  // the cell of the chunk's first element
+                for (uint32_t e0 = 0; e0 < total; e0 += 64) {
+                    if (s_near[c0] >= r2) break;  // this cell and all later ones are too far
+                    if (COUNT) n_read += min(64u, total - e0);
+                    float4 pn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    uint32_t sn = 0;
+                    uint32_t c0n = c0;
+                    if (e0 + 64 < total) c0n = fetch(e0 + 64, pn, sn);
+                    take(p, sl, e0 + lane < total);
+                    p = pn;
+                    sl = sn;
+                    c0 = c0n;
+                }
             }
             if (COUNT && lane == 0) {
                 atomicAdd(&L.tally[1], (unsigned long long)ncell);
