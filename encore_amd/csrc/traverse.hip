@@ -364,18 +364,18 @@ __device__ __forceinline__ uint32_t chunk_item(const BatchItemDev* items, uint32
     return lo;
 }
 
-// BVH4 node step on the node's sign-selected near/far rows (near x, far x,
-// near y, far y, near z, far z) and its refs: the four slab tests,
-// the hit children sorted by entry distance with a 5-comparator network,
-// the nearest visited next and the others pushed far to near with their
-// entry distance.
-template <int S, class Pop>
-__device__ __forceinline__ void node4_step(const RayState& r, const float4 (&d)[7], float tmin, float tbest,
-                                           RingStack<S, 2>& stk, uint32_t& cur, Pop&& pop_next) {
+// Entry distances of a BVH4 node's four children from the node's
+// sign-selected near/far rows (near x, far x, near y, far y, near z, far z)
+// and its refs: k[j] is child j's entry distance when the ray enters it
+// before tbest, +inf otherwise; c[j] its ref.
+__device__ __forceinline__ void keys4(const RayState& r, const float4 (&d)[7], float tmin, float tbest, float (&k)[4],
+                                      uint32_t (&c)[4]) {
     const float kInf = __int_as_float(0x7f800000);
     const float4 nx = d[0], fx = d[1], ny = d[2], fy = d[3], nz = d[4], fz = d[5], rf = d[6];
-    float k[4];
-    uint32_t c[4] = {__float_as_uint(rf.x), __float_as_uint(rf.y), __float_as_uint(rf.z), __float_as_uint(rf.w)};
+    c[0] = __float_as_uint(rf.x);
+    c[1] = __float_as_uint(rf.y);
+    c[2] = __float_as_uint(rf.z);
+    c[3] = __float_as_uint(rf.w);
     const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
     const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
     const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
@@ -388,6 +388,15 @@ __device__ __forceinline__ void node4_step(const RayState& r, const float4 (&d)[
         const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tbest)) * 1.00000048f;
         k[j] = (tn <= tf && c[j] != kRefEmpty) ? tn : kInf;
     }
+}
+
+// The entered children sorted by entry distance with a 5-comparator
+// network; the nearest is visited next and the others are pushed far to
+// near with their entry distance.
+template <int S, class Pop>
+__device__ __forceinline__ void order_push4(float (&k)[4], uint32_t (&c)[4], RingStack<S, 2>& stk, uint32_t& cur,
+                                            Pop&& pop_next) {
+    const float kInf = __int_as_float(0x7f800000);
     cswap(k[0], c[0], k[1], c[1]);
     cswap(k[2], c[2], k[3], c[3]);
     cswap(k[0], c[0], k[2], c[2]);
@@ -415,6 +424,16 @@ __device__ __forceinline__ void node4_step(const RayState& r, const float4 (&d)[
     } else {
         cur = pop_next();
     }
+}
+
+// BVH4 node step on the node's sign-selected near/far rows and refs.
+template <int S, class Pop>
+__device__ __forceinline__ void node4_step(const RayState& r, const float4 (&d)[7], float tmin, float tbest,
+                                           RingStack<S, 2>& stk, uint32_t& cur, Pop&& pop_next) {
+    float k[4];
+    uint32_t c[4];
+    keys4(r, d, tmin, tbest, k, c);
+    order_push4(k, c, stk, cur, pop_next);
 }
 
 // One traversal step per lane per iteration, through shared loads.  Every
@@ -473,7 +492,9 @@ __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
     // COUNT only: iterations, iterations with a leaf lane, refill rounds
     // (tallied by lane 0).
     uint32_t w_node = 0, w_tri = 0, w_round = 0;
-    uint32_t offx = 0, offy = 0, offz = 0;  // near-plane rows picked by direction sign
+    // Byte offsets of the near-plane rows picked by direction sign (BVH4):
+    // x in {0, 16}, y in {32, 48}, z in {64, 80}; the far row is at offset ^ 16.
+    uint32_t offx = 0, offy = 0, offz = 0;
     // Wave-uniform chunk state: the claimed id range [cl_base, cl_end) and
     // the staged chunk (ch_n rays from ch_base, ch_used handed out).
     uint32_t cl_base = 0, cl_end = 0, ch_base = 0, ch_n = 0, ch_used = 0;
@@ -581,9 +602,9 @@ __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
                     }
                     Pred::prepare(r);
                     if (WIDTH == 4) {
-                        offx = __builtin_signbit(r.dx) ? 4u : 0u;
-                        offy = __builtin_signbit(r.dy) ? 12u : 8u;
-                        offz = __builtin_signbit(r.dz) ? 20u : 16u;
+                        offx = __builtin_signbit(r.dx) ? 16u : 0u;
+                        offy = __builtin_signbit(r.dy) ? 48u : 32u;
+                        offz = __builtin_signbit(r.dz) ? 80u : 64u;
                     } else {
                         offx = offy = offz = 0u;
                     }
@@ -625,13 +646,14 @@ __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
             float4 d[NL];
             if (at_node) {
                 if (WIDTH == 4) {
-                    d[0] = ldg4(np + offx);
-                    d[1] = ldg4(np + (offx ^ 4u));
-                    d[2] = ldg4(np + offy);
-                    d[3] = ldg4(np + (offy ^ 4u));
-                    d[4] = ldg4(np + offz);
-                    d[5] = ldg4(np + (offz ^ 4u));
-                    d[6] = ldg4(np + 24);
+                    const char* nb = reinterpret_cast<const char*>(np);
+                    d[0] = ldg4(nb + offx);
+                    d[1] = ldg4(nb + (offx ^ 16u));
+                    d[2] = ldg4(nb + offy);
+                    d[3] = ldg4(nb + (offy ^ 16u));
+                    d[4] = ldg4(nb + offz);
+                    d[5] = ldg4(nb + (offz ^ 16u));
+                    d[6] = ldg4(nb + 96);
                 } else {
 #pragma unroll
                     for (int q = 0; q < NL; ++q) d[q] = ldg4(np + 4 * q);
@@ -770,8 +792,9 @@ __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
                 cur = kNone;  // popped when the leaf is done
             }
             const bool leaf = lpos < lend;
-            // Record addresses: node rows, or the leaf's next TPS triangles
-            // (a missing second triangle repeats the first).
+            // Record addresses: one base per lane (the node, or the leaf's
+            // next TPS triangles) plus a byte offset per row (a missing
+            // second triangle repeats the first).
             const float* np = WIDTH == 4
                                   ? reinterpret_cast<const float*>(static_cast<const BvhNode4*>(nodes) + cur)
                                   : reinterpret_cast<const float*>(static_cast<const BvhNode*>(nodes) + cur);
@@ -779,14 +802,15 @@ __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
             const bool two = TPS == 2 && lpos + 1 < lend;
             float4 d[NL];
             if (WIDTH == 4) {
-                const uint32_t o2 = two ? 12u : 0u;
-                d[0] = ldg4(leaf ? tp : np + offx);
-                d[1] = ldg4(leaf ? tp + 4 : np + (offx ^ 4u));
-                d[2] = ldg4(leaf ? tp + 8 : np + offy);
-                d[3] = ldg4(leaf ? tp + o2 : np + (offy ^ 4u));
-                d[4] = ldg4(leaf ? tp + o2 + 4 : np + offz);
-                d[5] = ldg4(leaf ? tp + o2 + 8 : np + (offz ^ 4u));
-                d[6] = ldg4(leaf ? tp : np + 24);
+                const char* base = reinterpret_cast<const char*>(leaf ? tp : np);
+                const uint32_t o2 = two ? 48u : 0u;
+                d[0] = ldg4(base + (leaf ? 0u : offx));
+                d[1] = ldg4(base + (leaf ? 16u : offx ^ 16u));
+                d[2] = ldg4(base + (leaf ? 32u : offy));
+                d[3] = ldg4(base + (leaf ? o2 : offy ^ 16u));
+                d[4] = ldg4(base + (leaf ? o2 + 16u : offz));
+                d[5] = ldg4(base + (leaf ? o2 + 32u : offz ^ 16u));
+                d[6] = ldg4(base + (leaf ? 0u : 96u));
             } else {
                 d[0] = ldg4(leaf ? tp : np);
                 d[1] = ldg4(leaf ? tp + 4 : np + 4);
@@ -835,47 +859,9 @@ __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
                     }
                 } else {
                     float k[4];
-                    uint32_t c[4] = {__float_as_uint(d[6].x), __float_as_uint(d[6].y), __float_as_uint(d[6].z),
-                                     __float_as_uint(d[6].w)};
-                    const float nxa[4] = {d[0].x, d[0].y, d[0].z, d[0].w}, fxa[4] = {d[1].x, d[1].y, d[1].z, d[1].w};
-                    const float nya[4] = {d[2].x, d[2].y, d[2].z, d[2].w}, fya[4] = {d[3].x, d[3].y, d[3].z, d[3].w};
-                    const float nza[4] = {d[4].x, d[4].y, d[4].z, d[4].w}, fza[4] = {d[5].x, d[5].y, d[5].z, d[5].w};
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const float tnx = __builtin_fmaf(nxa[j], r.ix, r.cnx), tfx = __builtin_fmaf(fxa[j], r.ix, r.cfx);
-                        const float tny = __builtin_fmaf(nya[j], r.iy, r.cny), tfy = __builtin_fmaf(fya[j], r.iy, r.cfy);
-                        const float tnz = __builtin_fmaf(nza[j], r.iz, r.cnz), tfz = __builtin_fmaf(fza[j], r.iz, r.cfz);
-                        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
-                        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tbest)) * 1.00000048f;
-                        k[j] = (tn <= tf && c[j] != kRefEmpty) ? tn : kInf;
-                    }
-                    cswap(k[0], c[0], k[1], c[1]);
-                    cswap(k[2], c[2], k[3], c[3]);
-                    cswap(k[0], c[0], k[2], c[2]);
-                    cswap(k[1], c[1], k[3], c[3]);
-                    cswap(k[1], c[1], k[2], c[2]);
-                    if (k[0] < kInf) {
-                        // Push children 1..nh-1 far to near: child j lands
-                        // at offset npush - j from the current top.
-                        const int npush = (k[1] < kInf) + (k[2] < kInf) + (k[3] < kInf);
-                        stk.reserve(npush);
-                        if (npush >= 3) {
-                            uint32_t v[SW] = {c[3], __float_as_uint(k[3])};
-                            stk.put(0, v);
-                        }
-                        if (npush >= 2) {
-                            uint32_t v[SW] = {c[2], __float_as_uint(k[2])};
-                            stk.put(npush - 2, v);
-                        }
-                        if (npush >= 1) {
-                            uint32_t v[SW] = {c[1], __float_as_uint(k[1])};
-                            stk.put(npush - 1, v);
-                        }
-                        stk.advance(npush);
-                        cur = c[0];
-                    } else {
-                        cur = pop_next();
-                    }
+                    uint32_t c[4];
+                    keys4(r, d, tmin, tbest, k, c);
+                    order_push4(k, c, stk, cur, pop_next);
                 }
             }
             if (ANY && finished) {
