@@ -20,15 +20,19 @@
 // sphere (widened by 2^-10 for the float arithmetic of the test, which
 // admits at most 2^-19 beyond it) touches, in insertion order; elements whose
 // reach is unbounded (R_i = 0, negative or infinite, where the distance term
-// vanishes or turns negative) or that would touch more than 4,096 cells go to
-// a global list.  One lane per lookup merges its cell's list with the global
-// list by index and evaluates the reference's weight and front test with its
-// float/double types and operation order, so the sums are the reference's
-// bit for bit.  Lookups outside the grid see the global list only.
+// vanishes or turns negative) or that would touch more than 65,536 cells go
+// to a global list.  One lane per lookup merges its cell's list with the
+// global list by index and evaluates the reference's weight and front test
+// with its float/double types and operation order, so the sums are the
+// reference's bit for bit; an element whose squared distance alone bounds
+// its weight to at most 1 / a (a per-element bound the host computes with
+// upward rounding) is passed over after four products.  Lookups outside the
+// grid see the global list only.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <vector>
 
 #include "rq_internal.h"
@@ -103,7 +107,12 @@ __global__ __launch_bounds__(256) void ic_query(IcLaunch L) {
         const float4 nn = L.nrm[e];
         // TCacheElement::GetWeight (IrrCacheOctree.h:55-77).
         const float vx = pr.x - px, vy = pr.y - py, vz = pr.z - pz;
-        const float distance = __builtin_sqrtf((vx * vx + vy * vy) + vz * vz);
+        const float d2 = (vx * vx + vy * vy) + vz * vz;
+        // nn.w: a squared distance at and beyond which the distance term
+        // alone keeps the weight at or below 1 / a (set by the host, see
+        // reject_d2): the element cannot count, so the rest is skipped.
+        if (d2 >= nn.w) continue;
+        const float distance = __builtin_sqrtf(d2);
         const float distCalc = (pr.w == 0.0f) ? 0.0f : distance / pr.w;
         const bool same = nx == nn.x && ny == nn.y && nz == nn.z;
         const float d = (nx * nn.x + ny * nn.y) + nz * nn.z;
@@ -164,7 +173,14 @@ namespace rq {
 namespace {
 
 constexpr uint32_t kIcMaxDim = 128;          // cells per axis
-constexpr uint64_t kIcMaxCellsPerElem = 4096;
+constexpr uint64_t kIcMaxCellsPerElem = 65536;
+// Cell edge: the median reach diameter over this.  A lookup examines the
+// elements whose reach touches its cell; at the final gather's cache (16,364
+// values) edges of 1, 1/2, 1/3, 1/4 and 1/8 diameters examine 117.5, 82.9,
+// 73.6, 69.4 and 64.0 values per lookup (19.8 count) and list 0.16, 0.56,
+// 1.44, 2.92 and 15.8 M references; 1/3 keeps the host's index build near
+// 3.5 ms per rebuild.
+constexpr double kIcCellsPerDiameter = 3.0;
 
 template <class F>
 rq_status run(F&& f) {
@@ -173,6 +189,48 @@ rq_status run(F&& f) {
 
 void require(bool ok, const char* msg) {
     if (!ok) throw Error(RQ_ERR_INVALID_ARG, msg);
+}
+
+// The least positive float T with fl(1 / T) <= tol (+inf when there is none):
+// a weight 1 / denom with denom >= T is at most tol, since float division
+// is monotonic.
+float reject_dist(float tol) {
+    if (!(tol > 0.0f) || !std::isfinite(tol)) return INFINITY;
+    float t = (float)(1.0 / (double)tol);
+    if (!(t > 0.0f)) t = std::numeric_limits<float>::denorm_min();
+    while (!(1.0f / t <= tol)) {
+        t = std::nextafter(t, INFINITY);
+        if (std::isinf(t)) return INFINITY;
+    }
+    for (;;) {
+        const float p = std::nextafter(t, 0.0f);
+        if (p > 0.0f && 1.0f / p <= tol) {
+            t = p;
+        } else {
+            break;
+        }
+    }
+    return t;
+}
+
+// Squared-distance bound of one element (kept in its normal's w on the
+// device): Z = Y^2 and Y = T R_i, each rounded up to a float.  A lookup with
+// float d2 >= Z has sqrt(d2) >= Y >= T R_i, so its distance rounds to at
+// least Y, distance / R_i to at least T, the denominator (the distance term
+// plus a non-negative or NaN normal term) to at least T or NaN, and the
+// weight to at most fl(1 / T) <= 1 / a, or NaN: GetWeight's element does
+// not count, as in the reference.  +inf where R_i is not a positive finite
+// float (the distance term is then 0 or negative: no bound).
+float reject_d2(float T, float ri) {
+    if (!(ri > 0.0f) || !std::isfinite(ri) || !std::isfinite(T)) return INFINITY;
+    auto up = [](double x) {
+        float f = (float)x;
+        if ((double)f < x) f = std::nextafter(f, INFINITY);
+        return f;
+    };
+    const float y = up((double)T * (double)ri);  // the double product is exact
+    if (!std::isfinite(y)) return INFINITY;
+    return up((double)y * (double)y);
 }
 
 // An element's reach (the sphere a lookup must enter for its weight to pass
@@ -201,14 +259,15 @@ void build_index(rq_irrcache* c) {
         }
         diam.push_back(2.0 * reach[i]);
     }
-    // Cell edge: the median diameter, and at least 1/(kIcMaxDim - 1) of the
-    // longest extent, so that floor(extent / h) + 1 <= kIcMaxDim cells cover
-    // every reach: no bounded element ends beyond its axis's last cell.
+    // Cell edge: a third of the median diameter, and at least 1/(kIcMaxDim -
+    // 1) of the longest extent, so that floor(extent / h) + 1 <= kIcMaxDim
+    // cells cover every reach: no bounded element ends beyond its axis's last
+    // cell.
     double h = 1.0;
     if (!diam.empty()) {
         std::nth_element(diam.begin(), diam.begin() + diam.size() / 2, diam.end());
         const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
-        h = std::max(diam[diam.size() / 2], ext / (kIcMaxDim - 1));
+        h = std::max(diam[diam.size() / 2] / kIcCellsPerDiameter, ext / (kIcMaxDim - 1));
         if (!(h > 0.0) || !std::isfinite(h)) h = 1.0;
     } else {
         lo[0] = lo[1] = lo[2] = 0.0;
@@ -267,9 +326,11 @@ void build_index(rq_irrcache* c) {
                     items[fill[((size_t)z * c->dims[1] + (size_t)y) * c->dims[0] + x]++] = i;
     }
     std::vector<float4> pos(std::max<uint32_t>(1, n)), nrm(std::max<uint32_t>(1, n)), val(std::max<uint32_t>(1, n));
+    const float T = reject_dist(1 / c->a);  // 1 / a as the lookup forms it
     for (uint32_t i = 0; i < n; ++i) {
         pos[i] = c->elems[i].pos_ri;
         nrm[i] = c->elems[i].nrm;
+        nrm[i].w = reject_d2(T, c->elems[i].pos_ri.w);
         val[i] = c->elems[i].val;
     }
     RQ_HIP(hipSetDevice(c->dev->ordinal));

@@ -71,6 +71,41 @@ def test_irrcache_closed_forms(device):
     c.destroy()
 
 
+def test_irrcache_weight_threshold(device):
+    """Lookups within a few ulps of the weight bound 1 / a: each element sits
+    alone (spacing far above its reach) and every lookup shares its normal, so
+    the weight is 1 / (distance / Ri) against the strict test w > 1 / a.  The
+    engine passes over an element once the squared distance alone bounds its
+    weight (a per-element bound rounded upward on the host); a bound rounded
+    the wrong way would drop lookups the oracle counts."""
+    up = (0.0, 1.0, 0.0)
+    c = rq.IrrCache(device, A, LO, SIZE)
+    o = oracle.IrrCache(A, LO, SIZE)
+    ris = np.float32([0.37, 1.0, 3.7, 16.0, 100.3, 0.01171875, 7.77])
+    pos = np.float32([[-60.0 + 17.0 * i, 3.0 * (i % 3), -2.0 * i] for i in range(len(ris))])
+    val = np.float32([[1.0 + i, 2.0, 0.5] for i in range(len(ris))])
+    nrm = np.tile(np.float32(up), (len(ris), 1))
+    assert c.add(val, pos, nrm, ris).tolist() == o.add(val, pos, nrm, ris).tolist() == [0] * len(ris)
+    q = []
+    for i, r in enumerate(ris):
+        reach = np.float32(A) * r
+        for k in range(-48, 49):
+            for axis in range(3):
+                d = np.float32(reach * np.float32(1.0 + k * 2.0 ** -23))
+                p = pos[i].copy()
+                p[axis] = np.float32(p[axis] + d)
+                q.append(p)
+    q = np.float32(q)
+    n = np.tile(np.float32(up), (len(q), 1))
+    got, ref = _query(c, q, n), o.query(q, n)
+    log("threshold lookups: %d, counted %d" % (len(q), int(got[2].sum())))
+    _same(got, ref)
+    # both outcomes occur near every element's bound
+    cnt = got[2].reshape(len(ris), -1)
+    assert np.all(cnt.max(axis=1) == 1) and np.all(cnt.min(axis=1) == 0)
+    c.destroy()
+
+
 def test_irrcache_matches_oracle(device):
     rng = np.random.default_rng(21)
     p, nrm = _cornell_points(12000, 5)
