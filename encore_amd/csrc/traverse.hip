@@ -53,11 +53,19 @@ __device__ __forceinline__ uint4 ldgu4(const void* p) {
     const U4v v = *(GU4v*)(uintptr_t)p;
     return make_uint4(v.x, v.y, v.z, v.w);
 }
-__device__ __forceinline__ void stg4(void* p, float4 v) {
-    const F4v o = {v.x, v.y, v.z, v.w};
-    *(GF4vOut*)(uintptr_t)p = o;
+// Streaming accesses (rays read once, hits written once): nontemporal, so
+// that they do not displace the scene's nodes and triangles in the caches.
+__device__ __forceinline__ float4 ldg4_stream(const void* p) {
+    const F4v v = __builtin_nontemporal_load((GF4v*)(uintptr_t)p);
+    return make_float4(v.x, v.y, v.z, v.w);
 }
-__device__ __forceinline__ void stg1(uint32_t* p, uint32_t v) { *(GU32Out*)(uintptr_t)p = v; }
+__device__ __forceinline__ void stg4_stream(void* p, float4 v) {
+    const F4v o = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(o, (GF4vOut*)(uintptr_t)p);
+}
+__device__ __forceinline__ void stg1_stream(uint32_t* p, uint32_t v) {
+    __builtin_nontemporal_store(v, (GU32Out*)(uintptr_t)p);
+}
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
@@ -247,8 +255,8 @@ struct PredWatertight {
 };
 
 __device__ __forceinline__ void init_ray(RayState& r, const rq_ray* rays, uint32_t id, float& tmin, float& tmax) {
-    const float4 p = ldg4(&rays[id].ox);
-    const float4 q = ldg4(&rays[id].dx);
+    const float4 p = ldg4_stream(&rays[id].ox);
+    const float4 q = ldg4_stream(&rays[id].dx);
     r.ox = p.x;
     r.oy = p.y;
     r.oz = p.z;
@@ -555,8 +563,8 @@ __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
                             rin = it->rays;
                             loc = id - it->begin;
                         }
-                        const float4 p = ldg4(&rin[loc].ox);
-                        const float4 q = ldg4(&rin[loc].dx);
+                        const float4 p = ldg4_stream(&rin[loc].ox);
+                        const float4 q = ldg4_stream(&rin[loc].dx);
                         stage.at(0, lane) = p.x;
                         stage.at(1, lane) = p.y;
                         stage.at(2, lane) = p.z;
@@ -771,10 +779,10 @@ __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
             // Retire a finished ray.
             if (local != kNone && cur == kRefSentinel) {
                 if (ANY) {
-                    stg1(static_cast<uint32_t*>(out) + local, (bprim != kNone) ? 1u : 0u);
+                    stg1_stream(static_cast<uint32_t*>(out) + local, (bprim != kNone) ? 1u : 0u);
                 } else {
                     const bool has = bprim != kNone;
-                    stg4(static_cast<rq_hit*>(out) + local,
+                    stg4_stream(static_cast<rq_hit*>(out) + local,
                          make_float4(tbest, has ? bu : 0.0f, has ? bv : 0.0f, __uint_as_float(bprim)));
                 }
                 if (COUNT) {
@@ -871,10 +879,10 @@ __global__ __launch_bounds__(kWave) void bvh_trace(TraceLaunch L) {
             // Retire a finished ray.
             if (cur == kRefSentinel && lpos >= lend) {
                 if (ANY) {
-                    stg1(static_cast<uint32_t*>(out) + local, (bprim != kNone) ? 1u : 0u);
+                    stg1_stream(static_cast<uint32_t*>(out) + local, (bprim != kNone) ? 1u : 0u);
                 } else {
                     const bool has = bprim != kNone;
-                    stg4(static_cast<rq_hit*>(out) + local,
+                    stg4_stream(static_cast<rq_hit*>(out) + local,
                          make_float4(tbest, has ? bu : 0.0f, has ? bv : 0.0f, __uint_as_float(bprim)));
                 }
                 if (COUNT) {
@@ -1031,9 +1039,9 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
             active = root_clip(r, bpos, bsize, tmin, tmax);
             if (!active) {  // the ray misses the scene box
                 if (ANY)
-                    stg1(occ_out + slot, 0u);
+                    stg1_stream(occ_out + slot, 0u);
                 else
-                    stg4(hits_out + slot, make_float4(rtmax, 0.0f, 0.0f, __uint_as_float(kNone)));
+                    stg4_stream(hits_out + slot, make_float4(rtmax, 0.0f, 0.0f, __uint_as_float(kNone)));
                 ray = kNone;
             }
         }
@@ -1239,9 +1247,9 @@ __global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
             if (finish) {
                 const bool has = (bprim != kNone) && (best < rtmax);
                 if (ANY) {
-                    stg1(occ_out + slot, has ? 1u : 0u);
+                    stg1_stream(occ_out + slot, has ? 1u : 0u);
                 } else {
-                    stg4(hits_out + slot, make_float4(has ? best : rtmax, has ? bu : 0.0f, has ? bv : 0.0f,
+                    stg4_stream(hits_out + slot, make_float4(has ? best : rtmax, has ? bu : 0.0f, has ? bv : 0.0f,
                                                       __uint_as_float(has ? bprim : kNone)));
                 }
                 if (COUNT) {
