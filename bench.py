@@ -123,17 +123,20 @@ def main():
     offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     stream = torch.cuda.current_stream()
     nb = len(sizes)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * nb)]
 
-    def plain_step(timed=False, out=None):
+    def events(n):
+        return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+
+    def plain_step(timed=None, out=None):
+        # timed: a list of 2 * nb events (start, end per launch), or None
         out = d_hits if out is None else out
         for b in range(nb):
             o, n = int(offs[b]), int(sizes[b])
             if timed:
-                ev[2 * b].record(stream)
+                timed[2 * b].record(stream)
             scene.intersect(d_rays[o:o + n], out[o:o + n], stream=stream)
             if timed:
-                ev[2 * b + 1].record(stream)
+                timed[2 * b + 1].record(stream)
         return nb
 
     if args.copy_probe:
@@ -160,31 +163,33 @@ def main():
             tuned = batch.autotune(stream=stream)
         log("launch config in %.1fs: %s" % (time.time() - t, tuned))
 
-    def batched_step(timed=False):
+    def batched_step(timed=None):
         if timed:
-            ev[0].record(stream)
+            timed[0].record(stream)
         batch.launch(graph=True, stream=stream)
         if timed:
-            ev[1].record(stream)
+            timed[1].record(stream)
         return 1
 
-    def run(step, steps, warmup):
+    def run(step, steps, warmup, launches):
+        # The steps run back to back (each with its own events), bracketed
+        # by a synchronisation on both sides; the per-launch event times are
+        # read after the second one.
         for _ in range(warmup):
             step()
         torch.cuda.synchronize()
-        launch_ms = []
+        evs = [events(2 * launches) for _ in range(steps)]
         t0 = time.perf_counter()
-        for _ in range(steps):
-            nl = step(timed=True)
-            torch.cuda.synchronize()
-            launch_ms += [ev[2 * b].elapsed_time(ev[2 * b + 1]) for b in range(nl)]
+        for i in range(steps):
+            step(timed=evs[i])
+        torch.cuda.synchronize()
         secs = time.perf_counter() - t0
+        launch_ms = [e[2 * b].elapsed_time(e[2 * b + 1]) for e in evs for b in range(launches)]
         return secs, launch_ms
 
     step = batched_step if args.mode == "batched" else plain_step
-    secs, launch_ms = run(step, args.steps, args.warmup)
-    # per-step wall time includes the host synchronisation used to read the
-    # per-launch events; it is the conservative figure.
+    secs, launch_ms = run(step, args.steps, args.warmup, 1 if args.mode == "batched" else nb)
+    # wall time over the steps, host launches included
     ms_per_step = secs * 1000.0 / args.steps
     value = N * args.steps / secs / 1e6
     kernel_ms_total = float(np.sum(launch_ms)) / args.steps
@@ -195,8 +200,8 @@ def main():
     plain_value = None
     batched_equals_plain = None
     if args.mode == "batched":
-        pstep = lambda timed=False: plain_step(timed, out=d_hits_plain)
-        psecs, _ = run(pstep, max(1, min(args.steps, 3)), 1)
+        pstep = lambda timed=None: plain_step(timed, out=d_hits_plain)
+        psecs, _ = run(pstep, max(1, min(args.steps, 3)), 1, nb)
         plain_value = N * max(1, min(args.steps, 3)) / psecs / 1e6
         # Bitwise, every ray: the timed persistent launch against one launch
         # per batch of the same kernel family.
