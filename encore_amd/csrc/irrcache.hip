@@ -120,14 +120,28 @@ __global__ __launch_bounds__(256) void ic_query(IcLaunch L) {
         const float denom = sqrtCalc + distCalc;
         const float w = denom == 0.0f ? 0.0f : 1.0f / denom;
         if (!(w > L.tol)) continue;
-        // SampleInFront (IrrCacheOctree.h:279-290).
+        // SampleInFront (IrrCacheOctree.h:279-290): the lookup lies in front
+        // unless dot(flush(u / |u|), g) < -0.1, u = p - x_i, g = n_i + n.
+        // The flush moves each component by less than 0.001 and the
+        // rounding of |u|, the divisions and the dot product moves the
+        // result by less than 3e-6 more, so with |g|_1 <= 6 the test passes
+        // whenever dot(u, g) >= -0.0934 |u|.  The unnormalised dot product
+        // r decides that without the square root and the divisions: r >= 0,
+        // or r^2 <= 0.008 d2 (d2 = |u|^2 as computed above, the same float),
+        // which bounds |dot(u, g)| by 0.0895 |u| with the rounding of r, r^2
+        // and 0.008 d2 included.  Otherwise the reference's own operations
+        // decide.
         const float gx = nn.x + nx, gy = nn.y + ny, gz = nn.z + nz;
         float ux = px - pr.x, uy = py - pr.y, uz = pz - pr.z;
-        const float m = __builtin_sqrtf((ux * ux + uy * uy) + uz * uz);
-        ux = flush_ic(ux / m);
-        uy = flush_ic(uy / m);
-        uz = flush_ic(uz / m);
-        if ((ux * gx + uy * gy) + uz * gz < -0.1f) continue;
+        const float r = (ux * gx + uy * gy) + uz * gz;
+        const float g1 = (__builtin_fabsf(gx) + __builtin_fabsf(gy)) + __builtin_fabsf(gz);
+        if (!(g1 <= 6.0f && (r >= 0.0f || r * r <= 0.008f * d2))) {
+            const float m = __builtin_sqrtf((ux * ux + uy * uy) + uz * uz);
+            ux = flush_ic(ux / m);
+            uy = flush_ic(uy / m);
+            uz = flush_ic(uz / m);
+            if ((ux * gx + uy * gy) + uz * gz < -0.1f) continue;
+        }
         const float4 v = L.val[e];
         ws += w;
         ar += v.x * w;

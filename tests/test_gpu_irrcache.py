@@ -106,6 +106,44 @@ def test_irrcache_weight_threshold(device):
     c.destroy()
 
 
+def test_irrcache_front_boundary(device):
+    """Lookups on both sides of SampleInFront's -0.1 bound (IrrCacheOctree.h
+    :279-290): values on a tilted plane, lookups below it at angles whose
+    sines run across 0.05 (the bound for equal normals, where g = 2 n), with
+    equal, slightly turned and clearly turned normals, at distances inside
+    the reach.  The engine decides most of them from the unnormalised dot
+    product and the rest with the reference's operations; counts, weights
+    and sums must equal the oracle's bit for bit, and both outcomes occur."""
+    rng = np.random.default_rng(5)
+    c = rq.IrrCache(device, A, LO, SIZE)
+    o = oracle.IrrCache(A, LO, SIZE)
+    n0 = np.float32([0.2, 0.9, 0.1])
+    n0 /= np.linalg.norm(n0)
+    t0 = np.cross(n0, [0.0, 0.0, 1.0])
+    t0 /= np.linalg.norm(t0)
+    ne = 24
+    pos = np.float32([[-50.0 + 4.0 * i, 1.5 * (i % 5), -30.0 + 2.5 * i] for i in range(ne)])
+    val = rng.uniform(0.1, 2.0, (ne, 3)).astype(np.float32)
+    nrm = np.tile(np.float32(n0), (ne, 1))
+    ri = np.full(ne, 20.0, np.float32)  # reach a * Ri = 1.6
+    assert c.add(val, pos, nrm, ri).tolist() == o.add(val, pos, nrm, ri).tolist() == [0] * ne
+    q, qn = [], []
+    for i in range(ne):
+        for k in range(64):
+            sn = 0.02 + 0.06 * k / 63.0  # sine of the angle below the plane
+            d = 0.3 + 1.2 * ((k * 7) % 64) / 64.0
+            u = (t0 * np.sqrt(1 - sn * sn) - n0 * sn) * d
+            q.append(pos[i] + u.astype(np.float32))
+            m = n0 if k % 3 == 0 else n0 + rng.normal(0, 0.02 if k % 3 == 1 else 0.2, 3)
+            qn.append((m / np.linalg.norm(m)).astype(np.float32))
+    q, qn = np.float32(q), np.float32(qn)
+    got, ref = _query(c, q, qn), o.query(q, qn)
+    log("front boundary: %d lookups, %d counted" % (len(q), int(got[2].sum())))
+    _same(got, ref)
+    assert got[2].max() >= 1 and got[2].min() == 0
+    c.destroy()
+
+
 def test_irrcache_matches_oracle(device):
     rng = np.random.default_rng(21)
     p, nrm = _cornell_points(12000, 5)
