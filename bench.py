@@ -612,6 +612,27 @@ def _final_gather_workload(args, dev, thr, trace_fn):
         out[name + "_rgb"] = rgb.cpu().numpy()
         out[name + "_ri"] = ri.cpu().numpy()
         log("final gather %s: %s" % (name, out[name]))
+    # ENCORE's default configuration (m_bPrecomputeIrr, PrecompIrrValue 1):
+    # the indirect map replaced by its precomputed irradiance
+    # (PhotonMap::PrecomputeIrradiance), and the indirect term at a sample hit
+    # the power of the nearest photon within 10 (1/20: 0.5), which is the
+    # k = 1 estimate of that map (PhotonMapperCPU.cpp:1129-1140).
+    r_pre = 0.5
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    pre, chosen, power = gi.precompute_irradiance(P[ind], N[ind], 1, k_ind, r_ind)
+    pre_s = time.perf_counter() - t
+    log("precomputed irradiance: %d photons in %.3fs" % (len(chosen), pre_s))
+    for name, sc in (("bvh", bvh), ("kd", kdsc)):
+        fn = lambda sc=sc: sc.final_gather(pre, 1, r_pre, gd, k_dir, r_dir, d_dif, d_q, d_dirs, nper, rgb, ri)
+        wall, ev = _time_gpu(fn, args.steps, args.warmup)
+        key = name + "_precomputed"
+        out[key] = {"Mrays_s": round(len(Q) * nper / (ev / 1000.0) / 1e6, 2), "ms": round(ev, 3),
+                    "Mpoints_s": round(len(Q) / (ev / 1000.0) / 1e6, 4)}
+        out[key + "_rgb"] = rgb.cpu().numpy()
+        out[key + "_ri"] = ri.cpu().numpy()
+        log("final gather %s: %s" % (key, out[key]))
+    out["precompute_irradiance_s"] = round(pre_s, 4)
     # CPU baseline and parity on a strided sample of shading points.
     stride = max(1, len(Q) // 2048)
     sel = np.arange(0, len(Q), stride)
@@ -623,13 +644,23 @@ def _final_gather_workload(args, dev, thr, trace_fn):
     o_rgb, o_ri = oracle.final_gather(xyz, idx, kd, oi, k_ind, r_ind, od, k_dir, r_dir, diffuse, Q[sel], sdirs, nper,
                                       nthreads=thr)
     cpu_s = time.perf_counter() - t
+    # the precomputed branch: the oracle gathers over the photons and powers
+    # the GPU's precomputation chose
+    oi_pre = oracle.PhotonMap(np.ascontiguousarray(P[ind][chosen]), np.ascontiguousarray(N[ind][chosen]), power)
+    t = time.perf_counter()
+    o_rgb_pre, o_ri_pre = oracle.final_gather(xyz, idx, kd, oi_pre, 1, r_pre, od, k_dir, r_dir, diffuse, Q[sel],
+                                              sdirs, nper, nthreads=thr)
+    cpu_pre_s = time.perf_counter() - t
     parity = {"sample_points": int(len(sel))}
-    for name in ("kd", "bvh"):
+    for name, ref_rgb, ref_ri in (("kd", o_rgb, o_ri), ("bvh", o_rgb, o_ri), ("kd_precomputed", o_rgb_pre, o_ri_pre),
+                                  ("bvh_precomputed", o_rgb_pre, o_ri_pre)):
         g, gr = out.pop(name + "_rgb")[sel], out.pop(name + "_ri")[sel]
-        parity[name] = {"bit_equal": int(np.all(g.view(np.uint32) == o_rgb.view(np.uint32), axis=1).sum()),
-                        "within_1e-3": int((np.all(np.isclose(g, o_rgb, rtol=1e-3, atol=1e-6), axis=1) &
-                                            np.isclose(gr, o_ri, rtol=1e-3)).sum())}
+        parity[name] = {"bit_equal": int(np.all(g.view(np.uint32) == ref_rgb.view(np.uint32), axis=1).sum()),
+                        "within_1e-3": int((np.all(np.isclose(g, ref_rgb, rtol=1e-3, atol=1e-6), axis=1) &
+                                            np.isclose(gr, ref_ri, rtol=1e-3)).sum())}
     log("parity", parity)
+    out["cpu_precomputed_Mrays_s"] = round(len(sel) * nper / cpu_pre_s / 1e6, 5)
+    pre.destroy()
     _line("final-gather Mrays/s (8x8 samples, indirect + direct estimates)", out["bvh"]["Mrays_s"], "Mrays/s",
           out["bvh"]["ms"],
           {"workload": "final gather at %d shading points x %d hemisphere samples on the Cornell box at 1/20 "

@@ -28,7 +28,8 @@ def fg(device):
     p, n = scenes.photon_queries(xyz, idx, kd.intersect, res=40)
     rng = np.random.default_rng(17)
     diffuse = rng.uniform(0.2, 0.9, (len(idx), 3)).astype(np.float32)
-    yield dict(xyz=xyz, idx=idx, kd=kd, p=p[:1024], n=n[:1024], diffuse=diffuse, **maps)
+    yield dict(xyz=xyz, idx=idx, kd=kd, p=p[:1024], n=n[:1024], diffuse=diffuse, ind_pos=pos[ind], ind_nrm=nrm[ind],
+               **maps)
     maps["ind_g"].destroy()
     maps["dir_g"].destroy()
 
@@ -106,6 +107,57 @@ def test_final_gather_bvh_matches_oracle_up_to_ties(device, fg, width):
     assert c["bad"] == 0
     assert np.all(same[~differ])
     assert differ.mean() < 0.15
+
+R_PRE = 0.5  # the precomputed branch's GetNearestNPhotons(1, 10, ...), at 1/20
+
+
+def test_final_gather_precomputed_irradiance_kd_matches_oracle(device, fg):
+    """ENCORE's default final gather (m_bPrecomputeIrr, set by the
+    constructor and by pmConfig.txt; PhotonMapperCPU.cpp:1129-1140): the
+    indirect term at a sample hit is the power of the nearest photon of the
+    precomputed map within 10 (GetNearestNPhotons(1, 10, ...)), black when
+    there is none.  For a map that is not a caustic map, GetRadianceEstimate
+    with N = 1 returns that photon's power exactly (PhotonMap.cpp:121-164: one
+    power times the weight 1 in double, rounded back to float, density 1), so
+    rq_final_gather with k_ind = 1 on the precomputed map is that branch.  The
+    map is made on the GPU (rq_photons_precompute_irradiance, ENCORE's every
+    photon: PrecompIrrValue 1) and the oracle gathers over the same photons and
+    powers; on the kd scene every result is the reference's bit for bit
+    except where two photons tie for nearest."""
+    f = fg
+    made, chosen, power = f["ind_g"].precompute_irradiance(f["ind_pos"], f["ind_nrm"], 1, K_IND, R_IND,
+                                                           layout="encore")
+    pre_o = oracle.PhotonMap(np.ascontiguousarray(f["ind_pos"][chosen]), np.ascontiguousarray(f["ind_nrm"][chosen]),
+                             power)
+    ns = 4
+    dirs = scenes.hemisphere_dirs(f["n"], ns, scenes.Stream(43))
+    log("case: final gather, precomputed irradiance, kd, %d points x %d samples, %d photons" % (
+        len(f["p"]), ns * ns, len(chosen)))
+    sc = rq.Scene(device, f["xyz"], f["idx"], accel="kd")
+    rgb = torch.full((len(f["p"]), 3), -1.0, device="cuda")
+    ri = torch.full((len(f["p"]),), -1.0, device="cuda")
+    sc.final_gather(made, 1, R_PRE, f["dir_g"], K_DIR, R_DIR, torch.from_numpy(f["diffuse"]).cuda(),
+                    torch.from_numpy(f["p"]).cuda(), torch.from_numpy(dirs).cuda(), ns * ns, rgb, ri)
+    g_rgb, g_ri = rgb.cpu().numpy(), ri.cpu().numpy()
+    o_rgb, o_ri = oracle.final_gather(f["xyz"], f["idx"], f["kd"], pre_o, 1, R_PRE, f["dir_o"], K_DIR, R_DIR,
+                                      f["diffuse"], f["p"], dirs, ns * ns)
+    same = np.all(g_rgb.view(np.uint32) == o_rgb.view(np.uint32), axis=1)
+    log("bit-equal points: %d of %d" % (same.sum(), len(same)))
+    assert np.array_equal(g_ri.view(np.uint32), o_ri.view(np.uint32))
+    assert same.mean() >= 0.995
+    assert np.allclose(g_rgb, o_rgb, rtol=2e-3, atol=1e-6)
+    # the indirect term alone: the nearest photon's power, bit for bit
+    rgb_i = torch.full((len(f["p"]), 3), -1.0, device="cuda")
+    sc.final_gather(made, 1, R_PRE, None, K_DIR, R_DIR, torch.from_numpy(f["diffuse"]).cuda(),
+                    torch.from_numpy(f["p"]).cuda(), torch.from_numpy(dirs).cuda(), ns * ns, rgb_i)
+    o_i, _ = oracle.final_gather(f["xyz"], f["idx"], f["kd"], pre_o, 1, R_PRE, None, K_DIR, R_DIR, f["diffuse"],
+                                 f["p"], dirs, ns * ns)
+    same_i = np.all(rgb_i.cpu().numpy().view(np.uint32) == o_i.view(np.uint32), axis=1)
+    assert same_i.mean() >= 0.995
+    assert (o_i > 0).any(axis=1).mean() > 0.9  # the test exercises real lookups
+    sc.destroy()
+    made.destroy()
+
 
 def test_final_gather_chunking_is_invisible(device, fg):
     """A call whose sample rays span several 4M-ray chunks returns exactly what

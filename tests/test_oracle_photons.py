@@ -114,6 +114,29 @@ def test_radiance_estimate_closed_form():
         np.testing.assert_allclose(rgbc[i], ref, rtol=1e-5)
 
 
+def test_single_photon_estimate_is_its_power():
+    """ENCORE's default final gather reads the nearest photon of the
+    precomputed map (GetNearestNPhotons(1, ...)[0].Power(),
+    PhotonMapperCPU.cpp:1129-1140).  GetRadianceEstimate with N = 1 on a map
+    that is not a caustic map adds that one power times the weight 1 in double
+    and rounds it back to float, with density 1 (PhotonMap.cpp:121-164), so
+    the two are the same bits; with no photon in range both are black."""
+    pos, nrm, pw = _random_photons(20000, 12, nclasses=3)
+    pm = oracle.PhotonMap(pos, nrm, pw)
+    perm, _ = pm.order()
+    rng = np.random.default_rng(13)
+    q = rng.uniform(0.0, 1.0, (3000, 3)).astype(np.float32)
+    qn = nrm[rng.integers(0, len(nrm), len(q))]
+    for r in (0.5, 0.004):
+        rgb = pm.radiance(q, qn, 1, r)
+        slots, _, cnt = pm.knn(q, qn, 1, r)
+        found = cnt > 0
+        assert np.array_equal(rgb[found].view(np.uint32), pw[perm[slots[found, 0]]].view(np.uint32))
+        assert np.all(rgb[~found] == 0.0)
+        if r < 0.01:
+            assert (~found).sum() > 100  # both outcomes occur
+
+
 def test_photon_workload_generator():
     xyz, idx = scenes.config4_scene()
     assert len(idx) == 34816
