@@ -400,37 +400,54 @@ __device__ __forceinline__ double lane_xor(double x) {
     return __longlong_as_double((long long)lane_xor<J>((uint64_t)__double_as_longlong(x)));
 }
 
-// One compare-exchange step of a bitonic sort: element e (lane e, or lane
-// e - 64 of the second register) and element e ^ J, within a run of SIZE
-// elements sorted ascending when e & SIZE is 0 (always, for SIZE = 128).
-template <uint32_t SIZE, uint32_t J, typename T>
-__device__ __forceinline__ T bitonic_step(T x, uint32_t e) {
+// The lanes at which a compare-exchange step of a bitonic sort keeps the
+// smaller of the pair: element e (lane e of the first register, lane e - 64
+// of the second, BASE = 64) pairs with element e ^ J within a run of SIZE
+// elements that is sorted ascending when e & SIZE is 0 (always, for SIZE =
+// 128), and keeps the smaller when its own bit J is clear in an ascending run
+// or set in a descending one.  A compile-time constant: the step selects with
+// it as a lane mask instead of testing lane bits.
+template <uint32_t SIZE, uint32_t J, uint32_t BASE>
+constexpr uint64_t bitonic_min_lanes() {
+    uint64_t m = 0;
+    for (uint32_t l = 0; l < 64; ++l) {
+        const uint32_t e = l + BASE;
+        const bool up = SIZE >= 128 || (e & SIZE) == 0;
+        if (((e & J) == 0) == up) m |= 1ull << l;
+    }
+    return m;
+}
+
+// One compare-exchange step: a lane keeps its own value when (own < other)
+// agrees with keeping the smaller, else the other's (equal values: either).
+template <uint32_t SIZE, uint32_t J, uint32_t BASE, typename T>
+__device__ __forceinline__ T bitonic_step(T x) {
     const T p = lane_xor<J>(x);
-    const bool up = SIZE >= 128 || (e & SIZE) == 0;
-    return ((e & J) == 0) == up ? min(x, p) : max(x, p);
+    const uint64_t keep_own = ~(__ballot(x < p) ^ bitonic_min_lanes<SIZE, J, BASE>());
+    return __builtin_amdgcn_inverse_ballot_w64(keep_own) ? x : p;
 }
 
 template <uint32_t SIZE, uint32_t J, typename T>
-__device__ __forceinline__ void bitonic_merge(T& x0, T& x1, uint32_t lane, bool two) {
+__device__ __forceinline__ void bitonic_merge(T& x0, T& x1, bool two) {
     if constexpr (J == 64) {
         const T lo = min(x0, x1), hi = max(x0, x1);
         x0 = lo;
         x1 = hi;
     } else {
-        x0 = bitonic_step<SIZE, J>(x0, lane);
-        if (two) x1 = bitonic_step<SIZE, J>(x1, lane + 64u);
+        x0 = bitonic_step<SIZE, J, 0>(x0);
+        if (two) x1 = bitonic_step<SIZE, J, 64>(x1);
     }
-    if constexpr (J > 1) bitonic_merge<SIZE, J / 2>(x0, x1, lane, two);
+    if constexpr (J > 1) bitonic_merge<SIZE, J / 2>(x0, x1, two);
 }
 
 // Sorts 64 keys (one per lane) ascending, or 128 (x0 then x1) when `two`
-// (wave-uniform).
+// (wave-uniform).  Every lane of the wave takes part.
 template <uint32_t SIZE = 2, typename T>
-__device__ __forceinline__ void bitonic_sort(T& x0, T& x1, uint32_t lane, bool two) {
+__device__ __forceinline__ void bitonic_sort(T& x0, T& x1, bool two) {
     if constexpr (SIZE <= 128) {
         if (SIZE == 128 && !two) return;
-        bitonic_merge<SIZE, SIZE / 2>(x0, x1, lane, two);
-        bitonic_sort<SIZE * 2>(x0, x1, lane, two);
+        bitonic_merge<SIZE, SIZE / 2>(x0, x1, two);
+        bitonic_sort<SIZE * 2>(x0, x1, two);
     }
 }
 
@@ -492,7 +509,7 @@ __device__ void wave_finish(const KnnLaunch& L, uint32_t q, uint32_t cand, bool 
         // order.  Sorted position `lane` holds the lane's photon.
         uint64_t x0 = lane < cand ? ((uint64_t)s_key[lane] << 7) | lane : ~0ull;
         uint64_t x1 = lane + 64u < cand ? ((uint64_t)s_key[lane + 64u] << 7) | (lane + 64u) : ~0ull;
-        bitonic_sort(x0, x1, lane, cand > 64u);
+        bitonic_sort(x0, x1, cand > 64u);
         const uint32_t kept = min(cand, k);
         const uint32_t key = (uint32_t)(x0 >> 7);
         const uint32_t slot = lane < kept ? s_slot[(uint32_t)x0 & 127u] : 0u;
@@ -909,7 +926,7 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
                 // sorted position e tells slot (key & 127) its rank e through
                 // s_hist, which is free until the first compaction.
                 uint32_t x0 = v ? key : 0xFFFFFFFFu, x1 = v1 ? key1 : 0xFFFFFFFFu;
-                bitonic_sort(x0, x1, lane, vm1 != 0);
+                bitonic_sort(x0, x1, vm1 != 0);
                 if (lane < ncell) s_hist[x0 & 127u] = lane;
                 if (lane + 64u < ncell) s_hist[x1 & 127u] = lane + 64u;
                 __builtin_amdgcn_wave_barrier();
