@@ -788,28 +788,40 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
         if (cls != kClassNever && isfinite(qx) && isfinite(qy) && isfinite(qz)) {
             float r2 = L.r2;
             const double qd[3] = {(double)qx, (double)qy, (double)qz};
-            const uint32_t off[3] = {lane & 3u, (lane >> 2) & 3u, lane >> 4};
-            bool v = true;
+            // Lane j < 12 works out axis j >> 2 at cell offset j & 3: the
+            // cell's coordinate, whether it lies within the grid and the
+            // reach, and the squared distance from the query to its widened
+            // box.  A lane's own cell is (lane & 3, (lane >> 2) & 3,
+            // lane >> 4); it takes its three pairs from those lanes.
+            const uint32_t la = min(lane >> 2, 2u);
+            const double qa = la == 0 ? qd[0] : la == 1 ? qd[1] : qd[2];
+            const double oa = la == 0 ? G.origin[0] : la == 1 ? G.origin[1] : G.origin[2];
+            const int64_t dim = la == 0 ? G.dims[0] : la == 1 ? G.dims[1] : G.dims[2];
+            // widening for assignment and for the rounding of these sums
+            const double m = G.cell * 0x1p-20 + (fabs(oa) + fabs(qa)) * 0x1p-48;
+            const double w = G.reach + fabs(qa) * 0x1p-40;
+            const int64_t lo = grid_coord(qa - w, oa, G.inv_cell);
+            const int64_t hi = grid_coord(qa + w, oa, G.inv_cell);
+            // More than 4 cells only when |q| exceeds 2^38 cells; the grid is
+            // refused for maps whose coordinates come near that, so such a
+            // query is farther than the reach from every photon.
+            far = __ballot(lane < 12u && hi - lo > 3) != 0;
+            const int64_t ca = lo + (int64_t)(lane & 3u);
+            const uint32_t okm = (uint32_t)__ballot(lane < 12u && ca <= hi && ca >= 0 && ca < dim);
+            const double blo = oa + (double)ca * G.cell - m;
+            const double bhi = oa + (double)(ca + 1) * G.cell + m;
+            const double da = fmax(fmax(blo - qa, qa - bhi), 0.0);
+            const double dsq = da * da;
+            const uint32_t i0 = lane & 3u, i1 = 4u + ((lane >> 2) & 3u), i2 = 8u + (lane >> 4);
+            bool v = ((okm >> i0) & (okm >> i1) & (okm >> i2) & 1u) != 0;
             double near2 = 0.0;
+            near2 += __shfl(dsq, (int)i0, 64);
+            near2 += __shfl(dsq, (int)i1, 64);
+            near2 += __shfl(dsq, (int)i2, 64);
             uint64_t c[3];
-            for (int a = 0; a < 3; ++a) {
-                // widening for assignment and for the rounding of these sums
-                const double m = G.cell * 0x1p-20 + (fabs(G.origin[a]) + fabs(qd[a])) * 0x1p-48;
-                const double w = G.reach + fabs(qd[a]) * 0x1p-40;
-                const int64_t lo = grid_coord(qd[a] - w, G.origin[a], G.inv_cell);
-                const int64_t hi = grid_coord(qd[a] + w, G.origin[a], G.inv_cell);
-                // More than 4 cells only when |q| exceeds 2^38 cells; the grid
-                // is refused for maps whose coordinates come near that, so such
-                // a query is farther than the reach from every photon.
-                far |= hi - lo > 3;
-                const int64_t ca = lo + (int64_t)off[a];
-                v = v && ca <= hi && ca >= 0 && ca < (int64_t)G.dims[a];
-                c[a] = (uint64_t)max(ca, (int64_t)0);
-                const double blo = G.origin[a] + (double)ca * G.cell - m;
-                const double bhi = G.origin[a] + (double)(ca + 1) * G.cell + m;
-                const double d = fmax(fmax(blo - qd[a], qd[a] - bhi), 0.0);
-                near2 += d * d;
-            }
+            c[0] = (uint64_t)max(__builtin_amdgcn_readlane(lo, 0) + (int64_t)(lane & 3u), (int64_t)0);
+            c[1] = (uint64_t)max(__builtin_amdgcn_readlane(lo, 4) + (int64_t)((lane >> 2) & 3u), (int64_t)0);
+            c[2] = (uint64_t)max(__builtin_amdgcn_readlane(lo, 8) + (int64_t)(lane >> 4), (int64_t)0);
             near2 *= 1.0 - 0x1p-20;
             v = v && near2 < (double)r2 && !far;
             uint2 range = make_uint2(0u, 0u), range1 = make_uint2(0u, 0u);  // slots lane, lane + 64
