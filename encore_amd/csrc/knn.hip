@@ -872,17 +872,19 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
                         uint32_t bf = 0, bc = 0;
                         if (j < nblk) {
                             const float4 lo = G.blocks[2 * (size_t)b], hi = G.blocks[2 * (size_t)b + 1];
-                            const double l[3] = {(double)lo.x, (double)lo.y, (double)lo.z};
-                            const double h[3] = {(double)hi.x, (double)hi.y, (double)hi.z};
-                            double nr = 0.0;
-                            for (int a = 0; a < 3; ++a) {
-                                const double d = fmax(fmax(l[a] - qd[a], qd[a] - h[a]), 0.0);
-                                nr += d * d;
-                            }
-                            // a photon's float d2 is within 2^-21 of its distance
-                            nr *= 1.0 - 0x1p-20;
-                            keep = nr < (double)r2;
-                            nf = __double2float_rd(nr);
+                            // The squared distance to the box with a photon's
+                            // float d2 operations: per axis the difference to
+                            // the nearer face (the box is the photons' float
+                            // min and max, so a photon's difference is at
+                            // least as large), squared, summed in the same
+                            // order.  Rounding is monotonic, so no photon of
+                            // the block has a smaller float d2.
+                            const float dx = fmaxf(fmaxf(lo.x - qx, qx - hi.x), 0.0f);
+                            const float dy = fmaxf(fmaxf(lo.y - qy, qy - hi.y), 0.0f);
+                            const float dz = fmaxf(fmaxf(lo.z - qz, qz - hi.z), 0.0f);
+                            const float nr = (dx * dx + dy * dy) + dz * dz;
+                            keep = nr < r2;
+                            nf = nr;
                             bf = __float_as_uint(lo.w);
                             bc = __float_as_uint(hi.w);
                         }
