@@ -796,17 +796,20 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
             const uint32_t la = min(lane >> 2, 2u);
             const double qa = la == 0 ? qd[0] : la == 1 ? qd[1] : qd[2];
             const double oa = la == 0 ? G.origin[0] : la == 1 ? G.origin[1] : G.origin[2];
-            const int64_t dim = la == 0 ? G.dims[0] : la == 1 ? G.dims[1] : G.dims[2];
+            const int32_t dim = (int32_t)(la == 0 ? G.dims[0] : la == 1 ? G.dims[1] : G.dims[2]);
             // widening for assignment and for the rounding of these sums
             const double m = G.cell * 0x1p-20 + (fabs(oa) + fabs(qa)) * 0x1p-48;
             const double w = G.reach + fabs(qa) * 0x1p-40;
-            const int64_t lo = grid_coord(qa - w, oa, G.inv_cell);
-            const int64_t hi = grid_coord(qa + w, oa, G.inv_cell);
+            // Cell coordinates (grid_coord) clamped to +-2^30: the grid has
+            // fewer than 2^21 cells along an axis, so a clamped coordinate
+            // lies outside it either way.
+            const int32_t lo = (int32_t)fmin(fmax(floor((qa - w - oa) * G.inv_cell), -0x1p30), 0x1p30);
+            const int32_t hi = (int32_t)fmin(fmax(floor((qa + w - oa) * G.inv_cell), -0x1p30), 0x1p30);
             // More than 4 cells only when |q| exceeds 2^38 cells; the grid is
             // refused for maps whose coordinates come near that, so such a
             // query is farther than the reach from every photon.
             far = __ballot(lane < 12u && hi - lo > 3) != 0;
-            const int64_t ca = lo + (int64_t)(lane & 3u);
+            const int32_t ca = lo + (int32_t)(lane & 3u);
             const uint32_t okm = (uint32_t)__ballot(lane < 12u && ca <= hi && ca >= 0 && ca < dim);
             const double blo = oa + (double)ca * G.cell - m;
             const double bhi = oa + (double)(ca + 1) * G.cell + m;
@@ -818,10 +821,10 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
             near2 += __shfl(dsq, (int)i0, 64);
             near2 += __shfl(dsq, (int)i1, 64);
             near2 += __shfl(dsq, (int)i2, 64);
-            uint64_t c[3];
-            c[0] = (uint64_t)max(__builtin_amdgcn_readlane(lo, 0) + (int64_t)(lane & 3u), (int64_t)0);
-            c[1] = (uint64_t)max(__builtin_amdgcn_readlane(lo, 4) + (int64_t)((lane >> 2) & 3u), (int64_t)0);
-            c[2] = (uint64_t)max(__builtin_amdgcn_readlane(lo, 8) + (int64_t)(lane >> 4), (int64_t)0);
+            // the lane's cell (used where it lies within the grid)
+            const uint32_t c[3] = {(uint32_t)(__builtin_amdgcn_readlane(lo, 0) + (int32_t)(lane & 3u)),
+                                   (uint32_t)(__builtin_amdgcn_readlane(lo, 4) + (int32_t)((lane >> 2) & 3u)),
+                                   (uint32_t)(__builtin_amdgcn_readlane(lo, 8) + (int32_t)(lane >> 4))};
             near2 *= 1.0 - 0x1p-20;
             v = v && near2 < (double)r2 && !far;
             uint2 range = make_uint2(0u, 0u), range1 = make_uint2(0u, 0u);  // slots lane, lane + 64
@@ -831,7 +834,8 @@ __global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch 
             uint32_t fb = 0;  // the cell's first block (grids with blocks)
             if (v) {
                 if (G.dense) {
-                    const uint64_t lin = c[0] + (uint64_t)G.dims[0] * (c[1] + (uint64_t)G.dims[1] * c[2]);
+                    // below 2^25 entries (the dense table's limit)
+                    const uint32_t lin = c[0] + G.dims[0] * (c[1] + G.dims[1] * c[2]);
                     range = G.dense[lin];
                     if (G.blocks) fb = G.dense_block[lin];
                 } else {
