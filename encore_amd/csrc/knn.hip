@@ -24,6 +24,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 
 #include "rq_internal.h"
@@ -58,7 +59,8 @@ __global__ __launch_bounds__(64) void knn_kernel(KnnLaunch L) {
     uint32_t* hs = smem + (size_t)k * kKnnWave + lane;            // heap slot
     uint32_t* st = smem + (size_t)2 * k * kKnnWave + lane;        // stack, word w of entry e at st[(e*3+w)*64]
     uint32_t* spill = L.spill + ((size_t)blockIdx.x * kKnnWave) * L.spill_depth * 3 + lane;
-    for (uint32_t q = blockIdx.x * kKnnWave + lane; q < L.nq; q += gridDim.x * kKnnWave) {
+    for (uint32_t qi = blockIdx.x * kKnnWave + lane; qi < L.nq; qi += gridDim.x * kKnnWave) {
+    const uint32_t q = L.order ? L.order[qi] : qi;  // neighbouring lanes take neighbouring queries
     const float qx = L.qpos[3 * (size_t)q], qy = L.qpos[3 * (size_t)q + 1], qz = L.qpos[3 * (size_t)q + 2];
     const uint32_t cls = lookup_class(L.table, L.table_mask, L.qnrm[3 * (size_t)q], L.qnrm[3 * (size_t)q + 1],
                                       L.qnrm[3 * (size_t)q + 2]);
@@ -591,7 +593,8 @@ __global__ __launch_bounds__(64) void knn_wave_kernel(KnnLaunch L, uint32_t stac
     st.lo = L.spill + (size_t)blockIdx.x * 3 * stack_cap;
     st.hi = st.lo + stack_cap;
     st.dp = st.hi + stack_cap;
-    for (uint32_t q = blockIdx.x; q < L.nq; q += gridDim.x) {
+    for (uint32_t qi = blockIdx.x; qi < L.nq; qi += gridDim.x) {
+        const uint32_t q = L.order ? L.order[qi] : qi;
         const float qx = L.qpos[3 * (size_t)q], qy = L.qpos[3 * (size_t)q + 1], qz = L.qpos[3 * (size_t)q + 2];
         const uint32_t cls = lookup_class(L.table, L.table_mask, L.qnrm[3 * (size_t)q], L.qnrm[3 * (size_t)q + 1],
                                           L.qnrm[3 * (size_t)q + 2]);
@@ -1116,6 +1119,24 @@ __global__ void knn_cell_prep(KnnLaunch L, float4* prep) {
                           __uint_as_float(cls));
 }
 
+// Queries of the tree walks in the order of a 30-bit Morton code of their
+// position within the map's box (clamped to it; ~0 for a position that is
+// not finite), so that the lanes of a wave walk nearby parts of the tree.
+__global__ void knn_tree_keys(KnnLaunch L, float3 lo, float3 inv, uint32_t* keys, uint32_t* ids) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.nq) return;
+    const float x[3] = {L.qpos[3 * (size_t)i], L.qpos[3 * (size_t)i + 1], L.qpos[3 * (size_t)i + 2]};
+    uint32_t key = 0xFFFFFFFFu;
+    if (isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2])) {
+        const float o[3] = {lo.x, lo.y, lo.z}, s[3] = {inv.x, inv.y, inv.z};
+        uint32_t c[3];
+        for (int a = 0; a < 3; ++a) c[a] = (uint32_t)fminf(fmaxf((x[a] - o[a]) * s[a], 0.0f), 1023.0f);
+        key = (uint32_t)grid_spread21(c[0]) | (uint32_t)grid_spread21(c[1]) << 1 | (uint32_t)grid_spread21(c[2]) << 2;
+    }
+    keys[i] = key;
+    ids[i] = i;
+}
+
 __global__ void knn_cell_keys(KnnLaunch L, unsigned long long* keys, uint32_t* ids) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= L.nq) return;
@@ -1157,6 +1178,33 @@ void knn_cell_order(const KnnLaunch& L, uint32_t* order, void* scratch, size_t s
     hipLaunchKernelGGL(dev::knn_cell_keys, dim3((nq + 255) / 256), dim3(256), 0, s, L, keys, ids);
     RQ_HIP(hipGetLastError());
     RQ_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, skeys, ids, order, (int)nq, 0, 64, s));
+}
+
+size_t knn_tree_order_bytes(uint32_t nq) {
+    size_t tmp = 0;
+    RQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                              (uint32_t*)nullptr, (uint32_t*)nullptr, (int)nq, 0, 32));
+    return 3 * align256((size_t)nq * 4) + align256(std::max<size_t>(tmp, 16));
+}
+
+void knn_tree_order(const KnnLaunch& L, const float bmin[3], const float bmax[3], uint32_t* order, void* scratch,
+                    size_t scratch_bytes, hipStream_t s) {
+    const uint32_t nq = L.nq;
+    char* w = static_cast<char*>(scratch);
+    auto* keys = reinterpret_cast<uint32_t*>(w);
+    auto* skeys = reinterpret_cast<uint32_t*>(w + align256((size_t)nq * 4));
+    auto* ids = reinterpret_cast<uint32_t*>(w + 2 * align256((size_t)nq * 4));
+    void* tmp = w + 3 * align256((size_t)nq * 4);
+    size_t tmp_bytes = scratch_bytes - (size_t)(static_cast<char*>(tmp) - w);
+    float inv[3];
+    for (int a = 0; a < 3; ++a) {
+        const float ext = bmax[a] - bmin[a];
+        inv[a] = ext > 0.0f && std::isfinite(ext) ? 1024.0f / ext : 0.0f;
+    }
+    hipLaunchKernelGGL(dev::knn_tree_keys, dim3((nq + 255) / 256), dim3(256), 0, s, L,
+                       make_float3(bmin[0], bmin[1], bmin[2]), make_float3(inv[0], inv[1], inv[2]), keys, ids);
+    RQ_HIP(hipGetLastError());
+    RQ_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, skeys, ids, order, (int)nq, 0, 30, s));
 }
 
 uint32_t knn_wave_stack_cap(uint32_t n) {

@@ -729,9 +729,14 @@ static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint
     const int grid = wave ? knn_wave_grid(nq, dev->num_cus) : knn_grid(nq, dev->num_cus, k);
     const uint32_t depth = knn_spill_words_per_lane(p->n);
     const uint32_t cap = knn_wave_stack_cap(p->n);
-    const size_t scratch_bytes = wave ? (size_t)grid * 3 * cap * sizeof(uint32_t)
-                                      : (size_t)grid * 64 * depth * 3 * sizeof(uint32_t);
-    uint32_t* scratch = stream_scratch(dev, s, scratch_bytes);
+    const size_t spill_bytes = ((wave ? (size_t)grid * 3 * cap * sizeof(uint32_t)
+                                      : (size_t)grid * 64 * depth * 3 * sizeof(uint32_t)) + 255) & ~(size_t)255;
+    // Many queries are walked in the Morton order of their positions (the
+    // results do not depend on the order).
+    const bool sorted = nq >= kKnnSortMin && knn_cells_sorted() && p->nvalid > 0;
+    const size_t order_bytes = sorted ? ((size_t)nq * 4 + 255) & ~(size_t)255 : 0;
+    const size_t sort_bytes = sorted ? knn_tree_order_bytes(nq) : 0;
+    uint32_t* scratch = stream_scratch(dev, s, spill_bytes + order_bytes + sort_bytes);
     KnnLaunch L{};
     L.hot = p->hot.as<PhotonHot>();
     L.power = p->power.as<PhotonPower>();
@@ -751,6 +756,12 @@ static void knn_common(rq_photons* p, const float* qpos, const float* qnrm, uint
     L.caustic = caustic;
     L.spill = scratch + 64;
     L.spill_depth = depth;
+    if (sorted) {
+        char* w = reinterpret_cast<char*>(scratch + 64) + spill_bytes;
+        uint32_t* order = reinterpret_cast<uint32_t*>(w);
+        knn_tree_order(L, p->bmin, p->bmax, order, w + order_bytes, sort_bytes, s);
+        L.order = order;
+    }
     if (wave)
         launch_knn_wave(L, grid, cap, s);
     else

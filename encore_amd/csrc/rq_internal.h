@@ -260,7 +260,7 @@ struct KnnLaunch {
     uint32_t* spill;     // per-lane stack spill
     uint32_t spill_depth;
     PhotonGridView grid;    // knn_cell_kernel only
-    const uint32_t* order;  // knn_cell_kernel: queries in this order (null: 0..nq-1)
+    const uint32_t* order;  // queries served in this order (null: 0..nq-1)
     const float4* qprep;    // knn_cell_kernel: position and class per served query
     unsigned long long* tally;  // knn_cell_kernel counting copy: queries, cells, photons read, candidates
     uint32_t compact_slack;     // knn_cell_kernel: compact at k + this many candidates (0: 64)
@@ -286,6 +286,11 @@ void launch_knn_cells(const KnnLaunch& L, float4* prep, int grid, hipStream_t s)
 // the sort (order receives nq query indices; scratch holds the rest).
 size_t knn_cell_order_bytes(uint32_t nq);
 void knn_cell_order(const KnnLaunch& L, uint32_t* order, void* scratch, size_t scratch_bytes, hipStream_t s);
+// Tree walks: the queries sorted by a Morton code of their position within
+// [bmin, bmax] (scratch of knn_tree_order_bytes(nq)).
+size_t knn_tree_order_bytes(uint32_t nq);
+void knn_tree_order(const KnnLaunch& L, const float bmin[3], const float bmax[3], uint32_t* order, void* scratch,
+                    size_t scratch_bytes, hipStream_t s);
 int knn_wave_grid(uint32_t nq, int num_cus);
 void launch_knn_wave(const KnnLaunch& L, int grid, uint32_t stack_cap, hipStream_t s);
 

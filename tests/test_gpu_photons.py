@@ -83,6 +83,25 @@ def test_knn_k_range(device, c4, k, r):
     _compare(P, N, W, Q[:3000], QN[:3000], k, r, gi, gd, gc)
 
 
+@pytest.mark.parametrize("k,r", [(1, 0.5), (8, 0.05)])
+def test_knn_tree_sorted_queries(device, c4, k, r, monkeypatch):
+    """Calls of at least 16,384 queries that the tree walks serve (k below
+    32) take the queries in the Morton order of their positions: every
+    output equals the caller-order walk's (RQ_KNN_SORT=0) bit for bit, and
+    the sets equal the oracle's."""
+    P, N, W, Q, QN = c4
+    assert len(Q) >= 16384
+    log("case: knn tree walk, sorted queries, k=%d r=%g, %d queries" % (k, r, len(Q)))
+    pm = rq.PhotonMap(device, P, N, W)
+    a = _gpu_knn(pm, Q, QN, k, r, radiance=True)
+    monkeypatch.setenv("RQ_KNN_SORT", "0")
+    b = _gpu_knn(pm, Q, QN, k, r, radiance=True)
+    for x, y in zip(a, b):
+        assert np.array_equal(np.ascontiguousarray(x).view(np.uint8), np.ascontiguousarray(y).view(np.uint8))
+    _compare(P, N, W, Q, QN, k, r, *a[:3])
+    assert a[2].mean() > 0.5
+
+
 @pytest.mark.parametrize("caustic", [False, True])
 def test_radiance_estimate(device, c4, caustic):
     P, N, W, Q, QN = c4
