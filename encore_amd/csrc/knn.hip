@@ -724,16 +724,18 @@ __device__ __forceinline__ uint2 cell_lookup(const PhotonGridView& G, uint64_t k
     return make_uint2(0u, 0u);
 }
 
-// Candidates: compaction at k + 64 (a sweep may ask for more, up to 512),
-// plus one chunk of 64.
-constexpr int kCellCand = 2 * (int)kKnnMaxK + 64;
+// Candidates: compaction at min(k + 64, 384) (a sweep may ask for k + more,
+// up to 384), plus one chunk of 64.  448 entries keep a 4-wave workgroup's
+// LDS at 25.6 KB, so that 6 workgroups fit a CU's 160 KB.
+constexpr int kCellCand = 448;
 
 // Workgroups of kCellWaves waves, one query per wave: a workgroup's waves
 // are spread over the CU's SIMDs, so a cap of whole workgroups per CU puts
-// the same number of waves on every SIMD.  The launch bound asks for 5
-// waves per SIMD: the kernel then compiles to 96 VGPRs and 36 bytes of
-// scratch per lane (a few values spilled), and runs 6% faster at config 4
-// and 5% faster in the final gather than unbounded at 110 VGPRs, 4 waves.
+// the same number of waves on every SIMD.  The launch bound asks for 6
+// waves per SIMD: the kernel then compiles to 80 VGPRs and 52 bytes of
+// scratch per lane (a few values spilled).  At config 4 (r = 0.25) that ran
+// 304-306 M queries/s against 285-287 at 5 waves (96 VGPRs, no spill) and
+// 295 at 7 (72 VGPRs, 104 bytes spilled); the final gather 228 against 221.
 constexpr int kCellWaves = 4;
 // Blocks a query may examine before it keeps to whole cells.
 constexpr uint32_t kCellMaxBlocks = 256;
@@ -749,7 +751,7 @@ constexpr uint32_t kCellRankByCount = 16;
 // or blocks), the photons read and the candidates (L.tally[0..3]: queries,
 // ranges, photons, candidates).
 template <bool COUNT>
-__global__ __launch_bounds__(64 * kCellWaves, 5) void knn_cell_kernel(KnnLaunch L) {
+__global__ __launch_bounds__(64 * kCellWaves, 6) void knn_cell_kernel(KnnLaunch L) {
     __shared__ uint32_t s_key_w[kCellWaves][kCellCand];
     __shared__ uint32_t s_slot_w[kCellWaves][kCellCand];
     __shared__ uint32_t s_hist_w[kCellWaves][256];
@@ -1229,12 +1231,13 @@ int knn_cell_grid(uint32_t nq, int num_cus) {
     int per_cu = 0;
     RQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)dev::knn_cell_kernel<false>,
                                                         dev::kKnnWave * dev::kCellWaves, 0));
-    // Five waves per SIMD (20 per CU): with one-wave workgroups the 4M-query
-    // final gather ran 160 M rays/s at 5 waves per SIMD, 146-149 M when 20
-    // workgroups per CU could land 6, 6, 6, 2 on the SIMDs, and 113 M with
-    // 24 per CU (more waves evict each other's cells).  RQ_KNN_CELL_WAVES
-    // lowers the count for sweeps.
-    int cap = 20;
+    // Six waves per SIMD (24 per CU, as the launch bound compiles for).
+    // With one-wave workgroups, before the kernel's blocks and its present
+    // register count, the 4M-query final gather ran 160 M rays/s at 5 waves
+    // per SIMD, 146-149 M when 20 workgroups per CU could land 6, 6, 6, 2 on
+    // the SIMDs, and 113 M with 24 per CU.  RQ_KNN_CELL_WAVES sets the count
+    // for sweeps.
+    int cap = 24;
     if (const char* e = std::getenv("RQ_KNN_CELL_WAVES")) cap = std::atoi(e);
     per_cu = std::max(1, std::min(per_cu, std::max(1, cap / dev::kCellWaves)));
     const int64_t need = ((int64_t)nq + dev::kCellWaves - 1) / dev::kCellWaves;
