@@ -724,8 +724,8 @@ __device__ __forceinline__ uint2 cell_lookup(const PhotonGridView& G, uint64_t k
     return make_uint2(0u, 0u);
 }
 
-// Candidates: compaction at min(k + 64, 384) (a sweep may ask for k + more,
-// up to 384), plus one chunk of 64.  448 entries keep a 4-wave workgroup's
+// Candidates: compaction at min(k + 64, 384) for k <= 64, min(k + 128, 384)
+// above (a sweep may ask for k + more, up to 384), plus one chunk of 64.  448 entries keep a 4-wave workgroup's
 // LDS at 25.6 KB, so that 6 workgroups fit a CU's 160 KB.
 constexpr int kCellCand = 448;
 
@@ -769,8 +769,12 @@ __global__ __launch_bounds__(64 * kCellWaves, 6) void knn_cell_kernel(KnnLaunch 
     const PhotonGridView& G = L.grid;
     const uint32_t lane = __lane_id();
     const uint32_t k = L.k;
-    const uint32_t compact_at =
-        min((uint32_t)kCellCand - 64u, k + (L.compact_slack ? L.compact_slack : 64u));
+    // Compaction at k + 64 for k <= 64 (the final sort then holds at most
+    // 128 candidates) and k + 128 above (fewer compactions before the final
+    // selection: the final gather's k = 100 estimates ran 228 M rays/s at
+    // k + 64, 233-235 at k + 128, 232 at k + 192, 227 at k + 256).
+    const uint32_t compact_at = min((uint32_t)kCellCand - 64u,
+                                    k + (L.compact_slack ? L.compact_slack : (k <= 64u ? 64u : 128u)));
     // Grid-stride over the prepared queries (cell order when sorted): the
     // waves running at one time hold a window of consecutive queries, which
     // share cells.  The next query's record is loaded a query ahead.
@@ -1251,7 +1255,8 @@ void launch_knn_cells(const KnnLaunch& L, float4* prep, int grid, hipStream_t s)
     KnnLaunch C = L;
     C.qprep = prep;
     // RQ_KNN_COMPACT_SLACK: compact the candidates once they exceed k by this
-    // many (default 64), for sweeps; results do not depend on it.
+    // many (default 64, or 128 for k > 64), for sweeps; results do not depend
+    // on it.
     if (const char* e = std::getenv("RQ_KNN_COMPACT_SLACK")) C.compact_slack = (uint32_t)std::max(1, std::atoi(e));
     if (C.tally)
         hipLaunchKernelGGL(dev::knn_cell_kernel<true>, dim3(grid), dim3(dev::kKnnWave * dev::kCellWaves), 0, s, C);

@@ -263,7 +263,7 @@ struct KnnLaunch {
     const uint32_t* order;  // queries served in this order (null: 0..nq-1)
     const float4* qprep;    // knn_cell_kernel: position and class per served query
     unsigned long long* tally;  // knn_cell_kernel counting copy: queries, cells, photons read, candidates
-    uint32_t compact_slack;     // knn_cell_kernel: compact at k + this many candidates (0: 64)
+    uint32_t compact_slack;     // knn_cell_kernel: compact at k + this many candidates (0: 64, or 128 for k > 64)
 };
 
 void launch_knn(const KnnLaunch& L, int grid, hipStream_t s);
