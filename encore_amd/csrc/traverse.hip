@@ -973,8 +973,14 @@ __device__ __forceinline__ uint32_t fetch_capped(uint32_t cur, uint32_t* counter
 // sizes) in one launch.  Lanes claim global ray indices; a lane's item gives
 // its rays, tree, triangle records, root box (L.item_boxes, 8 floats per
 // item) and output, so the scene pointers are per lane.
+//
+// Plain launches compile for 5 waves per SIMD (96 VGPRs, a few values
+// spilled): config 3's rays in one launch ran 65.2 against 55.7 Mrays/s at 4
+// waves (116 VGPRs), one launch per batch 26.8 against 25.5; config 2's
+// coherent eye rays 161 against 165.  The batched launch keeps 4 waves (81.3
+// against 74.0 Mrays/s at 5).
 template <class Pred, bool ANY, int S, bool COUNT = false, bool BATCHED = false>
-__global__ __launch_bounds__(256) void kd_trace(TraceLaunch L) {
+__global__ __launch_bounds__(256, BATCHED ? 4 : 5) void kd_trace(TraceLaunch L) {
     __shared__ uint32_t lds_stack[256 / kWave][S * 3][kWave];
     const int lane = lane_id();
     const int wave = threadIdx.x / kWave;
