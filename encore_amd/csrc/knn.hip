@@ -1183,7 +1183,14 @@ void knn_cell_order(const KnnLaunch& L, uint32_t* order, void* scratch, size_t s
     size_t tmp_bytes = scratch_bytes - (size_t)(static_cast<char*>(tmp) - w);
     hipLaunchKernelGGL(dev::knn_cell_keys, dim3((nq + 255) / 256), dim3(256), 0, s, L, keys, ids);
     RQ_HIP(hipGetLastError());
-    RQ_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, skeys, ids, order, (int)nq, 0, 64, s));
+    // Cell coordinates are clamped into the grid (fewer than 2^b cells per
+    // axis), so Morton keys use the low 3b bits; a query that is not finite
+    // has the key ~0, whose low 3b bits are all set, and sorts last either
+    // way.  Sorting those bits only takes 3b / 8 passes rather than 8.
+    uint32_t b = 1;
+    const uint32_t widest = std::max(std::max(L.grid.dims[0], L.grid.dims[1]), L.grid.dims[2]);
+    while (b < 21 && (1u << b) < widest) ++b;
+    RQ_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, skeys, ids, order, (int)nq, 0, (int)(3 * b), s));
 }
 
 size_t knn_tree_order_bytes(uint32_t nq) {
